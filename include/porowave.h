@@ -1,0 +1,243 @@
+/*
+ * porowave.h -- C ABI of the MI355X-native 3-D Biot poroelastic staggered-grid
+ * finite-difference solver.
+ *
+ * The reference has no library interface: its operator surface is a monolithic
+ * main() (Float/main.cu:300-1477, Half/main.cu:376-1850) whose parameters are
+ * #defines and literals and whose outputs are .dat text files.  Each entry point
+ * below replaces one part of that surface; the comment on each names what it
+ * replaces.  Plain C types only: opaque handle, plain pointers and sizes, status
+ * codes, thread-local error text.
+ *
+ * Conventions
+ *  - Grid sizes nx, ny, nz are the TOTAL grid including the CPML pad (the
+ *    reference's NX_ext = NX + 2*NP).  Flat index = ix + iy*nx + iz*nx*ny, x fastest
+ *    (Float/main.cu:96).  The outermost layer of cells is never updated
+ *    (Float/main.cu:103, 232).
+ *  - Staggering: p and tau_ii at (i,j,k); vux/vwx at (i+1/2,j,k); vuy/vwy at
+ *    (i,j+1/2,k); vuz/vwz at (i,j,k+1/2); txy (i+1/2,j+1/2,k); txz (i+1/2,j,k+1/2);
+ *    tyz (i,j+1/2,k+1/2).
+ *  - All outputs are in physical units (the compensated-FP16 exponent and dynamic
+ *    scales are undone inside the library).
+ *  - The handle owns its device memory; the caller owns host buffers.  One handle
+ *    is not thread-safe; distinct handles may run concurrently on distinct streams.
+ *  - pw_run() is asynchronous on the handle's stream; pw_get_* synchronise.
+ */
+#ifndef POROWAVE_H
+#define POROWAVE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PW_ABI_VERSION 1
+
+typedef struct pw_solver pw_solver;
+
+typedef enum {
+    PW_OK = 0,
+    PW_ERR_INVALID = 1,       /* bad argument or configuration */
+    PW_ERR_HIP = 2,           /* HIP runtime error */
+    PW_ERR_OUT_OF_MEMORY = 3, /* device allocation refused */
+    PW_ERR_STATE = 4,         /* call out of order (e.g. run before set_medium) */
+    PW_ERR_UNSTABLE = 5,      /* Courant condition violated by the medium */
+    PW_ERR_UNSUPPORTED = 6,   /* valid request this build does not implement */
+    PW_ERR_IO = 7             /* file could not be written */
+} pw_status;
+
+/* Storage precision of wavefields and CPML memory variables.  Arithmetic is FP32
+ * except for PW_FP64, which is the FP64 oracle run (everything in double). */
+typedef enum {
+    PW_FP32 = 0,   /* Float/main.cu storage */
+    PW_COMP16 = 1, /* Half/main.cu: FP16 value + int8 residual, FP16 coefficients */
+    PW_BF16 = 2,   /* BF16 wavefields + memory variables, FP16 coefficients */
+    PW_FP64 = 3    /* FP64 oracle run */
+} pw_precision;
+
+/* Wavefield ids for receivers, snapshots and pw_get_field. */
+enum {
+    PW_VUX = 0, PW_VUY, PW_VUZ, PW_VWX, PW_VWY, PW_VWZ,
+    PW_TXX, PW_TYY, PW_TZZ, PW_TXY, PW_TXZ, PW_TYZ, PW_P,
+    PW_NFIELDS
+};
+
+/* Cell-centred base medium fields handed to pw_set_medium, in this order: what
+ * create_model (Float/main.cu:1590-1655) produces and the rest of the reference
+ * program consumes. */
+enum {
+    PW_RHO = 0, /* bulk density rho = (1-phi) rho_s + phi rho_f */
+    PW_RHOF,    /* fluid density rho_f */
+    PW_C1,      /* viscous coupling eta / kappa */
+    PW_C2,      /* inertial coupling (1 + 2/m) tau rho_f / phi */
+    PW_HH,      /* H = alpha^2 M + Kb + 4/3 mu */
+    PW_H2U,     /* H - 2 mu */
+    PW_CC,      /* C = alpha M */
+    PW_MM,      /* M (Biot modulus) */
+    PW_MU,      /* shear modulus */
+    PW_NBASE
+};
+
+typedef struct pw_config {
+    int nx, ny, nz;         /* total grid incl. CPML (NX_ext, NY_ext, NZ_ext) */
+    int np;                 /* CPML thickness in cells (NP) */
+    double h;               /* grid spacing [m] (H) */
+    double dt;              /* time step [s] (DT) */
+    int nt;                 /* steps in the shot: sizes the seismogram buffer and
+                               sets the comp16 dynamic-scale ramp (NT) */
+    int precision;          /* pw_precision */
+    double cpml_r;          /* CPML reflection coefficient (RC; reference 1e-6) */
+    double cpml_v0;         /* CPML velocity v0 (reference 4000 m/s) */
+    int half_es;            /* comp16/bf16: WAVEFIELD_ES_INT (reference 0) */
+    double half_dyn_end;    /* comp16/bf16: DYNAMIC_SCALE_END (reference 16) */
+    int half_dyn_interval;  /* comp16/bf16: DYNAMIC_SCALE_INTERVAL (reference 100) */
+    int device;             /* HIP device ordinal */
+    void* stream;           /* hipStream_t to run on; NULL: the library creates one */
+    int check_cfl;          /* 1: pw_set_medium refuses media violating the Courant
+                               limit (the reference only prints a warning,
+                               Float/main.cu:940-942) */
+} pw_config;
+
+/* Fills a config with the reference's values (Float/main.cu:20-23, 305-322):
+ * 192^3 total, NP 32, H 0.01 m, DT 0.9 us, NT 3500, RC 1e-6, v0 4000, FP32. */
+void pw_config_default(pw_config* cfg);
+
+/* Replaces main()'s setup: cudaSetDevice + the 93 cudaMalloc calls
+ * (Float/main.cu:814-935).  One pooled device allocation per handle. */
+pw_status pw_create(const pw_config* cfg, pw_solver** out);
+
+/* Replaces the frees at Float/main.cu:1225-1474. NULL is accepted. */
+void pw_destroy(pw_solver* s);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* pw_last_error(void);
+
+/* Replaces extmodel (Float/main.cu:387-399, 1662-1794), the staggered averaging
+ * (:473-513), the Half exponent/quantisation precompute (Half/main.cu:745-1045)
+ * and the 93 H2D copies (Float/main.cu:948-1040).  base: PW_NBASE pointers to
+ * cell-centred interior fields of nxi*nyi*nzi doubles (nxi = nx - 2*np, ...),
+ * x fastest.  The pad is filled by edge replication, staggered coefficients are
+ * computed in FP64 on the GPU and stored in the handle's precision.  Resets the
+ * wavefields to zero and the step counter to 0. */
+pw_status pw_set_medium(pw_solver* s, const double* const* base, int nxi, int nyi, int nzi);
+
+/* Reference Ricker source (Float/main.cu:310-313, 321-322, 1059-1060): wavelet
+ * -(1 - 2 pi^2 f0^2 tau^2) exp(-pi^2 f0^2 tau^2), tau = it*dt - t0, t0 = 1.2/f0,
+ * added to txx, tyy, tzz and p at cell (sx, sy, sz) with the given weights
+ * (reference: {1, 1, 1, 0}; NULL means that). */
+pw_status pw_set_source(pw_solver* s, int sx, int sy, int sz, double f0, const double* weights4);
+
+/* Replaces the receiver gather (Float/main.cu:1083-1093): record field[r] at
+ * flat index[r] after every step into a device buffer of [n][nt] floats. */
+pw_status pw_set_receivers(pw_solver* s, int n, const int* field, const long long* index);
+
+/* The reference's receivers: every depth iz of the vertical line through (sx, sy)
+ * for the 5 channels sisx/sisz/sisvu/sisvw/sisp = txx/tzz/vux/vwx/p
+ * (Float/main.cu:1085-1093).  Channel c, depth iz is receiver c*nz + iz. */
+pw_status pw_set_receivers_reference(pw_solver* s, int sx, int sy);
+
+/* Replaces the snapshot copies (Float/main.cu:1094-1163): after step step[k],
+ * copy the y = plane_y slice of field[k] ([nz][nx] floats). */
+pw_status pw_set_snapshots(pw_solver* s, int n, const int* step, const int* field, int plane_y);
+
+/* Replaces the time loop (Float/main.cu:1048-1164): enqueue nsteps steps
+ * (source -> velocity kernel -> stress kernel -> receivers/snapshots) on the
+ * handle's stream.  Asynchronous.  Steps continue from the current counter and
+ * may not exceed cfg.nt in total. */
+pw_status pw_run(pw_solver* s, int nsteps);
+
+/* Blocks until all work enqueued on the handle's stream has finished. */
+pw_status pw_synchronize(pw_solver* s);
+
+/* Steps completed (enqueued) so far. */
+int pw_current_step(const pw_solver* s);
+
+/* Zero the wavefields and memory variables and reset the step counter; keeps the
+ * medium, source, receivers and snapshots (a new shot on the same handle). */
+pw_status pw_reset(pw_solver* s);
+
+/* Copies out the seismograms: [nrec][nt] floats (steps not yet run are 0). */
+pw_status pw_get_seismogram(pw_solver* s, float* host);
+
+/* Copies out the snapshots: [nsnap][nz][nx] floats. */
+pw_status pw_get_snapshots(pw_solver* s, float* host);
+
+/* Copies out one full wavefield decoded to physical FP32: nz*ny*nx floats. */
+pw_status pw_get_field(pw_solver* s, int field, float* host);
+
+/* Replaces wfile / wfile3d (Float/main.cu:1501-1516, 1563-1588): writes nr rows
+ * of nc values as "%e " text, each row followed by "\n" and a blank line. */
+pw_status pw_write_dat(const char* path, const float* data, int nr, int nc);
+
+typedef struct pw_info {
+    int precision;
+    int em, epml;               /* comp16/bf16 exponents (Half/main.cu:793-843) */
+    double bytes_per_cell_step; /* algorithmic HBM bytes per cell-step, this grid */
+    double bytes_velocity;      /* ... of which the velocity kernel (per cell) */
+    double bytes_stress;        /* ... of which the stress kernel (per cell) */
+    double vp_max;              /* fastest P velocity of the medium [m/s] */
+    double courant;             /* vp_max * dt / h * sqrt(3) */
+    long long device_bytes;     /* device memory held by the handle */
+    int slab_lo[3], slab_hi[3]; /* CPML slab widths per axis */
+} pw_info;
+
+pw_status pw_get_info(pw_solver* s, pw_info* out);
+
+/* Per-kernel timing: with profiling enabled, pw_run brackets every velocity and
+ * stress launch with HIP events on the handle's stream.  pw_get_kernel_times
+ * synchronises, returns the summed milliseconds and launch count since the last
+ * call, and clears them.  No counterpart in the reference, which times only the
+ * whole loop (Float/main.cu:1043-1046, 1176-1184). */
+pw_status pw_set_profiling(pw_solver* s, int enable);
+pw_status pw_get_kernel_times(pw_solver* s, double* ms_velocity, double* ms_stress, long long* launches);
+
+/* ---------------------------------------------------------------------------
+ * Medium front-ends (host, FP64).  Each writes PW_NBASE arrays of nxi*nyi*nzi
+ * doubles into out (contiguous, field-major).
+ * ------------------------------------------------------------------------- */
+
+/* create_model (Float/main.cu:1590-1655): fluid borehole of radius 10 cells on
+ * the vertical axis through the interior centre, in the reference formation.
+ * Values are the reference's float results widened to double. */
+pw_status pw_medium_borehole(int nxi, int nyi, int nzi, double* out);
+
+typedef struct pw_rock {
+    double Ks, Kb, mu, Kf;    /* grain, dry-frame, shear, fluid moduli [Pa] */
+    double rhos, rhof;        /* grain and fluid densities [kg/m^3] */
+    double phi;               /* porosity */
+    double kappa, eta;        /* permeability [m^2], viscosity [Pa s] */
+    double tortuosity, pore_m;/* tortuosity, pore-shape factor m (reference 8) */
+} pw_rock;
+
+/* Biot coefficients of one rock (the formulas of Float/main.cu:1635-1650 with
+ * moduli given directly): out[PW_NBASE]. */
+pw_status pw_biot_from_rock(const pw_rock* r, double* out);
+
+#define PW_MAX_LAYERS 16
+typedef struct pw_layered {
+    int nlayers;                 /* 1..PW_MAX_LAYERS */
+    int z_top[PW_MAX_LAYERS];    /* first total-grid z index of each layer, ascending,
+                                    z_top[0] == 0 */
+    pw_rock rock[PW_MAX_LAYERS];
+    double perturb;              /* relative amplitude of the Kb/mu/phi perturbation
+                                    (0.05 = 5 %; 0 disables) */
+    double corr_cells;           /* Gaussian smoothing sigma in cells */
+    double clamp;                /* clamp of the unit-variance field (3) */
+    unsigned long long seed;
+} pw_layered;
+
+/* Horizontally layered medium with a smooth random perturbation, generated on
+ * the handle's GPU in FP64 and installed as with pw_set_medium.  The perturbation
+ * of Kb, mu and phi is white noise (splitmix64 counter hash, Box-Muller)
+ * smoothed by a separable Gaussian, scaled to unit variance, clamped and applied
+ * as a factor (1 + perturb * n). */
+pw_status pw_set_medium_layered(pw_solver* s, const pw_layered* spec);
+
+/* Copies the cell-centred base fields installed by the last pw_set_medium* call:
+ * PW_NBASE arrays of nxi*nyi*nzi doubles (for feeding the same medium to other
+ * tools). */
+pw_status pw_get_base_fields(pw_solver* s, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,326 @@
+"""porowave: MI355X-native 3-D Biot poroelastic staggered-grid FD solver.
+
+Python mirror of the C ABI in include/porowave.h (ctypes over the in-tree
+libporowave.so).  The operator surface follows the reference programs
+(Float/main.cu, Half/main.cu): the same medium / source / receiver parameters,
+the same seismogram channels (sisx/sisz/sisvu/sisvw/sisp) and snapshot slices,
+and a .dat writer in the reference's text format.  There is no CPU fallback:
+every call runs the HIP kernels, and importing the library raises if the built
+extension is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libporowave.so")
+
+FIELDS = ["vux", "vuy", "vuz", "vwx", "vwy", "vwz", "txx", "tyy", "tzz", "txy", "txz", "tyz", "p"]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+BASE = ["rho", "rhof", "C1", "C2", "HH", "H2u", "C", "M", "mu"]
+PRECISIONS = {"fp32": 0, "comp16": 1, "bf16": 2, "fp64": 3}
+# The reference's seismogram channels (Float/main.cu:1085-1093, 1213-1222).
+REFERENCE_CHANNELS = [("sisx", "txx"), ("sisz", "tzz"), ("sisvu", "vux"), ("sisvw", "vwx"), ("sisp", "p")]
+STATUS = {0: "PW_OK", 1: "PW_ERR_INVALID", 2: "PW_ERR_HIP", 3: "PW_ERR_OUT_OF_MEMORY", 4: "PW_ERR_STATE",
+          5: "PW_ERR_UNSTABLE", 6: "PW_ERR_UNSUPPORTED", 7: "PW_ERR_IO"}
+MAX_LAYERS = 16
+
+
+class PorowaveError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("np", C.c_int),
+                ("h", C.c_double), ("dt", C.c_double), ("nt", C.c_int), ("precision", C.c_int),
+                ("cpml_r", C.c_double), ("cpml_v0", C.c_double),
+                ("half_es", C.c_int), ("half_dyn_end", C.c_double), ("half_dyn_interval", C.c_int),
+                ("device", C.c_int), ("stream", C.c_void_p), ("check_cfl", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [("precision", C.c_int), ("em", C.c_int), ("epml", C.c_int),
+                ("bytes_per_cell_step", C.c_double), ("bytes_velocity", C.c_double), ("bytes_stress", C.c_double),
+                ("vp_max", C.c_double), ("courant", C.c_double),
+                ("device_bytes", C.c_longlong), ("slab_lo", C.c_int * 3), ("slab_hi", C.c_int * 3)]
+
+
+class Rock(C.Structure):
+    _fields_ = [("Ks", C.c_double), ("Kb", C.c_double), ("mu", C.c_double), ("Kf", C.c_double),
+                ("rhos", C.c_double), ("rhof", C.c_double), ("phi", C.c_double),
+                ("kappa", C.c_double), ("eta", C.c_double), ("tortuosity", C.c_double), ("pore_m", C.c_double)]
+
+
+class Layered(C.Structure):
+    _fields_ = [("nlayers", C.c_int), ("z_top", C.c_int * MAX_LAYERS), ("rock", Rock * MAX_LAYERS),
+                ("perturb", C.c_double), ("corr_cells", C.c_double), ("clamp", C.c_double),
+                ("seed", C.c_ulonglong)]
+
+
+# symbol -> (restype, argtypes); the same list the header declares.
+_SIGS = {
+    "pw_config_default": (None, [C.POINTER(Config)]),
+    "pw_create": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "pw_destroy": (None, [C.c_void_p]),
+    "pw_last_error": (C.c_char_p, []),
+    "pw_set_medium": (C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.c_int, C.c_int, C.c_int]),
+    "pw_set_source": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]),
+    "pw_set_receivers": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
+    "pw_set_receivers_reference": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "pw_set_snapshots": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
+    "pw_run": (C.c_int, [C.c_void_p, C.c_int]),
+    "pw_synchronize": (C.c_int, [C.c_void_p]),
+    "pw_current_step": (C.c_int, [C.c_void_p]),
+    "pw_reset": (C.c_int, [C.c_void_p]),
+    "pw_get_seismogram": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pw_get_snapshots": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pw_get_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float)]),
+    "pw_write_dat": (C.c_int, [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int]),
+    "pw_get_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
+    "pw_medium_borehole": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "pw_biot_from_rock": (C.c_int, [C.POINTER(Rock), C.POINTER(C.c_double)]),
+    "pw_set_medium_layered": (C.c_int, [C.c_void_p, C.POINTER(Layered)]),
+    "pw_get_base_fields": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "pw_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "pw_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_longlong)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libporowave.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"porowave HIP library not built: {LIB_PATH} "
+                              "(run python -m mixedprecisionfaracoustic_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise PorowaveError(status, lib().pw_last_error().decode(errors="replace"))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def medium_borehole(nxi, nyi, nzi):
+    """create_model (Float/main.cu:1590-1655): base fields [9, nzi, nyi, nxi] float64."""
+    out = np.zeros((9, nzi, nyi, nxi), np.float64)
+    _check(lib().pw_medium_borehole(nxi, nyi, nzi, _dp(out)))
+    return out
+
+
+def rock(**kw):
+    r = Rock()
+    defaults = {"tortuosity": 1.0, "pore_m": 8.0}
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(r, k, float(v))
+    return r
+
+
+def biot_from_rock(r):
+    out = np.zeros(9, np.float64)
+    _check(lib().pw_biot_from_rock(C.byref(r), _dp(out)))
+    return out
+
+
+def layered(z_tops, rocks, perturb=0.0, corr_cells=8.0, clamp=3.0, seed=0):
+    L = Layered()
+    L.nlayers = len(rocks)
+    for k, (z, r) in enumerate(zip(z_tops, rocks)):
+        L.z_top[k] = int(z)
+        L.rock[k] = r
+    L.perturb, L.corr_cells, L.clamp, L.seed = float(perturb), float(corr_cells), float(clamp), int(seed)
+    return L
+
+
+def write_dat(path, data):
+    """wfile / wfile3d text format (Float/main.cu:1501-1516, 1563-1588)."""
+    a = np.ascontiguousarray(data, np.float32)
+    if a.ndim != 2:
+        raise ValueError("write_dat expects a 2-D array")
+    _check(lib().pw_write_dat(os.fsencode(path), _fp(a), a.shape[0], a.shape[1]))
+
+
+class Solver:
+    """One shot on one GPU (a pw_solver handle).
+
+    nx, ny, nz are the total grid including the CPML (the reference's NX_ext).
+    precision: "fp32" (Float path), "comp16" (Half path: FP16 + int8 residual),
+    "bf16", or "fp64" (the FP64 oracle run).
+    """
+
+    def __init__(self, nx, ny, nz, np_, h, dt, nt, precision="fp32", cpml_r=1e-6, cpml_v0=4000.0,
+                 device=0, check_cfl=True, half_es=0, half_dyn_end=16.0, half_dyn_interval=100, stream=None):
+        L = lib()
+        cfg = Config()
+        L.pw_config_default(C.byref(cfg))
+        cfg.nx, cfg.ny, cfg.nz, cfg.np = int(nx), int(ny), int(nz), int(np_)
+        cfg.h, cfg.dt, cfg.nt = float(h), float(dt), int(nt)
+        cfg.precision = PRECISIONS[precision]
+        cfg.cpml_r, cfg.cpml_v0 = float(cpml_r), float(cpml_v0)
+        cfg.half_es, cfg.half_dyn_end, cfg.half_dyn_interval = int(half_es), float(half_dyn_end), int(half_dyn_interval)
+        cfg.device = int(device)
+        cfg.stream = stream
+        cfg.check_cfl = 1 if check_cfl else 0
+        h_ = C.c_void_p()
+        _check(L.pw_create(C.byref(cfg), C.byref(h_)))
+        self._h = h_
+        self.cfg = cfg
+        self.precision = precision
+        self.nrec = 0
+        self.nsnap = 0
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.cfg.nz, self.cfg.ny, self.cfg.nx)
+
+    @property
+    def interior(self):
+        c = self.cfg
+        return (c.nz - 2 * c.np, c.ny - 2 * c.np, c.nx - 2 * c.np)
+
+    # -- set-up -----------------------------------------------------------
+    def set_medium(self, base):
+        nzi, nyi, nxi = self.interior
+        b = np.ascontiguousarray(base, np.float64)
+        if b.shape != (9, nzi, nyi, nxi):
+            raise ValueError(f"base must be [9, {nzi}, {nyi}, {nxi}], got {b.shape}")
+        ptrs = (C.POINTER(C.c_double) * 9)(*[_dp(b[k]) for k in range(9)])
+        _check(lib().pw_set_medium(self._h, ptrs, nxi, nyi, nzi))
+
+    def set_medium_layered(self, spec):
+        _check(lib().pw_set_medium_layered(self._h, C.byref(spec)))
+
+    def base_fields(self):
+        nzi, nyi, nxi = self.interior
+        out = np.zeros((9, nzi, nyi, nxi), np.float64)
+        _check(lib().pw_get_base_fields(self._h, _dp(out)))
+        return out
+
+    def set_source(self, sx, sy, sz, f0, weights=(1.0, 1.0, 1.0, 0.0)):
+        w = np.ascontiguousarray(weights, np.float64)
+        _check(lib().pw_set_source(self._h, int(sx), int(sy), int(sz), float(f0), _dp(w)))
+
+    def set_receivers(self, fields, indices):
+        f = np.ascontiguousarray([FIELD_ID[x] if isinstance(x, str) else int(x) for x in fields], np.int32)
+        i = np.ascontiguousarray(indices, np.int64)
+        if f.shape != i.shape:
+            raise ValueError("fields and indices must have the same length")
+        _check(lib().pw_set_receivers(self._h, len(f), f.ctypes.data_as(C.POINTER(C.c_int)),
+                                      i.ctypes.data_as(C.POINTER(C.c_longlong))))
+        self.nrec = len(f)
+
+    def set_receivers_reference(self, sx, sy):
+        _check(lib().pw_set_receivers_reference(self._h, int(sx), int(sy)))
+        self.nrec = 5 * self.cfg.nz
+
+    def set_snapshots(self, steps, fields, plane_y=None):
+        s = np.ascontiguousarray(steps, np.int32)
+        f = np.ascontiguousarray([FIELD_ID[x] if isinstance(x, str) else int(x) for x in fields], np.int32)
+        y = self.cfg.ny // 2 if plane_y is None else int(plane_y)
+        _check(lib().pw_set_snapshots(self._h, len(s), s.ctypes.data_as(C.POINTER(C.c_int)),
+                                      f.ctypes.data_as(C.POINTER(C.c_int)), y))
+        self.nsnap = len(s)
+
+    # -- stepping -----------------------------------------------------------
+    def run(self, nsteps):
+        _check(lib().pw_run(self._h, int(nsteps)))
+
+    def synchronize(self):
+        _check(lib().pw_synchronize(self._h))
+
+    def reset(self):
+        _check(lib().pw_reset(self._h))
+
+    @property
+    def step(self):
+        return lib().pw_current_step(self._h)
+
+    # -- outputs ------------------------------------------------------------
+    def seismogram(self):
+        out = np.zeros((max(self.nrec, 1), self.cfg.nt), np.float32)
+        _check(lib().pw_get_seismogram(self._h, _fp(out)))
+        return out[:self.nrec]
+
+    def reference_seismograms(self):
+        """Split the reference receiver set into {sisx, sisz, sisvu, sisvw, sisp}: [nz, nt] each."""
+        s = self.seismogram()
+        nz = self.cfg.nz
+        return {name: s[k * nz:(k + 1) * nz] for k, (name, _) in enumerate(REFERENCE_CHANNELS)}
+
+    def snapshots(self):
+        out = np.zeros((max(self.nsnap, 1), self.cfg.nz, self.cfg.nx), np.float32)
+        _check(lib().pw_get_snapshots(self._h, _fp(out)))
+        return out[:self.nsnap]
+
+    def field(self, name):
+        out = np.zeros(self.shape, np.float32)
+        _check(lib().pw_get_field(self._h, FIELD_ID[name] if isinstance(name, str) else int(name), _fp(out)))
+        return out
+
+    def set_profiling(self, enable=True):
+        _check(lib().pw_set_profiling(self._h, 1 if enable else 0))
+
+    def kernel_times(self):
+        """(ms_velocity_total, ms_stress_total, launches) since the last call."""
+        v, t, n = C.c_double(), C.c_double(), C.c_longlong()
+        _check(lib().pw_get_kernel_times(self._h, C.byref(v), C.byref(t), C.byref(n)))
+        return v.value, t.value, n.value
+
+    def info(self):
+        i = Info()
+        _check(lib().pw_get_info(self._h, C.byref(i)))
+        return {"precision": self.precision, "em": i.em, "epml": i.epml,
+                "bytes_per_cell_step": i.bytes_per_cell_step, "bytes_velocity": i.bytes_velocity,
+                "bytes_stress": i.bytes_stress, "vp_max": i.vp_max, "courant": i.courant,
+                "device_bytes": i.device_bytes, "slab_lo": list(i.slab_lo), "slab_hi": list(i.slab_hi)}
+
+
+def reference_solver(precision="fp32", nt=3500, **kw):
+    """The reference configuration (Float/main.cu:20-23, 305-322, 1590-1655):
+    192^3 total (128^3 + 32-cell PML), H = 1 cm, DT = 0.9 us, 3 kHz Ricker at
+    (96, 96, 52), borehole medium, receivers on the vertical line through the source."""
+    n = 128 + 64
+    s = Solver(n, n, n, 32, 0.01, 0.9e-6, nt, precision=precision, cpml_r=1e-6, cpml_v0=4000.0, **kw)
+    s.set_medium(medium_borehole(128, 128, 128))
+    s.set_source(n // 2, n // 2, 20 + 32, 3000.0, (1.0, 1.0, 1.0, 0.0))
+    s.set_receivers_reference(n // 2, n // 2)
+    return s

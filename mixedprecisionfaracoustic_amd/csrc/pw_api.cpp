@@ -1,0 +1,785 @@
+// pw_api.cpp -- the C ABI (include/porowave.h): handle lifetime, medium
+// installation, source/receiver/snapshot set-up and the step loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pw_device.h"
+
+namespace pw {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+pw_status fail(pw_status st, const std::string& msg)
+{
+    set_error(msg);
+    return st;
+}
+
+}  // namespace pw
+
+using namespace pw;
+
+struct pw_solver {
+    pw_config cfg;
+    int prec = PW_FP32;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    DevBuffers b{};
+    void* pool = nullptr;
+    size_t pool_bytes = 0;
+    size_t state_bytes_total = 0;  // bytes of the state + memory-variable region (zeroed on reset)
+    // medium
+    bool has_medium = false;
+    int em = -23, epml = 0;
+    float inv_scale_pml = 1.0f;
+    double vp_max = 0.0, courant = 0.0;
+    bool layered = false;
+    pw_layered spec{};
+    // comp16 / bf16 dynamic scale (Half/main.cu:800-807, 1326-1364)
+    float dyn_scale = 1.0f, stress_base = 1.0f, vel_base = 1.0f, source_scale = 1.0f;
+    // source
+    bool has_source = false;
+    unsigned src_off = 0;
+    double f0 = 0.0, w[4] = {1, 1, 1, 0};
+    // receivers
+    int nrec = 0;
+    int* d_rfield = nullptr;
+    unsigned* d_roff = nullptr;
+    float* d_seis = nullptr;
+    // snapshots
+    int nsnap = 0, snap_y = 0;
+    std::vector<int> snap_step, snap_field;
+    float* d_snaps = nullptr;
+    int step = 0;
+    // per-kernel timing (pw_set_profiling)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;     // created lazily, reused
+    std::vector<hipEvent_t> ev_pending;  // 4 per profiled step: v0, v1, t0, t1
+};
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+bool env_int(const char* name, int* v)
+{
+    const char* s = std::getenv(name);
+    if (!s || !*s) return false;
+    *v = std::atoi(s);
+    return true;
+}
+
+// CPML tables in the form the kernels consume: per index {c_full, e_full,
+// c_half, e_half} with c = dt*d/2.
+void build_tables(const pw_config& c, int prec, int n, std::vector<double>& out4, int* epml_out, float* isp_out,
+                  float* pml_max_out)
+{
+    out4.assign((size_t)n * 4, 0.0);
+    if (prec == PW_FP64) {
+        std::vector<double> d(n), d2(n), e(n), e2(n);
+        cpml_profile_f64(n, c.np, c.h, c.dt, c.cpml_r, c.cpml_v0, d.data(), d2.data(), e.data(), e2.data());
+        for (int i = 0; i < n; ++i) {
+            out4[4 * i + 0] = c.dt * d[i] * 0.5;
+            out4[4 * i + 1] = e[i];
+            out4[4 * i + 2] = c.dt * d2[i] * 0.5;
+            out4[4 * i + 3] = e2[i];
+        }
+        return;
+    }
+    std::vector<float> d(n), d2(n), e(n), e2(n);
+    const float DT = (float)c.dt;
+    cpml_profile_reference_f32(n, c.np, (float)c.h, DT, (float)c.cpml_r, (float)c.cpml_v0, d.data(), d2.data(),
+                               e.data(), e2.data());
+    if (prec == PW_FP32) {
+        for (int i = 0; i < n; ++i) {
+            out4[4 * i + 0] = (float)(DT * d[i]) * 0.5f;
+            out4[4 * i + 1] = e[i];
+            out4[4 * i + 2] = (float)(DT * d2[i]) * 0.5f;
+            out4[4 * i + 3] = e2[i];
+        }
+        return;
+    }
+    // Scaled forms: the FP16 tables of Half/main.cu:1031-1042 decoded as
+    // pml_dt_d / pml_e do (Half/main.cu:124-132).  epml comes from all three axes
+    // (pml_max is passed in by the caller after a first pass).
+    const float scale_pml = std::ldexp(1.0f, *epml_out), isp = std::ldexp(1.0f, -*epml_out);
+    float pmax = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        pmax = std::max(pmax, std::max(std::fabs(DT * d[i]), std::fabs(DT * d2[i])));
+        pmax = std::max(pmax, std::max(std::fabs(1.0f - e[i]), std::fabs(1.0f - e2[i])));
+        const _Float16 hd = (_Float16)(scale_pml * DT * d[i]), hd2 = (_Float16)(scale_pml * DT * d2[i]);
+        const _Float16 he = (_Float16)(scale_pml * (1.0f - e[i])), he2 = (_Float16)(scale_pml * (1.0f - e2[i]));
+        out4[4 * i + 0] = (isp * (float)hd) * 0.5f;
+        out4[4 * i + 1] = 1.0f - isp * (float)he;
+        out4[4 * i + 2] = (isp * (float)hd2) * 0.5f;
+        out4[4 * i + 3] = 1.0f - isp * (float)he2;
+    }
+    *isp_out = isp;
+    *pml_max_out = pmax;
+}
+
+void slab_widths(const std::vector<double>& t, int n, int* lo, int* hi)
+{
+    *lo = 0;
+    *hi = 0;
+    for (int i = 0; i < n / 2; ++i)
+        if (t[4 * i] != 0.0 || t[4 * i + 2] != 0.0) *lo = i + 1;
+    for (int i = n - 1; i >= n / 2; --i)
+        if (t[4 * i] != 0.0 || t[4 * i + 2] != 0.0) *hi = n - i;
+}
+
+hipError_t zero_state(pw_solver* s) { return launch_zero(s->pool, s->state_bytes_total, s->stream); }
+
+pw_status check_handle(pw_solver* s, const char* fn)
+{
+    if (!s) return fail(PW_ERR_INVALID, std::string(fn) + ": null handle");
+    return PW_OK;
+}
+
+}  // namespace
+
+extern "C" void pw_config_default(pw_config* c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->nx = c->ny = c->nz = 128 + 2 * 32;
+    c->np = 32;
+    c->h = 0.01;
+    c->dt = 0.9e-6;
+    c->nt = 3500;
+    c->precision = PW_FP32;
+    c->cpml_r = 1e-6;
+    c->cpml_v0 = 4000.0;
+    c->half_es = 0;
+    c->half_dyn_end = 16.0;
+    c->half_dyn_interval = 100;
+    c->device = 0;
+    c->stream = nullptr;
+    c->check_cfl = 1;
+}
+
+extern "C" const char* pw_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
+{
+    if (!cfg || !out) return fail(PW_ERR_INVALID, "pw_create: null argument");
+    *out = nullptr;
+    const pw_config& c = *cfg;
+    if (c.precision < PW_FP32 || c.precision > PW_FP64) return fail(PW_ERR_INVALID, "pw_create: unknown precision");
+    if (c.np < 1 || c.nx < 2 * c.np + 3 || c.ny < 2 * c.np + 3 || c.nz < 2 * c.np + 3)
+        return fail(PW_ERR_INVALID, "pw_create: grid must be at least 2*np+3 cells per axis");
+    if (!(c.h > 0) || !(c.dt > 0) || c.nt < 1) return fail(PW_ERR_INVALID, "pw_create: h, dt and nt must be positive");
+    if (!(c.cpml_r > 0 && c.cpml_r < 1) || !(c.cpml_v0 > 0))
+        return fail(PW_ERR_INVALID, "pw_create: cpml_r must be in (0,1) and cpml_v0 positive");
+    const int pitch = (c.nx + 63) / 64 * 64;
+    const unsigned long long cells = (unsigned long long)pitch * c.ny * c.nz;
+    if (cells >= (1ull << 32) - (1ull << 20))
+        return fail(PW_ERR_UNSUPPORTED, "pw_create: grid exceeds 2^32 padded cells (32-bit offsets)");
+
+    int ndev = 0;
+    PW_HIP_TRY(hipGetDeviceCount(&ndev));
+    if (c.device < 0 || c.device >= ndev) return fail(PW_ERR_INVALID, "pw_create: no such device");
+    PW_HIP_TRY(hipSetDevice(c.device));
+
+    pw_solver* s = new pw_solver();
+    s->cfg = c;
+    s->prec = c.precision;
+    Geom& g = s->b.g;
+    g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
+    g.pitch = pitch;
+    g.plane = (long long)pitch * c.ny;
+    g.cells = (long long)cells;
+
+    // CPML tables (scaled forms need epml over all three axes first).
+    const int n3[3] = {c.nx, c.ny, c.nz};
+    std::vector<double> tab[3];
+    int epml = 0;
+    float isp = 1.0f;
+    if (is_scaled(s->prec)) {
+        float pmax = 0.0f, pm;
+        for (int ax = 0; ax < 3; ++ax) {
+            build_tables(c, s->prec, n3[ax], tab[ax], &epml, &isp, &pm);
+            pmax = std::max(pmax, pm);
+        }
+        // Half/main.cu:833-839
+        if (pmax > 0.0f) {
+            epml = (int)std::floor(std::log2(2048.0f / pmax));
+            epml = std::min(20, std::max(-20, epml));
+        }
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+        float pm;
+        build_tables(c, s->prec, n3[ax], tab[ax], &epml, &isp, &pm);
+        slab_widths(tab[ax], n3[ax], &g.lo[ax], &g.hi[ax]);
+    }
+    s->epml = epml;
+    s->inv_scale_pml = isp;
+    const long long W = pitch;
+    g.slab_cells[0] = (long long)c.nz * c.ny * (g.lo[0] + g.hi[0]);
+    g.slab_cells[1] = (long long)c.nz * (g.lo[1] + g.hi[1]) * W;
+    g.slab_cells[2] = (long long)(g.lo[2] + g.hi[2]) * c.ny * W;
+
+    // One pooled allocation: [state | memory variables] (zeroed together) then
+    // [coefficients | tables].
+    const size_t sb = (size_t)state_bytes(s->prec), cb = (size_t)coef_bytes(s->prec), mb = (size_t)mem_bytes(s->prec);
+    const size_t rb = s->prec == PW_FP64 ? 8 : 4;
+    std::vector<size_t> off;
+    size_t total = 0;
+    auto take = [&](size_t bytes) {
+        off.push_back(total);
+        total += align256(bytes);
+    };
+    const bool c16 = s->prec == PW_COMP16;
+    for (int k = 0; k < PW_NFIELDS; ++k) take(cells * (c16 ? 2 : sb));
+    if (c16)
+        for (int k = 0; k < PW_NFIELDS; ++k) take(cells * 1);
+    for (int ax = 0; ax < 3; ++ax)
+        for (int k = 0; k < kMemPerAxis; ++k) take((size_t)g.slab_cells[ax] * mb);
+    const size_t zero_bytes = total;
+    const int nc = ncoef(s->prec);
+    for (int k = 0; k < nc; ++k) take(cells * cb);
+    for (int ax = 0; ax < 3; ++ax) take((size_t)n3[ax] * 4 * rb);
+
+    hipError_t e = hipMalloc(&s->pool, total);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(e == hipErrorOutOfMemory ? PW_ERR_OUT_OF_MEMORY : PW_ERR_HIP,
+                    "pw_create: device allocation of " + std::to_string(total) + " bytes failed: " +
+                        hipGetErrorString(e));
+    }
+    s->pool_bytes = total;
+    s->state_bytes_total = zero_bytes;
+    char* base = (char*)s->pool;
+    size_t i = 0;
+    for (int k = 0; k < PW_NFIELDS; ++k) s->b.f[k] = base + off[i++];
+    for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = c16 ? (int8_t*)(base + off[i++]) : nullptr;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int k = 0; k < kMemPerAxis; ++k) s->b.mem[ax][k] = base + off[i++];
+    for (int k = 0; k < kMaxCoef; ++k) s->b.coef[k] = k < nc ? base + off[i++] : nullptr;
+    for (int ax = 0; ax < 3; ++ax) s->b.tab[ax] = base + off[i++];
+    s->b.prec = s->prec;
+    s->b.dt = c.dt;
+    s->b.h = c.h;
+    int zc = c.nz >= 256 ? 16 : 4;
+    env_int("PW_ZC", &zc);
+    s->b.zc = std::max(1, zc);
+
+    if (c.stream) {
+        s->stream = (hipStream_t)c.stream;
+    } else {
+        e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+ (void)hipFree(s->pool);
+            delete s;
+            return fail(PW_ERR_HIP, std::string("pw_create: stream: ") + hipGetErrorString(e));
+        }
+        s->own_stream = true;
+    }
+    // Upload tables in the kernels' real type.
+    for (int ax = 0; ax < 3; ++ax) {
+        if (s->prec == PW_FP64) {
+            e = hipMemcpyAsync(s->b.tab[ax], tab[ax].data(), tab[ax].size() * 8, hipMemcpyHostToDevice, s->stream);
+        } else {
+            std::vector<float> f(tab[ax].begin(), tab[ax].end());
+            e = hipMemcpyAsync(s->b.tab[ax], f.data(), f.size() * 4, hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        }
+        if (e != hipSuccess) {
+            pw_destroy(s);
+            return fail(PW_ERR_HIP, std::string("pw_create: table upload: ") + hipGetErrorString(e));
+        }
+    }
+    e = zero_state(s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        pw_destroy(s);
+        return fail(PW_ERR_HIP, std::string("pw_create: zero state: ") + hipGetErrorString(e));
+    }
+    // default source: the reference's cell and wavelet (Float/main.cu:310-313, 321)
+    {
+        double w4[4] = {1, 1, 1, 0};
+        const int sz = std::min(20 + c.np, c.nz - 2);
+        pw_set_source(s, c.nx / 2, c.ny / 2, sz, 3000.0, w4);
+    }
+    *out = s;
+    return PW_OK;
+}
+
+extern "C" void pw_destroy(pw_solver* s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->cfg.device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+ (void)hipFree(s->d_rfield);
+ (void)hipFree(s->d_roff);
+ (void)hipFree(s->d_seis);
+ (void)hipFree(s->d_snaps);
+ (void)hipFree(s->pool);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+namespace {
+
+// Installs a medium given device-resident base fields (interior, FP64).
+pw_status install_medium(pw_solver* s, const double* d_base, int nxi, int nyi, int nzi)
+{
+    const pw_config& c = s->cfg;
+    MediumArgs m;
+    const size_t ni = (size_t)nxi * nyi * nzi;
+    for (int k = 0; k < PW_NBASE; ++k) m.base[k] = d_base + k * ni;
+    m.nxi = nxi; m.nyi = nyi; m.nzi = nzi; m.np = c.np;
+    m.nx = c.nx; m.ny = c.ny; m.nz = c.nz;
+    m.dt = c.dt;
+    double* d_res = nullptr;
+    PW_HIP_TRY(hipMalloc(&d_res, 4 * sizeof(double)));
+    double res[3] = {0, 0, 0};
+    hipError_t e = medium_reduce(m, d_res, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(res, d_res, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+ (void)hipFree(d_res);
+    if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_set_medium: reduction: ") + hipGetErrorString(e));
+    if (!(res[0] > 0) || !std::isfinite(res[0]) || !std::isfinite(res[1]) || !std::isfinite(res[2]))
+        return fail(PW_ERR_INVALID, "pw_set_medium: medium has no positive finite modulus");
+    s->vp_max = std::sqrt(res[2]);
+    s->courant = s->vp_max * c.dt / c.h * std::sqrt(3.0);
+    if (c.check_cfl && !(s->courant <= 1.0))
+        return fail(PW_ERR_UNSTABLE, "pw_set_medium: Courant number " + std::to_string(s->courant) +
+                                         " > 1 (vp_max " + std::to_string(s->vp_max) + " m/s)");
+    // Half exponent, Half/main.cu:793-807 (float arithmetic as there)
+    double sm = 1.0, ism = 1.0;
+    if (is_scaled(s->prec)) {
+        int em = -23;
+        const float Kmax = (float)res[0], Rmax = (float)res[1], DT = (float)c.dt;
+        if (Kmax > 0.0f && Rmax > 0.0f) {
+            const float er = 1.0f / std::sqrt(DT * DT * Kmax * Rmax);
+            em = (int)std::lrint(-std::log2(er * DT * Kmax));
+        }
+        s->em = em;
+        sm = std::ldexp(1.0, em);
+        ism = std::ldexp(1.0, -em);
+        s->source_scale = std::ldexp(1.0f, c.half_es);
+        s->stress_base = std::ldexp(1.0f, -c.half_es);
+        s->vel_base = std::ldexp(1.0f, em - c.half_es);
+    }
+    e = medium_coefficients(m, s->b, sm, ism, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_set_medium: coefficients: ") + hipGetErrorString(e));
+    s->has_medium = true;
+    return pw_reset(s);
+}
+
+}  // namespace
+
+extern "C" pw_status pw_set_medium(pw_solver* s, const double* const* base, int nxi, int nyi, int nzi)
+{
+    pw_status st = check_handle(s, "pw_set_medium");
+    if (st != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    if (!base) return fail(PW_ERR_INVALID, "pw_set_medium: null base");
+    if (nxi != c.nx - 2 * c.np || nyi != c.ny - 2 * c.np || nzi != c.nz - 2 * c.np)
+        return fail(PW_ERR_INVALID, "pw_set_medium: interior size must be (nx-2np, ny-2np, nz-2np)");
+    for (int k = 0; k < PW_NBASE; ++k)
+        if (!base[k]) return fail(PW_ERR_INVALID, "pw_set_medium: null base field");
+    PW_HIP_TRY(hipSetDevice(c.device));
+    const size_t ni = (size_t)nxi * nyi * nzi;
+    double* d_base = nullptr;
+    PW_HIP_TRY(hipMalloc(&d_base, PW_NBASE * ni * sizeof(double)));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < PW_NBASE && e == hipSuccess; ++k)
+        e = hipMemcpyAsync(d_base + k * ni, base[k], ni * sizeof(double), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+ (void)hipFree(d_base);
+        return fail(PW_ERR_HIP, std::string("pw_set_medium: upload: ") + hipGetErrorString(e));
+    }
+    st = install_medium(s, d_base, nxi, nyi, nzi);
+ (void)hipFree(d_base);
+    if (st == PW_OK) s->layered = false;
+    return st;
+}
+
+namespace {
+
+pw_status generate_into(pw_solver* s, const pw_layered* spec, double** d_base_out)
+{
+    const pw_config& c = s->cfg;
+    const int nxi = c.nx - 2 * c.np, nyi = c.ny - 2 * c.np, nzi = c.nz - 2 * c.np;
+    const size_t ni = (size_t)nxi * nyi * nzi;
+    double *d_base = nullptr, *d_work = nullptr, *d_part = nullptr;
+    PW_HIP_TRY(hipMalloc(&d_base, PW_NBASE * ni * sizeof(double)));
+    hipError_t e = hipMalloc(&d_work, (spec->perturb != 0.0 ? 4 : 3) * ni * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&d_part, 2 * 4096 * sizeof(double));
+    if (e == hipSuccess) e = generate_layered(*spec, nxi, nyi, nzi, c.np, d_base, d_work, d_part, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+ (void)hipFree(d_work);
+ (void)hipFree(d_part);
+    if (e != hipSuccess) {
+ (void)hipFree(d_base);
+        return fail(e == hipErrorOutOfMemory ? PW_ERR_OUT_OF_MEMORY : PW_ERR_HIP,
+                    std::string("pw_set_medium_layered: generator: ") + hipGetErrorString(e));
+    }
+    *d_base_out = d_base;
+    return PW_OK;
+}
+
+pw_status check_layered(const pw_solver* s, const pw_layered* L)
+{
+    if (!L) return fail(PW_ERR_INVALID, "pw_set_medium_layered: null spec");
+    if (L->nlayers < 1 || L->nlayers > PW_MAX_LAYERS) return fail(PW_ERR_INVALID, "pw_set_medium_layered: nlayers");
+    if (L->z_top[0] != 0) return fail(PW_ERR_INVALID, "pw_set_medium_layered: z_top[0] must be 0");
+    for (int k = 1; k < L->nlayers; ++k)
+        if (L->z_top[k] <= L->z_top[k - 1]) return fail(PW_ERR_INVALID, "pw_set_medium_layered: z_top not ascending");
+    for (int k = 0; k < L->nlayers; ++k) {
+        double out[PW_NBASE];
+        if (pw_biot_from_rock(&L->rock[k], out) != PW_OK) return PW_ERR_INVALID;
+    }
+    if (L->perturb < 0 || L->perturb * (L->clamp > 0 ? L->clamp : 3.0) >= 1.0)
+        return fail(PW_ERR_INVALID, "pw_set_medium_layered: perturb*clamp must be in [0, 1)");
+    (void)s;
+    return PW_OK;
+}
+
+}  // namespace
+
+extern "C" pw_status pw_set_medium_layered(pw_solver* s, const pw_layered* spec)
+{
+    pw_status st = check_handle(s, "pw_set_medium_layered");
+    if (st != PW_OK) return st;
+    if ((st = check_layered(s, spec)) != PW_OK) return st;
+    PW_HIP_TRY(hipSetDevice(s->cfg.device));
+    double* d_base = nullptr;
+    if ((st = generate_into(s, spec, &d_base)) != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    st = install_medium(s, d_base, c.nx - 2 * c.np, c.ny - 2 * c.np, c.nz - 2 * c.np);
+ (void)hipFree(d_base);
+    if (st == PW_OK) {
+        s->layered = true;
+        s->spec = *spec;
+    }
+    return st;
+}
+
+extern "C" pw_status pw_get_base_fields(pw_solver* s, double* out)
+{
+    pw_status st = check_handle(s, "pw_get_base_fields");
+    if (st != PW_OK) return st;
+    if (!out) return fail(PW_ERR_INVALID, "pw_get_base_fields: null output");
+    if (!s->layered) return fail(PW_ERR_STATE, "pw_get_base_fields: only media from pw_set_medium_layered are kept");
+    PW_HIP_TRY(hipSetDevice(s->cfg.device));
+    double* d_base = nullptr;
+    if ((st = generate_into(s, &s->spec, &d_base)) != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    const size_t ni = (size_t)(c.nx - 2 * c.np) * (c.ny - 2 * c.np) * (c.nz - 2 * c.np);
+    hipError_t e = hipMemcpy(out, d_base, PW_NBASE * ni * sizeof(double), hipMemcpyDeviceToHost);
+ (void)hipFree(d_base);
+    if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_get_base_fields: ") + hipGetErrorString(e));
+    return PW_OK;
+}
+
+extern "C" pw_status pw_set_source(pw_solver* s, int sx, int sy, int sz, double f0, const double* w4)
+{
+    pw_status st = check_handle(s, "pw_set_source");
+    if (st != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    if (sx < 1 || sy < 1 || sz < 1 || sx > c.nx - 2 || sy > c.ny - 2 || sz > c.nz - 2)
+        return fail(PW_ERR_INVALID, "pw_set_source: source cell must be inside the updated grid");
+    if (!(f0 > 0)) return fail(PW_ERR_INVALID, "pw_set_source: f0 must be positive");
+    s->src_off = (unsigned)((long long)sx + (long long)sy * s->b.g.pitch + (long long)sz * s->b.g.plane);
+    s->f0 = f0;
+    const double ref[4] = {1, 1, 1, 0};
+    for (int k = 0; k < 4; ++k) s->w[k] = w4 ? w4[k] : ref[k];
+    s->has_source = true;
+    return PW_OK;
+}
+
+extern "C" pw_status pw_set_receivers(pw_solver* s, int n, const int* field, const long long* index)
+{
+    pw_status st = check_handle(s, "pw_set_receivers");
+    if (st != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    if (n < 0 || (n > 0 && (!field || !index))) return fail(PW_ERR_INVALID, "pw_set_receivers: bad arguments");
+    const long long ncell = (long long)c.nx * c.ny * c.nz;
+    std::vector<unsigned> off(n);
+    for (int r = 0; r < n; ++r) {
+        if (field[r] < 0 || field[r] >= PW_NFIELDS || index[r] < 0 || index[r] >= ncell)
+            return fail(PW_ERR_INVALID, "pw_set_receivers: receiver " + std::to_string(r) + " out of range");
+        const long long ix = index[r] % c.nx, iy = (index[r] / c.nx) % c.ny, iz = index[r] / ((long long)c.nx * c.ny);
+        off[r] = (unsigned)(ix + iy * s->b.g.pitch + iz * s->b.g.plane);
+    }
+    PW_HIP_TRY(hipSetDevice(c.device));
+    PW_HIP_TRY(hipStreamSynchronize(s->stream));
+ (void)hipFree(s->d_rfield); (void)hipFree(s->d_roff); (void)hipFree(s->d_seis);
+    s->d_rfield = nullptr; s->d_roff = nullptr; s->d_seis = nullptr;
+    s->nrec = 0;
+    if (n == 0) return PW_OK;
+    PW_HIP_TRY(hipMalloc(&s->d_rfield, n * sizeof(int)));
+    PW_HIP_TRY(hipMalloc(&s->d_roff, n * sizeof(unsigned)));
+    const size_t seis_bytes = align256((size_t)n * c.nt * sizeof(float));
+    PW_HIP_TRY(hipMalloc(&s->d_seis, seis_bytes));
+    PW_HIP_TRY(hipMemcpy(s->d_rfield, field, n * sizeof(int), hipMemcpyHostToDevice));
+    PW_HIP_TRY(hipMemcpy(s->d_roff, off.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
+    PW_HIP_TRY(launch_zero(s->d_seis, seis_bytes, s->stream));
+    PW_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->nrec = n;
+    return PW_OK;
+}
+
+extern "C" pw_status pw_set_receivers_reference(pw_solver* s, int sx, int sy)
+{
+    pw_status st = check_handle(s, "pw_set_receivers_reference");
+    if (st != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    if (sx < 0 || sy < 0 || sx >= c.nx || sy >= c.ny) return fail(PW_ERR_INVALID, "pw_set_receivers_reference: bad line");
+    const int ch[5] = {PW_TXX, PW_TZZ, PW_VUX, PW_VWX, PW_P};  // sisx, sisz, sisvu, sisvw, sisp
+    std::vector<int> f;
+    std::vector<long long> idx;
+    for (int k = 0; k < 5; ++k)
+        for (int iz = 0; iz < c.nz; ++iz) {
+            f.push_back(ch[k]);
+            idx.push_back((long long)sx + (long long)sy * c.nx + (long long)iz * c.nx * c.ny);
+        }
+    return pw_set_receivers(s, (int)f.size(), f.data(), idx.data());
+}
+
+extern "C" pw_status pw_set_snapshots(pw_solver* s, int n, const int* step, const int* field, int plane_y)
+{
+    pw_status st = check_handle(s, "pw_set_snapshots");
+    if (st != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    if (n < 0 || (n > 0 && (!step || !field)) || plane_y < 0 || plane_y >= c.ny)
+        return fail(PW_ERR_INVALID, "pw_set_snapshots: bad arguments");
+    for (int k = 0; k < n; ++k)
+        if (field[k] < 0 || field[k] >= PW_NFIELDS || step[k] < 0 || step[k] >= c.nt)
+            return fail(PW_ERR_INVALID, "pw_set_snapshots: snapshot " + std::to_string(k) + " out of range");
+    PW_HIP_TRY(hipSetDevice(c.device));
+    PW_HIP_TRY(hipStreamSynchronize(s->stream));
+ (void)hipFree(s->d_snaps);
+    s->d_snaps = nullptr;
+    s->nsnap = 0;
+    s->snap_step.assign(step, step + n);
+    s->snap_field.assign(field, field + n);
+    s->snap_y = plane_y;
+    if (n == 0) return PW_OK;
+    const size_t bytes = align256((size_t)n * c.nz * c.nx * sizeof(float));
+    PW_HIP_TRY(hipMalloc(&s->d_snaps, bytes));
+    PW_HIP_TRY(launch_zero(s->d_snaps, bytes, s->stream));
+    PW_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->nsnap = n;
+    return PW_OK;
+}
+
+extern "C" pw_status pw_reset(pw_solver* s)
+{
+    pw_status st = check_handle(s, "pw_reset");
+    if (st != PW_OK) return st;
+    PW_HIP_TRY(hipSetDevice(s->cfg.device));
+    PW_HIP_TRY(zero_state(s));
+    if (s->d_seis) PW_HIP_TRY(launch_zero(s->d_seis, align256((size_t)s->nrec * s->cfg.nt * 4), s->stream));
+    if (s->d_snaps)
+        PW_HIP_TRY(launch_zero(s->d_snaps, align256((size_t)s->nsnap * s->cfg.nz * s->cfg.nx * 4), s->stream));
+    PW_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->step = 0;
+    s->dyn_scale = 1.0f;
+    return PW_OK;
+}
+
+namespace {
+
+// dynamic_scale_for_step, Half/main.cu:116-122
+float dynamic_scale_for_step(int it, int nt, float end)
+{
+    if (nt <= 1) return end;
+    const float alpha = (float)it / (float)(nt - 1);
+    return 1.0f + (end - 1.0f) * alpha;
+}
+
+}  // namespace
+
+extern "C" pw_status pw_run(pw_solver* s, int nsteps)
+{
+    pw_status st = check_handle(s, "pw_run");
+    if (st != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    if (!s->has_medium) return fail(PW_ERR_STATE, "pw_run: no medium installed");
+    if (nsteps < 0 || s->step + nsteps > c.nt)
+        return fail(PW_ERR_INVALID, "pw_run: steps beyond the configured nt");
+    PW_HIP_TRY(hipSetDevice(c.device));
+    const bool scaled = is_scaled(s->prec);
+    const float dyn_end = c.half_dyn_end > 0 ? (float)c.half_dyn_end : 16.0f;
+    for (int k = 0; k < nsteps; ++k) {
+        const int it = s->step;
+        if (scaled && (it == 0 || (c.half_dyn_interval > 0 && it % c.half_dyn_interval == 0))) {
+            const float target = dynamic_scale_for_step(it, c.nt, dyn_end);
+            const float ratio = target / s->dyn_scale;
+            if (std::fabs(ratio - 1.0f) > 1.0e-6f) {
+                PW_HIP_TRY(launch_rescale(s->b, s->stream, ratio));
+                s->dyn_scale = target;
+            }
+        }
+        if (s->has_source) {
+            double amp;
+            if (s->prec == PW_FP64) amp = ricker_f64(it, c.dt, s->f0);
+            else if (scaled) amp = s->dyn_scale * s->source_scale * ricker_f32(it, (float)c.dt, (float)s->f0);
+            else amp = ricker_f32(it, (float)c.dt, (float)s->f0);
+            PW_HIP_TRY(launch_source(s->b, s->stream, s->src_off, s->w, amp));
+        }
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        if (s->profiling) {
+            for (int q = 0; q < 4; ++q) {
+                if (s->ev_pool.empty()) {
+                    hipEvent_t e = nullptr;
+                    PW_HIP_TRY(hipEventCreate(&e));
+                    s->ev_pool.push_back(e);
+                }
+                ev[q] = s->ev_pool.back();
+                s->ev_pool.pop_back();
+                s->ev_pending.push_back(ev[q]);
+            }
+        }
+        if (ev[0]) PW_HIP_TRY(hipEventRecord(ev[0], s->stream));
+        PW_HIP_TRY(launch_velocity(s->b, s->stream));
+        if (ev[1]) PW_HIP_TRY(hipEventRecord(ev[1], s->stream));
+        if (ev[2]) PW_HIP_TRY(hipEventRecord(ev[2], s->stream));
+        PW_HIP_TRY(launch_stress(s->b, s->stream));
+        if (ev[3]) PW_HIP_TRY(hipEventRecord(ev[3], s->stream));
+        const float ss = scaled ? s->stress_base / s->dyn_scale : 1.0f;
+        const float sv = scaled ? s->vel_base / s->dyn_scale : 1.0f;
+        if (s->nrec > 0)
+            PW_HIP_TRY(launch_gather(s->b, s->stream, s->d_rfield, s->d_roff, s->nrec, s->d_seis, c.nt, it, ss, sv));
+        for (int q = 0; q < s->nsnap; ++q)
+            if (s->snap_step[q] == it) {
+                const float sc = s->snap_field[q] <= PW_VWZ ? sv : ss;
+                PW_HIP_TRY(launch_snapshot(s->b, s->stream, s->snap_field[q], s->snap_y,
+                                           s->d_snaps + (size_t)q * c.nz * c.nx, sc));
+            }
+        s->step++;
+    }
+    return PW_OK;
+}
+
+extern "C" pw_status pw_synchronize(pw_solver* s)
+{
+    pw_status st = check_handle(s, "pw_synchronize");
+    if (st != PW_OK) return st;
+    PW_HIP_TRY(hipSetDevice(s->cfg.device));
+    PW_HIP_TRY(hipStreamSynchronize(s->stream));
+    return PW_OK;
+}
+
+extern "C" int pw_current_step(const pw_solver* s) { return s ? s->step : -1; }
+
+extern "C" pw_status pw_get_seismogram(pw_solver* s, float* host)
+{
+    pw_status st = check_handle(s, "pw_get_seismogram");
+    if (st != PW_OK) return st;
+    if (!host) return fail(PW_ERR_INVALID, "pw_get_seismogram: null output");
+    if (s->nrec == 0) return PW_OK;
+    PW_HIP_TRY(hipSetDevice(s->cfg.device));
+    PW_HIP_TRY(hipStreamSynchronize(s->stream));
+    PW_HIP_TRY(hipMemcpy(host, s->d_seis, (size_t)s->nrec * s->cfg.nt * sizeof(float), hipMemcpyDeviceToHost));
+    return PW_OK;
+}
+
+extern "C" pw_status pw_get_snapshots(pw_solver* s, float* host)
+{
+    pw_status st = check_handle(s, "pw_get_snapshots");
+    if (st != PW_OK) return st;
+    if (!host) return fail(PW_ERR_INVALID, "pw_get_snapshots: null output");
+    if (s->nsnap == 0) return PW_OK;
+    PW_HIP_TRY(hipSetDevice(s->cfg.device));
+    PW_HIP_TRY(hipStreamSynchronize(s->stream));
+    PW_HIP_TRY(hipMemcpy(host, s->d_snaps, (size_t)s->nsnap * s->cfg.nz * s->cfg.nx * sizeof(float),
+                         hipMemcpyDeviceToHost));
+    return PW_OK;
+}
+
+extern "C" pw_status pw_get_field(pw_solver* s, int field, float* host)
+{
+    pw_status st = check_handle(s, "pw_get_field");
+    if (st != PW_OK) return st;
+    if (!host || field < 0 || field >= PW_NFIELDS) return fail(PW_ERR_INVALID, "pw_get_field: bad arguments");
+    const pw_config& c = s->cfg;
+    PW_HIP_TRY(hipSetDevice(c.device));
+    const size_t bytes = (size_t)c.nx * c.ny * c.nz * sizeof(float);
+    float* d = nullptr;
+    PW_HIP_TRY(hipMalloc(&d, bytes));
+    const bool scaled = is_scaled(s->prec);
+    const float sc = !scaled ? 1.0f : (field <= PW_VWZ ? s->vel_base : s->stress_base) / s->dyn_scale;
+    hipError_t e = launch_decode(s->b, s->stream, field, d, sc);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipMemcpy(host, d, bytes, hipMemcpyDeviceToHost);
+ (void)hipFree(d);
+    if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_get_field: ") + hipGetErrorString(e));
+    return PW_OK;
+}
+
+extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
+{
+    pw_status st = check_handle(s, "pw_get_info");
+    if (st != PW_OK) return st;
+    if (!out) return fail(PW_ERR_INVALID, "pw_get_info: null output");
+    const Geom& g = s->b.g;
+    out->precision = s->prec;
+    out->em = s->em;
+    out->epml = s->epml;
+    // Compulsory traffic of the two-kernel step: 39 wavefield touches, the
+    // coefficient arrays once, and 16 memory-variable touches per damped
+    // (cell, axis) (8 variables read + written).
+    const double n3[3] = {(double)g.nx, (double)g.ny, (double)g.nz};
+    double f = 0.0;
+    for (int ax = 0; ax < 3; ++ax) f += (g.lo[ax] + g.hi[ax]) / n3[ax];
+    // velocity: reads tau x6 + p, read-modify-writes vu x3 + vw x3 (19 touches);
+    // stress: reads vu x3 + vw x3, read-modify-writes tau x6 + p (20 touches).
+    const int cv = is_scaled(s->prec) ? 15 : 10, ct = 7;
+    out->bytes_velocity = 19.0 * state_bytes(s->prec) + cv * coef_bytes(s->prec) + 8.0 * f * mem_bytes(s->prec);
+    out->bytes_stress = 20.0 * state_bytes(s->prec) + ct * coef_bytes(s->prec) + 8.0 * f * mem_bytes(s->prec);
+    out->bytes_per_cell_step = out->bytes_velocity + out->bytes_stress;
+    out->vp_max = s->vp_max;
+    out->courant = s->courant;
+    out->device_bytes = (long long)s->pool_bytes;
+    for (int ax = 0; ax < 3; ++ax) {
+        out->slab_lo[ax] = g.lo[ax];
+        out->slab_hi[ax] = g.hi[ax];
+    }
+    return PW_OK;
+}
+
+extern "C" pw_status pw_set_profiling(pw_solver* s, int enable)
+{
+    pw_status st = check_handle(s, "pw_set_profiling");
+    if (st != PW_OK) return st;
+    s->profiling = enable != 0;
+    return PW_OK;
+}
+
+extern "C" pw_status pw_get_kernel_times(pw_solver* s, double* ms_velocity, double* ms_stress, long long* launches)
+{
+    pw_status st = check_handle(s, "pw_get_kernel_times");
+    if (st != PW_OK) return st;
+    PW_HIP_TRY(hipSetDevice(s->cfg.device));
+    PW_HIP_TRY(hipStreamSynchronize(s->stream));
+    double tv = 0.0, tt = 0.0;
+    const size_t nstep = s->ev_pending.size() / 4;
+    for (size_t k = 0; k < nstep; ++k) {
+        float a = 0.0f, b = 0.0f;
+        PW_HIP_TRY(hipEventElapsedTime(&a, s->ev_pending[4 * k], s->ev_pending[4 * k + 1]));
+        PW_HIP_TRY(hipEventElapsedTime(&b, s->ev_pending[4 * k + 2], s->ev_pending[4 * k + 3]));
+        tv += a;
+        tt += b;
+    }
+    for (hipEvent_t e : s->ev_pending) s->ev_pool.push_back(e);
+    s->ev_pending.clear();
+    if (ms_velocity) *ms_velocity = tv;
+    if (ms_stress) *ms_stress = tt;
+    if (launches) *launches = (long long)nstep;
+    return PW_OK;
+}

@@ -1,0 +1,76 @@
+// pw_device.h -- interface between the host API (pw_api.cpp) and the HIP
+// kernels (pw_kernels.hip, pw_medium_gpu.hip).  Plain pointers only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pw_internal.h"
+
+namespace pw {
+
+// Coefficient slots.  "physical" form (FP32 / FP64 storage): the reference's
+// Float arrays with the Crank-Nicolson pair folded to CC2 and 1/CC1
+// (Float/main.cu:106-113, 160-185).  "scaled" form (comp16 / bf16): the
+// reference's Half FP16 arrays (Half/main.cu:845-1045).
+enum PhysCoef {
+    PC_RIX = 0, PC_RIY, PC_RIZ, PC_RHOF, PC_Q2X, PC_Q2Y, PC_Q2Z, PC_U1X, PC_U1Y, PC_U1Z,
+    PC_C, PC_M, PC_HH, PC_H2U, PC_MUXY, PC_MUXZ, PC_MUYZ, PC_N
+};
+enum ScaledCoef {
+    SC_RITX = 0, SC_RITY, SC_RITZ, SC_FRX, SC_FRY, SC_FRZ, SC_AWX, SC_AWY, SC_AWZ,
+    SC_BPX, SC_BPY, SC_BPZ, SC_BTX, SC_BTY, SC_BTZ, SC_HH, SC_H2U, SC_C, SC_M,
+    SC_MUXY, SC_MUXZ, SC_MUYZ, SC_N
+};
+constexpr int kMaxCoef = 22;
+
+inline bool is_scaled(int prec) { return prec == PW_COMP16 || prec == PW_BF16; }
+inline int ncoef(int prec) { return is_scaled(prec) ? (int)SC_N : (int)PC_N; }
+// Bytes of one stored wavefield value / coefficient / memory variable.
+inline int state_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : prec == PW_COMP16 ? 3 : 2; }
+inline int coef_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : 2; }
+inline int mem_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : 2; }
+
+// Device pointers of one solver.  For comp16, f[] holds the FP16 "hi" arrays
+// and lo[] the int8 residuals; other precisions leave lo[] null.
+struct DevBuffers {
+    int prec;
+    Geom g;
+    void* f[PW_NFIELDS];
+    int8_t* lo[PW_NFIELDS];
+    void* coef[kMaxCoef];
+    void* mem[3][kMemPerAxis];
+    void* tab[3];    // per axis: n entries of {c_full, e_full, c_half, e_half} (float or double)
+    double dt, h;
+    int zc;          // z cells per block in the step kernels
+};
+
+hipError_t launch_velocity(const DevBuffers& b, hipStream_t st);
+hipError_t launch_stress(const DevBuffers& b, hipStream_t st);
+hipError_t launch_source(const DevBuffers& b, hipStream_t st, unsigned off, const double w[4], double amp);
+hipError_t launch_gather(const DevBuffers& b, hipStream_t st, const int* rfield, const unsigned* roff, int nrec,
+                         float* seis, int nt, int it, float s_stress, float s_vel);
+hipError_t launch_snapshot(const DevBuffers& b, hipStream_t st, int field, int y, float* dst, float scale);
+hipError_t launch_decode(const DevBuffers& b, hipStream_t st, int field, float* dst, float scale);
+hipError_t launch_rescale(const DevBuffers& b, hipStream_t st, float ratio);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);
+
+// ---- medium precompute (pw_medium_gpu.hip), all FP64 on the device ----
+struct MediumArgs {
+    const double* base[PW_NBASE];  // interior fields nxi*nyi*nzi
+    int nxi, nyi, nzi, np;
+    int nx, ny, nz;                // total grid
+    double dt;
+};
+// res[0] = Kmax, res[1] = Rmax, res[2] = max fast-P velocity squared (Half/main.cu:745-791)
+hipError_t medium_reduce(const MediumArgs& m, double* d_res3, hipStream_t st);
+// Writes the coefficient arrays of b (physical or scaled form).
+hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double scale_m, double inv_scale_m,
+                               hipStream_t st);
+
+// ---- layered synthetic medium generator (pw_medium_gpu.hip) ----
+hipError_t generate_layered(const pw_layered& spec, int nxi, int nyi, int nzi, int np, double* d_base /*9*n*/,
+                            double* d_work /*4*n*/, double* d_partial /* >= 2*4096 */, hipStream_t st);
+
+}  // namespace pw

@@ -1,0 +1,412 @@
+// pw_medium_gpu.hip -- FP64 medium precompute on the GPU.
+//
+// Replaces the host precompute of the reference: extmodel edge replication
+// (Float/main.cu:1662-1794), staggered averaging (:473-513), the Crank-Nicolson
+// coefficient pair of FD_V (:106-113) and the Half exponent scan and FP16
+// coefficient quantisation (Half/main.cu:745-1045).  Everything is evaluated in
+// FP64 from the cell-centred base fields and rounded once to the storage type.
+// Also holds the layered synthetic-medium generator.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "pw_device.h"
+
+namespace pw {
+
+typedef _Float16 f16;
+
+struct BaseView {
+    const double* b[PW_NBASE];
+    int nxi, nyi, nzi, np;
+    // padded (extmodel) value of field f at total-grid cell (ix, iy, iz)
+    __device__ __forceinline__ double at(int f, int ix, int iy, int iz) const
+    {
+        int kx = ix - np, ky = iy - np, kz = iz - np;
+        kx = kx < 0 ? 0 : (kx >= nxi ? nxi - 1 : kx);
+        ky = ky < 0 ? 0 : (ky >= nyi ? nyi - 1 : ky);
+        kz = kz < 0 ? 0 : (kz >= nzi ? nzi - 1 : kz);
+        return b[f][(size_t)kx + (size_t)ky * nxi + (size_t)kz * nxi * nyi];
+    }
+};
+
+// Staggered quantities of one cell, Float/main.cu:473-513 (zero on the outer
+// layer, which the reference never computes).
+struct Stag {
+    double rix, riy, riz, c1x, c1y, c1z, c2x, c2y, c2z, muxy, muxz, muyz;
+};
+
+__device__ __forceinline__ double harm4(double a, double b, double c, double d)
+{
+    if (a == 0.0 || b == 0.0 || c == 0.0 || d == 0.0) return 0.0;
+    return 1.0 / (0.25 * (1.0 / a + 1.0 / b + 1.0 / c + 1.0 / d));
+}
+
+__device__ Stag staggered(const BaseView& v, int ix, int iy, int iz, int nx, int ny, int nz)
+{
+    Stag s = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (ix < 1 || iy < 1 || iz < 1 || ix > nx - 2 || iy > ny - 2 || iz > nz - 2) return s;
+    const double r0 = v.at(PW_RHO, ix, iy, iz);
+    s.rix = 2.0 / (r0 + v.at(PW_RHO, ix + 1, iy, iz));
+    s.riy = 2.0 / (r0 + v.at(PW_RHO, ix, iy + 1, iz));
+    s.riz = 2.0 / (r0 + v.at(PW_RHO, ix, iy, iz + 1));
+    const double a0 = v.at(PW_C1, ix, iy, iz), b0 = v.at(PW_C2, ix, iy, iz);
+    s.c1x = 0.5 * (a0 + v.at(PW_C1, ix + 1, iy, iz));
+    s.c1y = 0.5 * (a0 + v.at(PW_C1, ix, iy + 1, iz));
+    s.c1z = 0.5 * (a0 + v.at(PW_C1, ix, iy, iz + 1));
+    s.c2x = 0.5 * (b0 + v.at(PW_C2, ix + 1, iy, iz));
+    s.c2y = 0.5 * (b0 + v.at(PW_C2, ix, iy + 1, iz));
+    s.c2z = 0.5 * (b0 + v.at(PW_C2, ix, iy, iz + 1));
+    const double m000 = v.at(PW_MU, ix, iy, iz);
+    const double m100 = v.at(PW_MU, ix + 1, iy, iz), m010 = v.at(PW_MU, ix, iy + 1, iz);
+    const double m001 = v.at(PW_MU, ix, iy, iz + 1), m110 = v.at(PW_MU, ix + 1, iy + 1, iz);
+    const double m101 = v.at(PW_MU, ix + 1, iy, iz + 1), m011 = v.at(PW_MU, ix, iy + 1, iz + 1);
+    s.muxz = harm4(m000, m100, m001, m101);
+    s.muyz = harm4(m000, m010, m001, m011);
+    s.muxy = harm4(m000, m100, m010, m110);
+    return s;
+}
+
+// Largest squared P velocity of the cell: the high-frequency Biot fast wave,
+// the larger root of det([[H, C], [C, M]] - v^2 [[rho, rhof], [rhof, m]]) = 0
+// with m = C2; a cell without viscous coupling (the borehole fluid) uses H/rho.
+__device__ double vp2_cell(double rho, double rhof, double c1, double m, double H, double C, double M)
+{
+    const double a = rho * m - rhof * rhof;
+    if (c1 == 0.0 || a <= 1e-12 * rho * m) return H / rho;
+    const double b = H * m + M * rho - 2.0 * C * rhof;
+    const double c = H * M - C * C;
+    const double disc = fmax(b * b - 4.0 * a * c, 0.0);
+    return (b + sqrt(disc)) / (2.0 * a);
+}
+
+__device__ __forceinline__ void atomic_max_pos(double* addr, double v)
+{
+    // Non-negative doubles order like their bit patterns.
+    atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double* res)
+{
+    __shared__ double sk[256], sr[256], sv[256];
+    const long long n = (long long)nx * ny * nz;
+    double kmax = 0.0, rmax = 0.0, vmax = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int ix = (int)(i % nx), iy = (int)((i / nx) % ny), iz = (int)(i / ((long long)nx * ny));
+        const Stag s = staggered(v, ix, iy, iz, nx, ny, nz);
+        const double rhof = v.at(PW_RHOF, ix, iy, iz);
+        const double HH = v.at(PW_HH, ix, iy, iz), H2u = v.at(PW_H2U, ix, iy, iz);
+        const double C = v.at(PW_CC, ix, iy, iz), M = v.at(PW_MM, ix, iy, iz), mu = v.at(PW_MU, ix, iy, iz);
+        // Kmax, Half/main.cu:754-761
+        kmax = fmax(kmax, fmax(fmax(fabs(HH), fabs(H2u)), fmax(fabs(C), fabs(M))));
+        kmax = fmax(kmax, fmax(fabs(mu), fmax(fabs(s.muxy), fmax(fabs(s.muxz), fabs(s.muyz)))));
+        // Rmax, Half/main.cu:763-788
+        const double ri[3] = {s.rix, s.riy, s.riz}, c1[3] = {s.c1x, s.c1y, s.c1z}, c2[3] = {s.c2x, s.c2y, s.c2z};
+        for (int d = 0; d < 3; ++d) {
+            if (ri[d] > 0.0) rmax = fmax(rmax, ri[d]);
+            const double Q = c2[d] - rhof * rhof * ri[d];
+            if (c1[d] != 0.0 && fabs(Q) > 1.0e-12) {
+                rmax = fmax(rmax, 1.0 / fabs(Q));
+                rmax = fmax(rmax, fabs(rhof * ri[d] / Q));
+            }
+        }
+        vmax = fmax(vmax, vp2_cell(v.at(PW_RHO, ix, iy, iz), rhof, v.at(PW_C1, ix, iy, iz), v.at(PW_C2, ix, iy, iz),
+                                   HH, C, M));
+    }
+    sk[threadIdx.x] = kmax; sr[threadIdx.x] = rmax; sv[threadIdx.x] = vmax;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sk[threadIdx.x] = fmax(sk[threadIdx.x], sk[threadIdx.x + w]);
+            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + w]);
+            sv[threadIdx.x] = fmax(sv[threadIdx.x], sv[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomic_max_pos(res + 0, sk[0]);
+        atomic_max_pos(res + 1, sr[0]);
+        atomic_max_pos(res + 2, sv[0]);
+    }
+}
+
+template <class T> __device__ __forceinline__ T to_store(double v) { return (T)v; }
+template <> __device__ __forceinline__ f16 to_store<f16>(double v) { return (f16)(float)v; }
+
+struct CoefOut {
+    void* c[kMaxCoef];
+    int pitch;
+    long long plane;
+};
+
+// Physical form (FP32 / FP64 storage): rho^-1 (x,y,z), rhof, CC2 (x,y,z),
+// 1/CC1 (x,y,z; 0 where C1 == 0), C, M, H, H-2mu, mu_xy, mu_xz, mu_yz.
+template <class T>
+__global__ void k_coef_physical(BaseView v, int nx, int ny, int nz, double dt, CoefOut out)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= nx) return;
+    const size_t o = (size_t)ix + (size_t)iy * out.pitch + (size_t)iz * out.plane;
+    const Stag s = staggered(v, ix, iy, iz, nx, ny, nz);
+    const double rhof = v.at(PW_RHOF, ix, iy, iz);
+    const double ri[3] = {s.rix, s.riy, s.riz}, c1[3] = {s.c1x, s.c1y, s.c1z}, c2[3] = {s.c2x, s.c2y, s.c2z};
+    T** c = (T**)out.c;
+    for (int d = 0; d < 3; ++d) {
+        const double Q = c2[d] - rhof * rhof * ri[d];
+        c[PC_RIX + d][o] = to_store<T>(ri[d]);
+        c[PC_Q2X + d][o] = to_store<T>(Q - 0.5 * c1[d] * dt);
+        c[PC_U1X + d][o] = to_store<T>(c1[d] == 0.0 ? 0.0 : 1.0 / (Q + 0.5 * c1[d] * dt));
+    }
+    c[PC_RHOF][o] = to_store<T>(rhof);
+    c[PC_C][o] = to_store<T>(v.at(PW_CC, ix, iy, iz));
+    c[PC_M][o] = to_store<T>(v.at(PW_MM, ix, iy, iz));
+    c[PC_HH][o] = to_store<T>(v.at(PW_HH, ix, iy, iz));
+    c[PC_H2U][o] = to_store<T>(v.at(PW_H2U, ix, iy, iz));
+    c[PC_MUXY][o] = to_store<T>(s.muxy);
+    c[PC_MUXZ][o] = to_store<T>(s.muxz);
+    c[PC_MUYZ][o] = to_store<T>(s.muyz);
+}
+
+// Scaled form (comp16 / bf16), Half/main.cu:967-1044: FP16 arrays prescaled by
+// dt and the velocity exponent 2^em.
+__global__ void k_coef_scaled(BaseView v, int nx, int ny, int nz, double dt, double sm, double ism, CoefOut out)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= nx) return;
+    const size_t o = (size_t)ix + (size_t)iy * out.pitch + (size_t)iz * out.plane;
+    const Stag s = staggered(v, ix, iy, iz, nx, ny, nz);
+    const double rhof = v.at(PW_RHOF, ix, iy, iz);
+    const double ri[3] = {s.rix, s.riy, s.riz}, c1[3] = {s.c1x, s.c1y, s.c1z}, c2[3] = {s.c2x, s.c2y, s.c2z};
+    f16** c = (f16**)out.c;
+    for (int d = 0; d < 3; ++d) {
+        const double Q = c2[d] - rhof * rhof * ri[d];
+        const double den = Q + 0.5 * c1[d] * dt;
+        double aw = 0.0, bp = 0.0, bt = 0.0;
+        if (c1[d] != 0.0 && fabs(Q) > 1.0e-12 && fabs(den) > 1.0e-12) {
+            aw = (Q - 0.5 * c1[d] * dt) / den;
+            bp = ism * dt / den;
+            bt = ism * dt * rhof * ri[d] / den;
+        }
+        c[SC_RITX + d][o] = to_store<f16>(ism * dt * ri[d]);
+        c[SC_FRX + d][o] = to_store<f16>(rhof * ri[d]);
+        c[SC_AWX + d][o] = to_store<f16>(aw);
+        c[SC_BPX + d][o] = to_store<f16>(bp);
+        c[SC_BTX + d][o] = to_store<f16>(bt);
+    }
+    c[SC_HH][o] = to_store<f16>(sm * dt * v.at(PW_HH, ix, iy, iz));
+    c[SC_H2U][o] = to_store<f16>(sm * dt * v.at(PW_H2U, ix, iy, iz));
+    c[SC_C][o] = to_store<f16>(sm * dt * v.at(PW_CC, ix, iy, iz));
+    c[SC_M][o] = to_store<f16>(sm * dt * v.at(PW_MM, ix, iy, iz));
+    c[SC_MUXZ][o] = to_store<f16>(sm * dt * s.muxz);
+    c[SC_MUXY][o] = to_store<f16>(sm * dt * s.muxy);
+    c[SC_MUYZ][o] = to_store<f16>(sm * dt * s.muyz);
+}
+
+static BaseView make_view(const MediumArgs& m)
+{
+    BaseView v;
+    for (int k = 0; k < PW_NBASE; ++k) v.b[k] = m.base[k];
+    v.nxi = m.nxi; v.nyi = m.nyi; v.nzi = m.nzi; v.np = m.np;
+    return v;
+}
+
+hipError_t medium_reduce(const MediumArgs& m, double* d_res3, hipStream_t st)
+{
+    hipError_t e = hipMemsetAsync(d_res3, 0, 3 * sizeof(double), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_medium_reduce, dim3(2048), dim3(256), 0, st, make_view(m), m.nx, m.ny, m.nz, d_res3);
+    return hipGetLastError();
+}
+
+hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double scale_m, double inv_scale_m,
+                               hipStream_t st)
+{
+    CoefOut out;
+    for (int k = 0; k < kMaxCoef; ++k) out.c[k] = b.coef[k];
+    out.pitch = b.g.pitch;
+    out.plane = b.g.plane;
+    const dim3 grid((unsigned)((m.nx + 127) / 128), (unsigned)m.ny, (unsigned)m.nz), block(128);
+    const BaseView v = make_view(m);
+    switch (b.prec) {
+    case PW_FP32: hipLaunchKernelGGL(k_coef_physical<float>, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, out); break;
+    case PW_FP64: hipLaunchKernelGGL(k_coef_physical<double>, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, out); break;
+    case PW_COMP16:
+    case PW_BF16:
+        hipLaunchKernelGGL(k_coef_scaled, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, scale_m, inv_scale_m, out);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// Layered synthetic medium (FP64)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Standard normal white noise, one independent stream per field.
+__global__ void k_noise(double* out, long long n, unsigned long long seed, int stream)
+{
+    const unsigned long long key = seed ^ ((unsigned long long)(stream + 1) * 0xD1B54A32D192ED03ull);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const unsigned long long h1 = splitmix64(key + 2ull * (unsigned long long)i);
+        const unsigned long long h2 = splitmix64(key + 2ull * (unsigned long long)i + 1ull);
+        const double u1 = ((double)(h1 >> 11) + 1.0) * 0x1.0p-53;   // (0, 1]
+        const double u2 = (double)(h2 >> 11) * 0x1.0p-53;           // [0, 1)
+        out[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+}
+
+struct GaussW {
+    double w[97];
+    int radius;
+};
+
+// One separable Gaussian pass along axis (0 = x, 1 = y, 2 = z), edge-clamped.
+__global__ void k_smooth(const double* __restrict__ in, double* __restrict__ out, int nx, int ny, int nz, int axis,
+                         GaussW g)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= nx) return;
+    const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    const int i0 = axis == 0 ? ix : (axis == 1 ? iy : iz);
+    const long long stride = axis == 0 ? 1 : (axis == 1 ? nx : (long long)nx * ny);
+    const long long base = (long long)ix + (long long)iy * nx + (long long)iz * nx * ny - (long long)i0 * stride;
+    double acc = 0.0;
+    for (int k = -g.radius; k <= g.radius; ++k) {
+        int j = i0 + k;
+        j = j < 0 ? 0 : (j >= n ? n - 1 : j);
+        acc += g.w[k + g.radius] * in[base + (long long)j * stride];
+    }
+    out[(long long)ix + (long long)iy * nx + (long long)iz * nx * ny] = acc;
+}
+
+// Per-block sums (fixed grid, fixed order) for a deterministic mean / variance.
+__global__ void k_partial_sums(const double* __restrict__ in, long long n, double* __restrict__ part)
+{
+    __shared__ double s1[256], s2[256];
+    double a = 0.0, b = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double v = in[i];
+        a += v;
+        b += v * v;
+    }
+    s1[threadIdx.x] = a; s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) { s1[threadIdx.x] += s1[threadIdx.x + w]; s2[threadIdx.x] += s2[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[blockIdx.x] = s1[0]; part[gridDim.x + blockIdx.x] = s2[0]; }
+}
+
+struct LayerSpec {
+    int nlayers;
+    int z_top[PW_MAX_LAYERS];
+    pw_rock rock[PW_MAX_LAYERS];
+    double amp, clamp;
+    double mean[3], inv_std[3];
+    int use_noise;
+};
+
+// Biot coefficients per interior cell (formulas of Float/main.cu:1635-1650).
+__global__ void k_layered(double* base, const double* __restrict__ n_kb, const double* __restrict__ n_mu,
+                          const double* __restrict__ n_phi, int nxi, int nyi, int nzi, int np, LayerSpec L)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= nxi) return;
+    const long long n = (long long)nxi * nyi * nzi;
+    const long long i = (long long)ix + (long long)iy * nxi + (long long)iz * nxi * nyi;
+    const int zt = iz + np;
+    int l = 0;
+    while (l + 1 < L.nlayers && zt >= L.z_top[l + 1]) ++l;
+    const pw_rock r = L.rock[l];
+    double fkb = 1.0, fmu = 1.0, fphi = 1.0;
+    if (L.use_noise) {
+        const double a = fmin(fmax((n_kb[i] - L.mean[0]) * L.inv_std[0], -L.clamp), L.clamp);
+        const double b = fmin(fmax((n_mu[i] - L.mean[1]) * L.inv_std[1], -L.clamp), L.clamp);
+        const double c = fmin(fmax((n_phi[i] - L.mean[2]) * L.inv_std[2], -L.clamp), L.clamp);
+        fkb = 1.0 + L.amp * a;
+        fmu = 1.0 + L.amp * b;
+        fphi = 1.0 + L.amp * c;
+    }
+    const double Kb = r.Kb * fkb, mu = r.mu * fmu, phi = r.phi * fphi;
+    const double a1 = 1.0 - Kb / r.Ks;
+    const double M = r.Kf * r.Ks / (phi * r.Ks + (a1 - phi) * r.Kf);
+    const double HH = a1 * a1 * M + Kb + mu * 4.0 / 3.0;
+    base[PW_RHO * n + i] = (1.0 - phi) * r.rhos + phi * r.rhof;
+    base[PW_RHOF * n + i] = r.rhof;
+    base[PW_C1 * n + i] = r.eta / r.kappa;
+    base[PW_C2 * n + i] = (1.0 + 2.0 / r.pore_m) * r.tortuosity * r.rhof / phi;
+    base[PW_HH * n + i] = HH;
+    base[PW_H2U * n + i] = HH - 2.0 * mu;
+    base[PW_CC * n + i] = M * a1;
+    base[PW_MM * n + i] = M;
+    base[PW_MU * n + i] = mu;
+}
+
+hipError_t generate_layered(const pw_layered& spec, int nxi, int nyi, int nzi, int np, double* d_base,
+                            double* d_work, double* d_partial, hipStream_t st)
+{
+    const long long n = (long long)nxi * nyi * nzi;
+    LayerSpec L;
+    L.nlayers = spec.nlayers;
+    for (int k = 0; k < PW_MAX_LAYERS; ++k) {
+        L.z_top[k] = spec.z_top[k];
+        L.rock[k] = spec.rock[k];
+    }
+    L.amp = spec.perturb;
+    L.clamp = spec.clamp > 0 ? spec.clamp : 3.0;
+    L.use_noise = spec.perturb != 0.0;
+    const dim3 g3((unsigned)((nxi + 127) / 128), (unsigned)nyi, (unsigned)nzi);
+    if (L.use_noise) {
+        GaussW gw;
+        const double sig = spec.corr_cells > 0 ? spec.corr_cells : 8.0;
+        gw.radius = (int)std::ceil(3.0 * sig);
+        if (gw.radius > 48) gw.radius = 48;
+        double sum = 0.0;
+        for (int k = -gw.radius; k <= gw.radius; ++k) sum += std::exp(-0.5 * k * k / (sig * sig));
+        for (int k = -gw.radius; k <= gw.radius; ++k) gw.w[k + gw.radius] = std::exp(-0.5 * k * k / (sig * sig)) / sum;
+        double* tmp = d_work + 3 * n;
+        const int nblk = 1024;
+        std::vector<double> part(2 * nblk);
+        for (int f = 0; f < 3; ++f) {
+            double* fld = d_work + f * n;
+            hipLaunchKernelGGL(k_noise, dim3(4096), dim3(256), 0, st, fld, n, spec.seed, f);
+            hipLaunchKernelGGL(k_smooth, g3, dim3(128), 0, st, fld, tmp, nxi, nyi, nzi, 0, gw);
+            hipLaunchKernelGGL(k_smooth, g3, dim3(128), 0, st, tmp, fld, nxi, nyi, nzi, 1, gw);
+            hipLaunchKernelGGL(k_smooth, g3, dim3(128), 0, st, fld, tmp, nxi, nyi, nzi, 2, gw);
+            hipError_t e = hipMemcpyAsync(fld, tmp, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_partial_sums, dim3(nblk), dim3(256), 0, st, fld, n, d_partial);
+            e = hipMemcpyAsync(part.data(), d_partial, 2 * nblk * sizeof(double), hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) return e;
+            e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return e;
+            double s = 0.0, s2 = 0.0;
+            for (int b = 0; b < nblk; ++b) { s += part[b]; s2 += part[nblk + b]; }
+            const double mean = s / (double)n;
+            const double var = s2 / (double)n - mean * mean;
+            L.mean[f] = mean;
+            L.inv_std[f] = var > 0 ? 1.0 / std::sqrt(var) : 0.0;
+        }
+    } else {
+        for (int f = 0; f < 3; ++f) { L.mean[f] = 0; L.inv_std[f] = 0; }
+    }
+    hipLaunchKernelGGL(k_layered, g3, dim3(128), 0, st, d_base, d_work, d_work + n, d_work + 2 * n, nxi, nyi, nzi, np,
+                       L);
+    return hipGetLastError();
+}
+
+}  // namespace pw
