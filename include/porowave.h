@@ -184,8 +184,8 @@ pw_status pw_get_info(pw_solver* s, pw_info* out);
 
 /* Per-kernel timing: with profiling enabled, pw_run brackets every velocity and
  * stress launch with HIP events on the handle's stream.  pw_get_kernel_times
- * synchronises, returns the summed milliseconds and launch count since the last
- * call, and clears them.  No counterpart in the reference, which times only the
+ * synchronises, returns the summed milliseconds of each kernel and the number of
+ * profiled steps since the last call, and clears them.  No counterpart in the reference, which times only the
  * whole loop (Float/main.cu:1043-1046, 1176-1184). */
 pw_status pw_set_profiling(pw_solver* s, int enable);
 pw_status pw_get_kernel_times(pw_solver* s, double* ms_velocity, double* ms_stress, long long* launches);

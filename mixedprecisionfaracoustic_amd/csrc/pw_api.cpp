@@ -62,7 +62,9 @@ struct pw_solver {
     // per-kernel timing (pw_set_profiling)
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;     // created lazily, reused
-    std::vector<hipEvent_t> ev_pending;  // 4 per profiled step: v0, v1, t0, t1
+    std::vector<hipEvent_t> ev_pending;  // (start, stop) pairs around step-kernel launches
+    std::vector<char> ev_kind;           // per pair: 0 velocity, 1 stress
+    long long prof_steps = 0;
 };
 
 namespace {
@@ -271,6 +273,12 @@ extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
     int zc = c.nz >= 256 ? 16 : 4;
     env_int("PW_ZC", &zc);
     s->b.zc = std::max(1, zc);
+    int xcd = 1;
+    env_int("PW_XCD_MAP", &xcd);
+    s->b.xcd_map = xcd ? 1 : 0;
+    int slab = 0;
+    env_int("PW_SLAB", &slab);
+    s->b.slab = std::max(0, slab);
 
     if (c.stream) {
         s->stream = (hipStream_t)c.stream;
@@ -632,25 +640,53 @@ extern "C" pw_status pw_run(pw_solver* s, int nsteps)
             else amp = ricker_f32(it, (float)c.dt, (float)s->f0);
             PW_HIP_TRY(launch_source(s->b, s->stream, s->src_off, s->w, amp));
         }
-        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-        if (s->profiling) {
-            for (int q = 0; q < 4; ++q) {
-                if (s->ev_pool.empty()) {
-                    hipEvent_t e = nullptr;
-                    PW_HIP_TRY(hipEventCreate(&e));
-                    s->ev_pool.push_back(e);
+        const int nz = c.nz;
+        // one step kernel launch, bracketed by events when profiling
+        auto step_kernel = [&](int kind, int zb, int ze) -> pw_status {
+            if (ze <= zb) return PW_OK;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (s->profiling) {
+                for (hipEvent_t* e : {&e0, &e1}) {
+                    if (s->ev_pool.empty()) {
+                        hipEvent_t ne = nullptr;
+                        PW_HIP_TRY(hipEventCreate(&ne));
+                        s->ev_pool.push_back(ne);
+                    }
+                    *e = s->ev_pool.back();
+                    s->ev_pool.pop_back();
+                    s->ev_pending.push_back(*e);
                 }
-                ev[q] = s->ev_pool.back();
-                s->ev_pool.pop_back();
-                s->ev_pending.push_back(ev[q]);
+                s->ev_kind.push_back((char)kind);
+                PW_HIP_TRY(hipEventRecord(e0, s->stream));
+            }
+            PW_HIP_TRY(kind == 0 ? launch_velocity(s->b, s->stream, zb, ze) : launch_stress(s->b, s->stream, zb, ze));
+            if (e1) PW_HIP_TRY(hipEventRecord(e1, s->stream));
+            return PW_OK;
+        };
+        pw_status ks = PW_OK;
+        if (s->b.slab <= 0) {
+            if ((ks = step_kernel(0, 1, nz - 1)) != PW_OK) return ks;
+            if ((ks = step_kernel(1, 1, nz - 1)) != PW_OK) return ks;
+        } else {
+            // z-slab interleave: velocity on [zv, ve), then stress on the planes
+            // whose inputs are complete.  Stress at plane z needs the new velocity at
+            // z+1, and velocity at plane z still needs the old stress at z-1, so the
+            // stress pass stays two planes behind until the velocity pass is done.
+            // The planes just written stay in the memory-side cache for the stress
+            // pass that re-reads them.
+            int zv = 1, zt = 1;
+            while (zv < nz - 1) {
+                const int ve = std::min(zv + s->b.slab, nz - 1);
+                if ((ks = step_kernel(0, zv, ve)) != PW_OK) return ks;
+                zv = ve;
+                const int te = ve == nz - 1 ? nz - 1 : ve - 1;
+                if (te > zt) {
+                    if ((ks = step_kernel(1, zt, te)) != PW_OK) return ks;
+                    zt = te;
+                }
             }
         }
-        if (ev[0]) PW_HIP_TRY(hipEventRecord(ev[0], s->stream));
-        PW_HIP_TRY(launch_velocity(s->b, s->stream));
-        if (ev[1]) PW_HIP_TRY(hipEventRecord(ev[1], s->stream));
-        if (ev[2]) PW_HIP_TRY(hipEventRecord(ev[2], s->stream));
-        PW_HIP_TRY(launch_stress(s->b, s->stream));
-        if (ev[3]) PW_HIP_TRY(hipEventRecord(ev[3], s->stream));
+        if (s->profiling) s->prof_steps++;
         const float ss = scaled ? s->stress_base / s->dyn_scale : 1.0f;
         const float sv = scaled ? s->vel_base / s->dyn_scale : 1.0f;
         if (s->nrec > 0)
@@ -739,7 +775,9 @@ extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
     for (int ax = 0; ax < 3; ++ax) f += (g.lo[ax] + g.hi[ax]) / n3[ax];
     // velocity: reads tau x6 + p, read-modify-writes vu x3 + vw x3 (19 touches);
     // stress: reads vu x3 + vw x3, read-modify-writes tau x6 + p (20 touches).
-    const int cv = is_scaled(s->prec) ? 15 : 10, ct = 7;
+    // coefficient arrays: scaled form 15 (velocity) + 7 (stress) FP16 arrays;
+    // physical form 4 (rho, rhof, C1, C2) + 5 (C, M, H, H-2mu, mu) cell-centred arrays
+    const int cv = is_scaled(s->prec) ? 15 : 4, ct = is_scaled(s->prec) ? 7 : 5;
     out->bytes_velocity = 19.0 * state_bytes(s->prec) + cv * coef_bytes(s->prec) + 8.0 * f * mem_bytes(s->prec);
     out->bytes_stress = 20.0 * state_bytes(s->prec) + ct * coef_bytes(s->prec) + 8.0 * f * mem_bytes(s->prec);
     out->bytes_per_cell_step = out->bytes_velocity + out->bytes_stress;
@@ -768,18 +806,17 @@ extern "C" pw_status pw_get_kernel_times(pw_solver* s, double* ms_velocity, doub
     PW_HIP_TRY(hipSetDevice(s->cfg.device));
     PW_HIP_TRY(hipStreamSynchronize(s->stream));
     double tv = 0.0, tt = 0.0;
-    const size_t nstep = s->ev_pending.size() / 4;
-    for (size_t k = 0; k < nstep; ++k) {
-        float a = 0.0f, b = 0.0f;
-        PW_HIP_TRY(hipEventElapsedTime(&a, s->ev_pending[4 * k], s->ev_pending[4 * k + 1]));
-        PW_HIP_TRY(hipEventElapsedTime(&b, s->ev_pending[4 * k + 2], s->ev_pending[4 * k + 3]));
-        tv += a;
-        tt += b;
+    for (size_t k = 0; k < s->ev_kind.size(); ++k) {
+        float a = 0.0f;
+        PW_HIP_TRY(hipEventElapsedTime(&a, s->ev_pending[2 * k], s->ev_pending[2 * k + 1]));
+        (s->ev_kind[k] == 0 ? tv : tt) += a;
     }
     for (hipEvent_t e : s->ev_pending) s->ev_pool.push_back(e);
     s->ev_pending.clear();
+    s->ev_kind.clear();
     if (ms_velocity) *ms_velocity = tv;
     if (ms_stress) *ms_stress = tt;
-    if (launches) *launches = (long long)nstep;
+    if (launches) *launches = s->prof_steps;
+    s->prof_steps = 0;
     return PW_OK;
 }

@@ -10,14 +10,13 @@
 
 namespace pw {
 
-// Coefficient slots.  "physical" form (FP32 / FP64 storage): the reference's
-// Float arrays with the Crank-Nicolson pair folded to CC2 and 1/CC1
-// (Float/main.cu:106-113, 160-185).  "scaled" form (comp16 / bf16): the
+// Coefficient slots.  "physical" form (FP32 / FP64 storage): the nine
+// cell-centred medium fields on the padded grid (the reference's *_ext arrays
+// after extmodel); the kernels form the staggered averages and the
+// Crank-Nicolson pair on the fly with the reference's arithmetic
+// (Float/main.cu:106-113, 473-513).  "scaled" form (comp16 / bf16): the
 // reference's Half FP16 arrays (Half/main.cu:845-1045).
-enum PhysCoef {
-    PC_RIX = 0, PC_RIY, PC_RIZ, PC_RHOF, PC_Q2X, PC_Q2Y, PC_Q2Z, PC_U1X, PC_U1Y, PC_U1Z,
-    PC_C, PC_M, PC_HH, PC_H2U, PC_MUXY, PC_MUXZ, PC_MUYZ, PC_N
-};
+enum PhysCoef { PC_RHO = 0, PC_RHOF, PC_C1, PC_C2, PC_C, PC_M, PC_HH, PC_H2U, PC_MU, PC_N };
 enum ScaledCoef {
     SC_RITX = 0, SC_RITY, SC_RITZ, SC_FRX, SC_FRY, SC_FRZ, SC_AWX, SC_AWY, SC_AWZ,
     SC_BPX, SC_BPY, SC_BPZ, SC_BTX, SC_BTY, SC_BTZ, SC_HH, SC_H2U, SC_C, SC_M,
@@ -44,10 +43,13 @@ struct DevBuffers {
     void* tab[3];    // per axis: n entries of {c_full, e_full, c_half, e_half} (float or double)
     double dt, h;
     int zc;          // z cells per block in the step kernels
+    int xcd_map;     // 1: XCD-aware block -> tile mapping in the step kernels
+    int slab;        // >0: interleave velocity/stress launches over z-slabs of this many planes
 };
 
-hipError_t launch_velocity(const DevBuffers& b, hipStream_t st);
-hipError_t launch_stress(const DevBuffers& b, hipStream_t st);
+// Step kernels over the planes [zb, ze) (the updated planes are 1 .. nz-2).
+hipError_t launch_velocity(const DevBuffers& b, hipStream_t st, int zb, int ze);
+hipError_t launch_stress(const DevBuffers& b, hipStream_t st, int zb, int ze);
 hipError_t launch_source(const DevBuffers& b, hipStream_t st, unsigned off, const double w[4], double amp);
 hipError_t launch_gather(const DevBuffers& b, hipStream_t st, const int* rfield, const unsigned* roff, int nrec,
                          float* seis, int nt, int it, float s_stress, float s_vel);

@@ -133,7 +133,10 @@ template <class P> struct StepArgs {
     unsigned plane;
     int lo[3], hi[3];
     int zc;
-    typename P::R dt, inv_h;
+    int gx, gy, nblocks;   // tile grid (x tiles, y tiles) and total blocks of the 1-D launch
+    int zb, ze;            // planes [zb, ze) updated by this launch
+    int xcd_map;           // 1: XCD-aware block -> tile mapping
+    typename P::R dt, h;
 };
 
 template <class P> __host__ typename P::Field make_field(const DevBuffers& b, int k);
@@ -157,8 +160,14 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     a.nx = b.g.nx; a.ny = b.g.ny; a.nz = b.g.nz; a.pitch = b.g.pitch;
     a.plane = (unsigned)b.g.plane;
     a.zc = b.zc;
+    a.gx = (b.g.nx + 63) / 64;
+    a.gy = (b.g.ny + 3) / 4;
+    a.zb = 1;
+    a.ze = b.g.nz - 1;
+    a.nblocks = a.gx * a.gy * ((b.g.nz - 2 + b.zc - 1) / b.zc);
+    a.xcd_map = b.xcd_map;
     a.dt = (R)b.dt;
-    a.inv_h = (R)(1.0 / b.h);
+    a.h = (R)b.h;
     return a;
 }
 
@@ -168,6 +177,29 @@ __device__ __forceinline__ int slab_index(int i, int n, int lo, int hi)
     if (i < lo) return i;
     if (i >= n - hi) return i - (n - hi) + lo;
     return -1;
+}
+
+// Block -> (x tile, y tile, z chunk).  The hardware deals workgroups to the 8
+// XCDs round-robin (block L runs on XCD L % 8), each XCD with its own L2.  With
+// the XCD map, XCD k gets one contiguous run of tiles ordered x fastest, then y,
+// then z, so the x and y neighbour tiles whose halo rows a tile re-reads were
+// just fetched into the same L2.
+struct Tile { int bx, by, bz; };
+__device__ __forceinline__ Tile tile_of_block(int gx, int gy, int nblocks, int xcd_map)
+{
+    const unsigned L = blockIdx.x;
+    unsigned t = L;
+    if (xcd_map) {
+        const unsigned q = (unsigned)nblocks / 8u, r = (unsigned)nblocks % 8u;
+        const unsigned k = L % 8u, j = L / 8u;
+        t = k * q + (k < r ? k : r) + j;
+    }
+    Tile tl;
+    tl.bx = (int)(t % (unsigned)gx);
+    const unsigned row = t / (unsigned)gx;
+    tl.by = (int)(row % (unsigned)gy);
+    tl.bz = (int)(row / (unsigned)gy);
+    return tl;
 }
 
 // One CPML term in single-variable form (see header comment).
@@ -182,21 +214,50 @@ __device__ __forceinline__ typename P::R cpml(typename P::Mem* phi, unsigned k, 
 }
 
 // --------------------------------------------------------------------------
+// Reference arithmetic of the staggered medium (Float/main.cu:473-513) and of
+// the Crank-Nicolson pair (Float/main.cu:106-113), formed on the fly from the
+// cell-centred fields so that FP32 runs see the reference's exact coefficients.
+// --------------------------------------------------------------------------
+// rho_temp = 2 / (rho_i + rho_j)  (float in the reference)
+template <class R> __device__ __forceinline__ R inv_mean(R a, R b) { return (R)2 / (a + b); }
+// C1x = 0.5 * (a + b): the sum is formed in the storage type, the halving is exact
+template <class R> __device__ __forceinline__ R half_sum(R a, R b) { return (a + b) * (R)0.5; }
+// mu_xy = 1 / (0.25 * (1/a + 1/b + 1/c + 1/d)), 0 if any is 0: reciprocals and
+// their sum in the storage type, the last two operations in double
+template <class R> __device__ __forceinline__ R harm4(R a, R b, R c, R d)
+{
+    if (a == (R)0 || b == (R)0 || c == (R)0 || d == (R)0) return (R)0;
+    const R s = (R)1 / a + (R)1 / b + (R)1 / c + (R)1 / d;
+    return (R)(1.0 / (0.25 * (double)s));
+}
+// CC1 = 0.5*C1*DT + C2 - rhof^2 rho^-1, CC2 = C2 - rhof^2 rho^-1 - 0.5*C1*DT:
+// the reference's double literal 0.5 makes these double expressions
+template <class R>
+__device__ __forceinline__ void cn_pair(R c1, R c2, R rf, R ri, R dt, R& cc1, R& cc2)
+{
+    const double q = (double)(rf * rf * ri);
+    const double h = 0.5 * (double)c1 * (double)dt;
+    cc1 = (R)(h + (double)c2 - q);
+    cc2 = (R)((double)c2 - q - h);
+}
+
+// --------------------------------------------------------------------------
 // Velocity kernel: Float/main.cu:64-192 (physical) / Half/main.cu:156-268 (scaled)
 // --------------------------------------------------------------------------
 template <class P>
 __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
 {
     using R = typename P::R;
-    const int ix = blockIdx.x * 64 + threadIdx.x;
-    const int iy = blockIdx.y * 4 + threadIdx.y;
+    const Tile tl = tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map);
+    const int ix = tl.bx * 64 + threadIdx.x;
+    const int iy = tl.by * 4 + threadIdx.y;
     if (ix < 1 || ix > a.nx - 2 || iy < 1 || iy > a.ny - 2) return;
-    const int z0 = 1 + blockIdx.z * a.zc;
-    const int z1 = min(z0 + a.zc, a.nz - 1);
+    const int z0 = a.zb + tl.bz * a.zc;
+    const int z1 = min(z0 + a.zc, a.ze);
     if (z0 >= z1) return;
     const unsigned PL = a.plane, W = (unsigned)a.pitch;
     unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z0 * PL;
-    const R ih = a.inv_h;
+    const R H = a.h, dt = a.dt;
 
     const int xs = slab_index(ix, a.nx, a.lo[0], a.hi[0]);
     const int ys = slab_index(iy, a.ny, a.lo[1], a.hi[1]);
@@ -209,6 +270,13 @@ __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
     R tyz_m = P::ld(a.f[PW_TYZ], o - PL);
     R tzz_c = P::ld(a.f[PW_TZZ], o);
     R p_c = P::ld(a.f[PW_P], o);
+    // physical form: cell-centred rho, C1, C2 of this plane, carried up the column
+    R rho_c = 0, c1_c = 0, c2_c = 0;
+    if constexpr (!P::scaled) {
+        rho_c = P::ldc(a.c[PC_RHO], o);
+        c1_c = P::ldc(a.c[PC_C1], o);
+        c2_c = P::ldc(a.c[PC_C2], o);
+    }
 
     for (int iz = z0; iz < z1; ++iz, o += PL) {
         const R txz_c = P::ld(a.f[PW_TXZ], o);
@@ -222,9 +290,10 @@ __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
         const R tyz_h = P::ld(a.f[PW_TYZ], o - W);
         const R p_r = P::ld(a.f[PW_P], o + 1), p_q = P::ld(a.f[PW_P], o + W);
 
-        R x1 = (txx_r - txx_c) * ih, x2 = (txy_c - txy_h) * ih, x3 = (txz_c - txz_m) * ih, s1 = (p_r - p_c) * ih;
-        R y1 = (tyy_q - tyy_c) * ih, y2 = (txy_c - txy_l) * ih, y3 = (tyz_c - tyz_m) * ih, s2 = (p_q - p_c) * ih;
-        R z1 = (tzz_n - tzz_c) * ih, z2 = (txz_c - txz_l) * ih, z3 = (tyz_c - tyz_h) * ih, s3 = (p_n - p_c) * ih;
+        // Float/main.cu:115-128
+        R x1 = (txx_r - txx_c) / H, x2 = (txy_c - txy_h) / H, x3 = (txz_c - txz_m) / H, s1 = (p_r - p_c) / H;
+        R y1 = (tyy_q - tyy_c) / H, y2 = (txy_c - txy_l) / H, y3 = (tyz_c - tyz_m) / H, s2 = (p_q - p_c) / H;
+        R z1 = (tzz_n - tzz_c) / H, z2 = (txz_c - txz_l) / H, z3 = (tyz_c - tyz_h) / H, s3 = (p_n - p_c) / H;
 
         if (xs >= 0) {
             const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs;
@@ -254,19 +323,30 @@ __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
         const R wx0 = P::ld(a.f[PW_VWX], o), wy0 = P::ld(a.f[PW_VWY], o), wz0 = P::ld(a.f[PW_VWZ], o);
         R wx1, wy1, wz1, ux1, uy1, uz1;
         if constexpr (!P::scaled) {
-            // vw = (vw*CC2 - DT*(rhof*rho^-1*S + dp)) / CC1, 0 where C1 == 0;
-            // vu += DT*rho^-1*S - rhof*rho^-1*(vw_new - vw_old)
+            const R rho_n = P::ldc(a.c[PC_RHO], o + PL), rho_x = P::ldc(a.c[PC_RHO], o + 1),
+                    rho_y = P::ldc(a.c[PC_RHO], o + W);
+            const R c1_n = P::ldc(a.c[PC_C1], o + PL), c1_x = P::ldc(a.c[PC_C1], o + 1),
+                    c1_y = P::ldc(a.c[PC_C1], o + W);
+            const R c2_n = P::ldc(a.c[PC_C2], o + PL), c2_x = P::ldc(a.c[PC_C2], o + 1),
+                    c2_y = P::ldc(a.c[PC_C2], o + W);
             const R rf = P::ldc(a.c[PC_RHOF], o);
-            const R rix = P::ldc(a.c[PC_RIX], o), riy = P::ldc(a.c[PC_RIY], o), riz = P::ldc(a.c[PC_RIZ], o);
-            const R bx = rf * rix, by = rf * riy, bz = rf * riz;
-            wx1 = (wx0 * P::ldc(a.c[PC_Q2X], o) - a.dt * (bx * sx + s1)) * P::ldc(a.c[PC_U1X], o);
-            wy1 = (wy0 * P::ldc(a.c[PC_Q2Y], o) - a.dt * (by * sy + s2)) * P::ldc(a.c[PC_U1Y], o);
-            wz1 = (wz0 * P::ldc(a.c[PC_Q2Z], o) - a.dt * (bz * sz + s3)) * P::ldc(a.c[PC_U1Z], o);
-            ux1 = P::ld(a.f[PW_VUX], o) + a.dt * rix * sx - bx * (wx1 - wx0);
-            uy1 = P::ld(a.f[PW_VUY], o) + a.dt * riy * sy - by * (wy1 - wy0);
-            uz1 = P::ld(a.f[PW_VUZ], o) + a.dt * riz * sz - bz * (wz1 - wz0);
+            const R rix = inv_mean(rho_c, rho_x), riy = inv_mean(rho_c, rho_y), riz = inv_mean(rho_c, rho_n);
+            const R c1x = half_sum(c1_c, c1_x), c1y = half_sum(c1_c, c1_y), c1z = half_sum(c1_c, c1_n);
+            const R c2x = half_sum(c2_c, c2_x), c2y = half_sum(c2_c, c2_y), c2z = half_sum(c2_c, c2_n);
+            R cc1x, cc2x, cc1y, cc2y, cc1z, cc2z;
+            cn_pair(c1x, c2x, rf, rix, dt, cc1x, cc2x);
+            cn_pair(c1y, c2y, rf, riy, dt, cc1y, cc2y);
+            cn_pair(c1z, c2z, rf, riz, dt, cc1z, cc2z);
+            // Float/main.cu:160-189
+            wx1 = c1x == (R)0 ? (R)0 : (wx0 * cc2x - dt * (rf * rix * sx + s1)) / cc1x;
+            wy1 = c1y == (R)0 ? (R)0 : (wy0 * cc2y - dt * (rf * riy * sy + s2)) / cc1y;
+            wz1 = c1z == (R)0 ? (R)0 : (wz0 * cc2z - dt * (rf * riz * sz + s3)) / cc1z;
+            ux1 = P::ld(a.f[PW_VUX], o) + dt * rix * sx - rf * rix * (wx1 - wx0);
+            uy1 = P::ld(a.f[PW_VUY], o) + dt * riy * sy - rf * riy * (wy1 - wy0);
+            uz1 = P::ld(a.f[PW_VUZ], o) + dt * riz * sz - rf * riz * (wz1 - wz0);
+            rho_c = rho_n; c1_c = c1_n; c2_c = c2_n;
         } else {
-            // vw = Aw*vw - Bt*S - Bp*dp (0 where Aw = Bp = Bt = 0);
+            // Half/main.cu:243-262: vw = Aw*vw - Bt*S - Bp*dp (0 where Aw = Bp = Bt = 0);
             // vu += rho~^-1*S - rhof*rho^-1*(vw_new - vw_old)
             const R awx = P::ldc(a.c[SC_AWX], o), bpx = P::ldc(a.c[SC_BPX], o), btx = P::ldc(a.c[SC_BTX], o);
             const R awy = P::ldc(a.c[SC_AWY], o), bpy = P::ldc(a.c[SC_BPY], o), bty = P::ldc(a.c[SC_BTY], o);
@@ -292,15 +372,16 @@ template <class P>
 __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
 {
     using R = typename P::R;
-    const int ix = blockIdx.x * 64 + threadIdx.x;
-    const int iy = blockIdx.y * 4 + threadIdx.y;
+    const Tile tl = tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map);
+    const int ix = tl.bx * 64 + threadIdx.x;
+    const int iy = tl.by * 4 + threadIdx.y;
     if (ix < 1 || ix > a.nx - 2 || iy < 1 || iy > a.ny - 2) return;
-    const int z0 = 1 + blockIdx.z * a.zc;
-    const int z1 = min(z0 + a.zc, a.nz - 1);
+    const int z0 = a.zb + tl.bz * a.zc;
+    const int z1 = min(z0 + a.zc, a.ze);
     if (z0 >= z1) return;
     const unsigned PL = a.plane, W = (unsigned)a.pitch;
     unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z0 * PL;
-    const R ih = a.inv_h;
+    const R H = a.h, dt = a.dt;
 
     const int xs = slab_index(ix, a.nx, a.lo[0], a.hi[0]);
     const int ys = slab_index(iy, a.ny, a.lo[1], a.hi[1]);
@@ -313,6 +394,13 @@ __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
     R vwz_m = P::ld(a.f[PW_VWZ], o - PL);
     R vux_c = P::ld(a.f[PW_VUX], o);
     R vuy_c = P::ld(a.f[PW_VUY], o);
+    // physical form: cell-centred mu at (x, y), (x+1, y), (x, y+1) of this plane
+    R mu_c = 0, mu_x = 0, mu_y = 0;
+    if constexpr (!P::scaled) {
+        mu_c = P::ldc(a.c[PC_MU], o);
+        mu_x = P::ldc(a.c[PC_MU], o + 1);
+        mu_y = P::ldc(a.c[PC_MU], o + W);
+    }
 
     for (int iz = z0; iz < z1; ++iz, o += PL) {
         const R vux_n = P::ld(a.f[PW_VUX], o + PL), vuy_n = P::ld(a.f[PW_VUY], o + PL);
@@ -323,11 +411,12 @@ __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
         const R vwx_c = P::ld(a.f[PW_VWX], o), vwx_l = P::ld(a.f[PW_VWX], o - 1);
         const R vwy_c = P::ld(a.f[PW_VWY], o), vwy_h = P::ld(a.f[PW_VWY], o - W);
 
-        R uxx = (vux_c - vux_l) * ih, uyy = (vuy_c - vuy_h) * ih, uzz = (vuz_c - vuz_m) * ih;
-        R wx = (vwx_c - vwx_l) * ih, wy = (vwy_c - vwy_h) * ih, wz = (vwz_c - vwz_m) * ih;
-        R uxy = (vux_q - vux_c) * ih, uyx = (vuy_r - vuy_c) * ih;
-        R uxz = (vux_n - vux_c) * ih, uzx = (vuz_r - vuz_c) * ih;
-        R uyz = (vuy_n - vuy_c) * ih, uzy = (vuz_q - vuz_c) * ih;
+        // Float/main.cu:234-249
+        R uxx = (vux_c - vux_l) / H, uyy = (vuy_c - vuy_h) / H, uzz = (vuz_c - vuz_m) / H;
+        R wx = (vwx_c - vwx_l) / H, wy = (vwy_c - vwy_h) / H, wz = (vwz_c - vwz_m) / H;
+        R uxy = (vux_q - vux_c) / H, uyx = (vuy_r - vuy_c) / H;
+        R uxz = (vux_n - vux_c) / H, uzx = (vuz_r - vuz_c) / H;
+        R uyz = (vuy_n - vuy_c) / H, uzy = (vuz_q - vuz_c) / H;
 
         if (xs >= 0) {
             const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs;
@@ -356,17 +445,25 @@ __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
         const R du = uxx + uyy + uzz, dw = wx + wy + wz;
         R p1, xx1, yy1, zz1, xy1, yz1, xz1;
         if constexpr (!P::scaled) {
+            const R mu_n = P::ldc(a.c[PC_MU], o + PL), mu_xn = P::ldc(a.c[PC_MU], o + PL + 1),
+                    mu_yn = P::ldc(a.c[PC_MU], o + PL + W), mu_xy = P::ldc(a.c[PC_MU], o + 1 + W);
+            // Float/main.cu:480-491
+            const R muxz = harm4(mu_c, mu_x, mu_n, mu_xn);
+            const R muyz = harm4(mu_c, mu_y, mu_n, mu_yn);
+            const R muxy = harm4(mu_c, mu_x, mu_y, mu_xy);
             const R C = P::ldc(a.c[PC_C], o), M = P::ldc(a.c[PC_M], o);
             const R HH = P::ldc(a.c[PC_HH], o), H2u = P::ldc(a.c[PC_H2U], o);
-            const R dt = a.dt;
+            // Float/main.cu:287-293
             p1 = P::ld(a.f[PW_P], o) - dt * (C * du + M * dw);
             xx1 = P::ld(a.f[PW_TXX], o) + dt * (H2u * (uyy + uzz) + HH * uxx + C * dw);
             yy1 = P::ld(a.f[PW_TYY], o) + dt * (H2u * (uxx + uzz) + HH * uyy + C * dw);
             zz1 = P::ld(a.f[PW_TZZ], o) + dt * (H2u * (uxx + uyy) + HH * uzz + C * dw);
-            xy1 = P::ld(a.f[PW_TXY], o) + P::ldc(a.c[PC_MUXY], o) * dt * (uxy + uyx);
-            yz1 = P::ld(a.f[PW_TYZ], o) + P::ldc(a.c[PC_MUYZ], o) * dt * (uyz + uzy);
-            xz1 = P::ld(a.f[PW_TXZ], o) + P::ldc(a.c[PC_MUXZ], o) * dt * (uxz + uzx);
+            xy1 = P::ld(a.f[PW_TXY], o) + muxy * dt * (uxy + uyx);
+            yz1 = P::ld(a.f[PW_TYZ], o) + muyz * dt * (uyz + uzy);
+            xz1 = P::ld(a.f[PW_TXZ], o) + muxz * dt * (uxz + uzx);
+            mu_c = mu_n; mu_x = mu_xn; mu_y = mu_yn;
         } else {
+            // Half/main.cu:362-370
             const R C = P::ldc(a.c[SC_C], o), M = P::ldc(a.c[SC_M], o);
             const R HH = P::ldc(a.c[SC_HH], o), H2u = P::ldc(a.c[SC_H2U], o);
             p1 = P::ld(a.f[PW_P], o) - C * du - M * dw;
@@ -489,10 +586,14 @@ __global__ void k_zero(uint4* __restrict__ p, unsigned long long n16)
 // --------------------------------------------------------------------------
 // Launchers
 // --------------------------------------------------------------------------
-static dim3 step_grid(const DevBuffers& b)
+// Restricts a step launch to planes [zb, ze).
+template <class P> static StepArgs<P> range_args(const DevBuffers& b, int zb, int ze)
 {
-    const int nzi = b.g.nz - 2;
-    return dim3((unsigned)((b.g.nx + 63) / 64), (unsigned)((b.g.ny + 3) / 4), (unsigned)((nzi + b.zc - 1) / b.zc));
+    StepArgs<P> a = make_args<P>(b);
+    a.zb = zb;
+    a.ze = ze;
+    a.nblocks = a.gx * a.gy * ((ze - zb + b.zc - 1) / b.zc);
+    return a;
 }
 
 template <class F> static hipError_t dispatch(int prec, F&& f)
@@ -506,20 +607,24 @@ template <class F> static hipError_t dispatch(int prec, F&& f)
     }
 }
 
-hipError_t launch_velocity(const DevBuffers& b, hipStream_t st)
+hipError_t launch_velocity(const DevBuffers& b, hipStream_t st, int zb, int ze)
 {
+    if (ze <= zb) return hipSuccess;
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
-        k_velocity<P><<<step_grid(b), dim3(64, 4), 0, st>>>(make_args<P>(b));
+        const StepArgs<P> a = range_args<P>(b, zb, ze);
+        k_velocity<P><<<a.nblocks, dim3(64, 4), 0, st>>>(a);
         return hipGetLastError();
     });
 }
 
-hipError_t launch_stress(const DevBuffers& b, hipStream_t st)
+hipError_t launch_stress(const DevBuffers& b, hipStream_t st, int zb, int ze)
 {
+    if (ze <= zb) return hipSuccess;
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
-        k_stress<P><<<step_grid(b), dim3(64, 4), 0, st>>>(make_args<P>(b));
+        const StepArgs<P> a = range_args<P>(b, zb, ze);
+        k_stress<P><<<a.nblocks, dim3(64, 4), 0, st>>>(a);
         return hipGetLastError();
     });
 }

@@ -142,67 +142,89 @@ struct CoefOut {
     long long plane;
 };
 
-// Physical form (FP32 / FP64 storage): rho^-1 (x,y,z), rhof, CC2 (x,y,z),
-// 1/CC1 (x,y,z; 0 where C1 == 0), C, M, H, H-2mu, mu_xy, mu_xz, mu_yz.
+// Physical form (FP32 / FP64 storage): the nine cell-centred fields on the padded
+// grid (extmodel, Float/main.cu:1662-1794), rounded once to the storage type.
 template <class T>
 __global__ void k_coef_physical(BaseView v, int nx, int ny, int nz, double dt, CoefOut out)
 {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
     if (ix >= nx) return;
     const size_t o = (size_t)ix + (size_t)iy * out.pitch + (size_t)iz * out.plane;
-    const Stag s = staggered(v, ix, iy, iz, nx, ny, nz);
-    const double rhof = v.at(PW_RHOF, ix, iy, iz);
-    const double ri[3] = {s.rix, s.riy, s.riz}, c1[3] = {s.c1x, s.c1y, s.c1z}, c2[3] = {s.c2x, s.c2y, s.c2z};
     T** c = (T**)out.c;
-    for (int d = 0; d < 3; ++d) {
-        const double Q = c2[d] - rhof * rhof * ri[d];
-        c[PC_RIX + d][o] = to_store<T>(ri[d]);
-        c[PC_Q2X + d][o] = to_store<T>(Q - 0.5 * c1[d] * dt);
-        c[PC_U1X + d][o] = to_store<T>(c1[d] == 0.0 ? 0.0 : 1.0 / (Q + 0.5 * c1[d] * dt));
-    }
-    c[PC_RHOF][o] = to_store<T>(rhof);
+    c[PC_RHO][o] = to_store<T>(v.at(PW_RHO, ix, iy, iz));
+    c[PC_RHOF][o] = to_store<T>(v.at(PW_RHOF, ix, iy, iz));
+    c[PC_C1][o] = to_store<T>(v.at(PW_C1, ix, iy, iz));
+    c[PC_C2][o] = to_store<T>(v.at(PW_C2, ix, iy, iz));
     c[PC_C][o] = to_store<T>(v.at(PW_CC, ix, iy, iz));
     c[PC_M][o] = to_store<T>(v.at(PW_MM, ix, iy, iz));
     c[PC_HH][o] = to_store<T>(v.at(PW_HH, ix, iy, iz));
     c[PC_H2U][o] = to_store<T>(v.at(PW_H2U, ix, iy, iz));
-    c[PC_MUXY][o] = to_store<T>(s.muxy);
-    c[PC_MUXZ][o] = to_store<T>(s.muxz);
-    c[PC_MUYZ][o] = to_store<T>(s.muyz);
+    c[PC_MU][o] = to_store<T>(v.at(PW_MU, ix, iy, iz));
+    (void)ny; (void)nz; (void)dt;
 }
 
 // Scaled form (comp16 / bf16), Half/main.cu:967-1044: FP16 arrays prescaled by
-// dt and the velocity exponent 2^em.
-__global__ void k_coef_scaled(BaseView v, int nx, int ny, int nz, double dt, double sm, double ism, CoefOut out)
+// dt and the velocity exponent 2^em.  Evaluated with the reference's own
+// arithmetic (float staggered averages of Float/main.cu:480-509 with their double
+// steps, float Half formulas, no contraction) so that every FP16 coefficient is
+// the one the reference stores: in stiff media Aw is close to -1 and one FP16 ulp
+// of Aw is a large part of 1 + Aw.
+__device__ __forceinline__ float harm4_ref(float a, float b, float c, float d)
 {
+#pragma clang fp contract(off)
+    if (a == 0 || b == 0 || c == 0 || d == 0) return 0.0f;
+    return (float)(1 / (0.25 * (double)(1 / a + 1 / b + 1 / c + 1 / d)));
+}
+
+__global__ void k_coef_scaled(BaseView v, int nx, int ny, int nz, float DT, float sm, float ism, CoefOut out)
+{
+#pragma clang fp contract(off)
     const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
     if (ix >= nx) return;
     const size_t o = (size_t)ix + (size_t)iy * out.pitch + (size_t)iz * out.plane;
-    const Stag s = staggered(v, ix, iy, iz, nx, ny, nz);
-    const double rhof = v.at(PW_RHOF, ix, iy, iz);
-    const double ri[3] = {s.rix, s.riy, s.riz}, c1[3] = {s.c1x, s.c1y, s.c1z}, c2[3] = {s.c2x, s.c2y, s.c2z};
     f16** c = (f16**)out.c;
-    for (int d = 0; d < 3; ++d) {
-        const double Q = c2[d] - rhof * rhof * ri[d];
-        const double den = Q + 0.5 * c1[d] * dt;
-        double aw = 0.0, bp = 0.0, bt = 0.0;
-        if (c1[d] != 0.0 && fabs(Q) > 1.0e-12 && fabs(den) > 1.0e-12) {
-            aw = (Q - 0.5 * c1[d] * dt) / den;
-            bp = ism * dt / den;
-            bt = ism * dt * rhof * ri[d] / den;
-        }
-        c[SC_RITX + d][o] = to_store<f16>(ism * dt * ri[d]);
-        c[SC_FRX + d][o] = to_store<f16>(rhof * ri[d]);
-        c[SC_AWX + d][o] = to_store<f16>(aw);
-        c[SC_BPX + d][o] = to_store<f16>(bp);
-        c[SC_BTX + d][o] = to_store<f16>(bt);
+    auto F = [&](int f, int dx, int dy, int dz) { return (float)v.at(f, ix + dx, iy + dy, iz + dz); };
+    const float rhof0 = F(PW_RHOF, 0, 0, 0);
+    float ri[3] = {0, 0, 0}, c1[3] = {0, 0, 0}, c2[3] = {0, 0, 0}, muxz = 0, muyz = 0, muxy = 0;
+    if (ix >= 1 && iy >= 1 && iz >= 1 && ix <= nx - 2 && iy <= ny - 2 && iz <= nz - 2) {
+        const float r0 = F(PW_RHO, 0, 0, 0), a0 = F(PW_C1, 0, 0, 0), b0 = F(PW_C2, 0, 0, 0);
+        ri[0] = 2 / (r0 + F(PW_RHO, 1, 0, 0));
+        ri[1] = 2 / (r0 + F(PW_RHO, 0, 1, 0));
+        ri[2] = 2 / (r0 + F(PW_RHO, 0, 0, 1));
+        c1[0] = (float)(0.5 * (double)(a0 + F(PW_C1, 1, 0, 0)));
+        c1[1] = (float)(0.5 * (double)(a0 + F(PW_C1, 0, 1, 0)));
+        c1[2] = (float)(0.5 * (double)(a0 + F(PW_C1, 0, 0, 1)));
+        c2[0] = (float)(0.5 * (double)(b0 + F(PW_C2, 1, 0, 0)));
+        c2[1] = (float)(0.5 * (double)(b0 + F(PW_C2, 0, 1, 0)));
+        c2[2] = (float)(0.5 * (double)(b0 + F(PW_C2, 0, 0, 1)));
+        const float m0 = F(PW_MU, 0, 0, 0);
+        muxz = harm4_ref(m0, F(PW_MU, 1, 0, 0), F(PW_MU, 0, 0, 1), F(PW_MU, 1, 0, 1));
+        muyz = harm4_ref(m0, F(PW_MU, 0, 1, 0), F(PW_MU, 0, 0, 1), F(PW_MU, 0, 1, 1));
+        muxy = harm4_ref(m0, F(PW_MU, 1, 0, 0), F(PW_MU, 0, 1, 0), F(PW_MU, 1, 1, 0));
     }
-    c[SC_HH][o] = to_store<f16>(sm * dt * v.at(PW_HH, ix, iy, iz));
-    c[SC_H2U][o] = to_store<f16>(sm * dt * v.at(PW_H2U, ix, iy, iz));
-    c[SC_C][o] = to_store<f16>(sm * dt * v.at(PW_CC, ix, iy, iz));
-    c[SC_M][o] = to_store<f16>(sm * dt * v.at(PW_MM, ix, iy, iz));
-    c[SC_MUXZ][o] = to_store<f16>(sm * dt * s.muxz);
-    c[SC_MUXY][o] = to_store<f16>(sm * dt * s.muxy);
-    c[SC_MUYZ][o] = to_store<f16>(sm * dt * s.muyz);
+    const float Q_EPS = 1.0e-12f;
+    for (int d = 0; d < 3; ++d) {
+        const float Q = c2[d] - rhof0 * rhof0 * ri[d];
+        const float den = Q + 0.5f * c1[d] * DT;
+        float aw = 0.0f, bp = 0.0f, bt = 0.0f;
+        if (c1[d] != 0.0f && fabsf(Q) > Q_EPS && fabsf(den) > Q_EPS) {
+            aw = (Q - 0.5f * c1[d] * DT) / den;
+            bp = ism * DT / den;
+            bt = ism * DT * rhof0 * ri[d] / den;
+        }
+        c[SC_RITX + d][o] = (f16)(ism * DT * ri[d]);
+        c[SC_FRX + d][o] = (f16)(rhof0 * ri[d]);
+        c[SC_AWX + d][o] = (f16)aw;
+        c[SC_BPX + d][o] = (f16)bp;
+        c[SC_BTX + d][o] = (f16)bt;
+    }
+    c[SC_HH][o] = (f16)(sm * DT * F(PW_HH, 0, 0, 0));
+    c[SC_H2U][o] = (f16)(sm * DT * F(PW_H2U, 0, 0, 0));
+    c[SC_C][o] = (f16)(sm * DT * F(PW_CC, 0, 0, 0));
+    c[SC_M][o] = (f16)(sm * DT * F(PW_MM, 0, 0, 0));
+    c[SC_MUXZ][o] = (f16)(sm * DT * muxz);
+    c[SC_MUXY][o] = (f16)(sm * DT * muxy);
+    c[SC_MUYZ][o] = (f16)(sm * DT * muyz);
 }
 
 static BaseView make_view(const MediumArgs& m)
@@ -235,7 +257,8 @@ hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double 
     case PW_FP64: hipLaunchKernelGGL(k_coef_physical<double>, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, out); break;
     case PW_COMP16:
     case PW_BF16:
-        hipLaunchKernelGGL(k_coef_scaled, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, scale_m, inv_scale_m, out);
+        hipLaunchKernelGGL(k_coef_scaled, grid, block, 0, st, v, m.nx, m.ny, m.nz, (float)m.dt, (float)scale_m,
+                           (float)inv_scale_m, out);
         break;
     default: return hipErrorInvalidValue;
     }

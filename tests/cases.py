@@ -53,16 +53,19 @@ def make_solver(grid, nt, precision, base=None, spec=None):
 
 
 def receivers(grid, n_line=None):
-    """Vertical line through the source for five channels, plus a horizontal line."""
+    """The reference's five channels on the vertical line through the source
+    (Float/main.cu:1085-1093), plus a horizontal line off the source's symmetry
+    planes (iy = sy + 3) for the shear and transverse channels: on a symmetry
+    plane those are pure rounding noise and carry no parity information."""
     nx, ny, nz = grid["nx"], grid["ny"], grid["nz"]
     sx, sy, sz = grid["src"]
     recs = []
     for f in ("txx", "tzz", "vux", "vwx", "p"):
         for iz in range(nz):
             recs.append((f, sx + sy * nx + iz * nx * ny))
-    for f in ("vuz", "tyz", "txy"):
+    for f in ("vuy", "vuz", "vwz", "tyy", "tyz", "txy", "txz"):
         for ix in range(0, nx, 3):
-            recs.append((f, ix + sy * nx + (sz + 2) * nx * ny))
+            recs.append((f, ix + (sy + 3) * nx + (sz + 2) * nx * ny))
     return recs
 
 
