@@ -172,8 +172,10 @@ typedef struct pw_info {
     int precision;
     int em, epml;               /* comp16/bf16 exponents (Half/main.cu:793-843) */
     double bytes_per_cell_step; /* algorithmic HBM bytes per cell-step, this grid */
-    double bytes_velocity;      /* ... of which the velocity kernel (per cell) */
-    double bytes_stress;        /* ... of which the stress kernel (per cell) */
+    double bytes_velocity;      /* ... of which the velocity kernel (per cell); the
+                                   fused step kernel when fused != 0 */
+    double bytes_stress;        /* ... of which the stress kernel (0 when fused) */
+    int fused;                  /* 1: one fused launch per step (opt-in, PW_FUSED=1) */
     double vp_max;              /* fastest P velocity of the medium [m/s] */
     double courant;             /* vp_max * dt / h * sqrt(3) */
     long long device_bytes;     /* device memory held by the handle */

@@ -44,6 +44,7 @@ class Config(C.Structure):
 class Info(C.Structure):
     _fields_ = [("precision", C.c_int), ("em", C.c_int), ("epml", C.c_int),
                 ("bytes_per_cell_step", C.c_double), ("bytes_velocity", C.c_double), ("bytes_stress", C.c_double),
+                ("fused", C.c_int),
                 ("vp_max", C.c_double), ("courant", C.c_double),
                 ("device_bytes", C.c_longlong), ("slab_lo", C.c_int * 3), ("slab_hi", C.c_int * 3)]
 
@@ -310,7 +311,7 @@ class Solver:
         _check(lib().pw_get_info(self._h, C.byref(i)))
         return {"precision": self.precision, "em": i.em, "epml": i.epml,
                 "bytes_per_cell_step": i.bytes_per_cell_step, "bytes_velocity": i.bytes_velocity,
-                "bytes_stress": i.bytes_stress, "vp_max": i.vp_max, "courant": i.courant,
+                "bytes_stress": i.bytes_stress, "fused": i.fused, "vp_max": i.vp_max, "courant": i.courant,
                 "device_bytes": i.device_bytes, "slab_lo": list(i.slab_lo), "slab_hi": list(i.slab_hi)}
 
 

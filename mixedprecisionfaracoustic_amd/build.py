@@ -27,6 +27,9 @@ def flags():
     return ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             # keep the compiler from emitting packed FP32 instructions
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            # multiply-adds fuse only inside one source expression, so the same
+            # formula gives the same bits in every kernel that spells it
+            "-ffp-contract=on",
             "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
             "-I", os.path.join(HERE, "..", "include")]
 

@@ -240,11 +240,18 @@ extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
         total += align256(bytes);
     };
     const bool c16 = s->prec == PW_COMP16;
-    for (int k = 0; k < PW_NFIELDS; ++k) take(cells * (c16 ? 2 : sb));
-    if (c16)
-        for (int k = 0; k < PW_NFIELDS; ++k) take(cells * 1);
-    for (int ax = 0; ax < 3; ++ax)
-        for (int k = 0; k < kMemPerAxis; ++k) take((size_t)g.slab_cells[ax] * mb);
+    // The fused one-launch step (double-buffered) is bit-identical to the two
+    // kernels but currently slower on MI355X (see the design notes); opt-in.
+    int fused = 0;
+    env_int("PW_FUSED", &fused);
+    const int nbuf = fused ? 2 : 1;  // the fused step reads time n and writes time n+1
+    for (int b2 = 0; b2 < nbuf; ++b2) {
+        for (int k = 0; k < PW_NFIELDS; ++k) take(cells * (c16 ? 2 : sb));
+        if (c16)
+            for (int k = 0; k < PW_NFIELDS; ++k) take(cells * 1);
+        for (int ax = 0; ax < 3; ++ax)
+            for (int k = 0; k < kMemPerAxis; ++k) take((size_t)g.slab_cells[ax] * mb);
+    }
     const size_t zero_bytes = total;
     const int nc = ncoef(s->prec);
     for (int k = 0; k < nc; ++k) take(cells * cb);
@@ -265,6 +272,11 @@ extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = c16 ? (int8_t*)(base + off[i++]) : nullptr;
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k < kMemPerAxis; ++k) s->b.mem[ax][k] = base + off[i++];
+    s->b.fused = fused ? 1 : 0;
+    for (int k = 0; k < PW_NFIELDS; ++k) s->b.fo[k] = fused ? base + off[i++] : nullptr;
+    for (int k = 0; k < PW_NFIELDS; ++k) s->b.loo[k] = (fused && c16) ? (int8_t*)(base + off[i++]) : nullptr;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int k = 0; k < kMemPerAxis; ++k) s->b.memo[ax][k] = fused ? base + off[i++] : nullptr;
     for (int k = 0; k < kMaxCoef; ++k) s->b.coef[k] = k < nc ? base + off[i++] : nullptr;
     for (int ax = 0; ax < 3; ++ax) s->b.tab[ax] = base + off[i++];
     s->b.prec = s->prec;
@@ -659,12 +671,23 @@ extern "C" pw_status pw_run(pw_solver* s, int nsteps)
                 s->ev_kind.push_back((char)kind);
                 PW_HIP_TRY(hipEventRecord(e0, s->stream));
             }
-            PW_HIP_TRY(kind == 0 ? launch_velocity(s->b, s->stream, zb, ze) : launch_stress(s->b, s->stream, zb, ze));
+            PW_HIP_TRY(kind == 2   ? launch_fused(s->b, s->stream)
+                       : kind == 0 ? launch_velocity(s->b, s->stream, zb, ze)
+                                   : launch_stress(s->b, s->stream, zb, ze));
             if (e1) PW_HIP_TRY(hipEventRecord(e1, s->stream));
             return PW_OK;
         };
         pw_status ks = PW_OK;
-        if (s->b.slab <= 0) {
+        if (s->b.fused) {
+            if ((ks = step_kernel(2, 1, nz - 1)) != PW_OK) return ks;
+            // time n+1 becomes the current state
+            for (int q = 0; q < PW_NFIELDS; ++q) {
+                std::swap(s->b.f[q], s->b.fo[q]);
+                std::swap(s->b.lo[q], s->b.loo[q]);
+            }
+            for (int ax = 0; ax < 3; ++ax)
+                for (int q = 0; q < kMemPerAxis; ++q) std::swap(s->b.mem[ax][q], s->b.memo[ax][q]);
+        } else if (s->b.slab <= 0) {
             if ((ks = step_kernel(0, 1, nz - 1)) != PW_OK) return ks;
             if ((ks = step_kernel(1, 1, nz - 1)) != PW_OK) return ks;
         } else {
@@ -781,6 +804,13 @@ extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
     out->bytes_velocity = 19.0 * state_bytes(s->prec) + cv * coef_bytes(s->prec) + 8.0 * f * mem_bytes(s->prec);
     out->bytes_stress = 20.0 * state_bytes(s->prec) + ct * coef_bytes(s->prec) + 8.0 * f * mem_bytes(s->prec);
     out->bytes_per_cell_step = out->bytes_velocity + out->bytes_stress;
+    out->fused = s->b.fused;
+    if (s->b.fused) {
+        // one launch: every wavefield read once and written once (26 touches)
+        out->bytes_velocity = 26.0 * state_bytes(s->prec) + (cv + ct) * coef_bytes(s->prec) + 16.0 * f * mem_bytes(s->prec);
+        out->bytes_stress = 0.0;
+        out->bytes_per_cell_step = out->bytes_velocity;
+    }
     out->vp_max = s->vp_max;
     out->courant = s->courant;
     out->device_bytes = (long long)s->pool_bytes;
@@ -809,7 +839,7 @@ extern "C" pw_status pw_get_kernel_times(pw_solver* s, double* ms_velocity, doub
     for (size_t k = 0; k < s->ev_kind.size(); ++k) {
         float a = 0.0f;
         PW_HIP_TRY(hipEventElapsedTime(&a, s->ev_pending[2 * k], s->ev_pending[2 * k + 1]));
-        (s->ev_kind[k] == 0 ? tv : tt) += a;
+        (s->ev_kind[k] == 1 ? tt : tv) += a;  // the fused kernel counts as velocity
     }
     for (hipEvent_t e : s->ev_pending) s->ev_pool.push_back(e);
     s->ev_pending.clear();

@@ -45,11 +45,18 @@ struct DevBuffers {
     int zc;          // z cells per block in the step kernels
     int xcd_map;     // 1: XCD-aware block -> tile mapping in the step kernels
     int slab;        // >0: interleave velocity/stress launches over z-slabs of this many planes
+    int fused;       // 1: one fused launch per step over double-buffered state
+    // write side of the fused step (time n+1); swapped with f/lo/mem after each step
+    void* fo[PW_NFIELDS];
+    int8_t* loo[PW_NFIELDS];
+    void* memo[3][kMemPerAxis];
 };
 
 // Step kernels over the planes [zb, ze) (the updated planes are 1 .. nz-2).
 hipError_t launch_velocity(const DevBuffers& b, hipStream_t st, int zb, int ze);
 hipError_t launch_stress(const DevBuffers& b, hipStream_t st, int zb, int ze);
+// One fused step (velocity then stress) reading f/lo/mem and writing fo/loo/memo.
+hipError_t launch_fused(const DevBuffers& b, hipStream_t st);
 hipError_t launch_source(const DevBuffers& b, hipStream_t st, unsigned off, const double w[4], double amp);
 hipError_t launch_gather(const DevBuffers& b, hipStream_t st, const int* rfield, const unsigned* roff, int nrec,
                          float* seis, int nt, int it, float s_stress, float s_vel);

@@ -48,23 +48,29 @@ __device__ __forceinline__ float c16_unit(float hf)
     const unsigned E = (__float_as_uint(a) >> 23) & 0xffu;
     return __uint_as_float((E - 18u) << 23);
 }
-__device__ __forceinline__ float c16_inv_unit(float hf)
+// The same unit from the FP16 bit pattern: biased FP16 exponent e5 (0 for
+// zero/subnormal) gives 2^(max(e5, 1) - 32), i.e. float bits (max(e5,1)+95) << 23
+// with the 127 bias: e5 = 1 -> 2^-32, the subnormal value.
+__device__ __forceinline__ float c16_unit_bits(unsigned short hb)
 {
-    const float a = __builtin_fabsf(hf);
-    if (a < 6.103515625e-5f) return 4294967296.0f;
-    const unsigned E = (__float_as_uint(a) >> 23) & 0xffu;
-    return __uint_as_float((272u - E) << 23);   // 2^(145-E)
+    const unsigned e5 = (hb >> 10) & 31u;
+    return __uint_as_float(((e5 > 1u ? e5 : 1u) + 94u) << 23);
+}
+__device__ __forceinline__ float c16_inv_unit_bits(unsigned short hb)
+{
+    const unsigned e5 = (hb >> 10) & 31u;
+    return __uint_as_float((160u - (e5 > 1u ? e5 : 1u)) << 23);
 }
 __device__ __forceinline__ float c16_decode(f16 hi, int8_t lo)
 {
-    const float hf = (float)hi;
-    return hf + (float)lo * c16_unit(hf);
+    return (float)hi + (float)lo * c16_unit_bits(__builtin_bit_cast(unsigned short, hi));
 }
 __device__ __forceinline__ void c16_encode(float v, f16& hi, int8_t& lo)
 {
     const f16 h = (f16)v;
     const float hf = (float)h;
-    const float scaled = (v - hf) * c16_inv_unit(hf);   // exact: power-of-two scale
+    // exact: the unit is a power of two
+    const float scaled = (v - hf) * c16_inv_unit_bits(__builtin_bit_cast(unsigned short, h));
     int q = (int)(scaled + (scaled >= 0.0f ? 0.5f : -0.5f));
     q = q > 127 ? 127 : (q < -127 ? -127 : q);
     hi = h;
@@ -85,6 +91,7 @@ struct PolF32 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v) { m[o] = v; }
+    static __device__ __forceinline__ R rt(R v) { return v; }  // value as stored
 };
 struct PolF64 {
     using R = double; using Coef = double; using Mem = double;
@@ -95,6 +102,7 @@ struct PolF64 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v) { m[o] = v; }
+    static __device__ __forceinline__ R rt(R v) { return v; }
 };
 struct PolC16 {
     using R = float; using Coef = f16; using Mem = f16;
@@ -112,6 +120,13 @@ struct PolC16 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return (float)c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return (float)m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v) { m[o] = (f16)v; }
+    static __device__ __forceinline__ R rt(R v)
+    {
+        f16 h;
+        int8_t l;
+        c16_encode(v, h, l);
+        return c16_decode(h, l);
+    }
 };
 struct PolB16 {
     using R = float; using Coef = f16; using Mem = bf16;
@@ -122,6 +137,7 @@ struct PolB16 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return (float)c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return (float)m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v) { m[o] = (bf16)v; }
+    static __device__ __forceinline__ R rt(R v) { return (float)(bf16)v; }
 };
 
 template <class P> struct StepArgs {
@@ -241,8 +257,21 @@ __device__ __forceinline__ void cn_pair(R c1, R c2, R rf, R ri, R dt, R& cc1, R&
     cc2 = (R)((double)c2 - q - h);
 }
 
+// Staggered difference divided by the grid step.  The FP32/FP64 forms divide
+// as the reference does; the FP16-stored forms multiply by 1/h (the rounding
+// difference is far below FP16 storage resolution).
+template <class P> struct Deriv {
+    typename P::R h, ih;
+    __device__ __forceinline__ explicit Deriv(typename P::R h_) : h(h_), ih((typename P::R)1 / h_) {}
+    __device__ __forceinline__ typename P::R operator()(typename P::R d) const
+    {
+        if constexpr (P::scaled) return d * ih;
+        else return d / h;
+    }
+};
+
 // --------------------------------------------------------------------------
-// Velocity kernel: Float/main.cu:64-192 (physical) / Half/main.cu:156-268 (scaled)
+// Velocity kernel: Float/main.cu:64-192 (physical), Half/main.cu:156-268 (scaled)
 // --------------------------------------------------------------------------
 template <class P>
 __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
@@ -257,7 +286,8 @@ __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
     if (z0 >= z1) return;
     const unsigned PL = a.plane, W = (unsigned)a.pitch;
     unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z0 * PL;
-    const R H = a.h, dt = a.dt;
+    const Deriv<P> D(a.h);
+    const R dt = a.dt;
 
     const int xs = slab_index(ix, a.nx, a.lo[0], a.hi[0]);
     const int ys = slab_index(iy, a.ny, a.lo[1], a.hi[1]);
@@ -291,9 +321,9 @@ __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
         const R p_r = P::ld(a.f[PW_P], o + 1), p_q = P::ld(a.f[PW_P], o + W);
 
         // Float/main.cu:115-128
-        R x1 = (txx_r - txx_c) / H, x2 = (txy_c - txy_h) / H, x3 = (txz_c - txz_m) / H, s1 = (p_r - p_c) / H;
-        R y1 = (tyy_q - tyy_c) / H, y2 = (txy_c - txy_l) / H, y3 = (tyz_c - tyz_m) / H, s2 = (p_q - p_c) / H;
-        R z1 = (tzz_n - tzz_c) / H, z2 = (txz_c - txz_l) / H, z3 = (tyz_c - tyz_h) / H, s3 = (p_n - p_c) / H;
+        R x1 = D(txx_r - txx_c), x2 = D(txy_c - txy_h), x3 = D(txz_c - txz_m), s1 = D(p_r - p_c);
+        R y1 = D(tyy_q - tyy_c), y2 = D(txy_c - txy_l), y3 = D(tyz_c - tyz_m), s2 = D(p_q - p_c);
+        R z1 = D(tzz_n - tzz_c), z2 = D(txz_c - txz_l), z3 = D(tyz_c - tyz_h), s3 = D(p_n - p_c);
 
         if (xs >= 0) {
             const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs;
@@ -366,7 +396,7 @@ __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
 }
 
 // --------------------------------------------------------------------------
-// Stress kernel: Float/main.cu:194-295 (physical) / Half/main.cu:270-371 (scaled)
+// Stress kernel: Float/main.cu:194-295 (physical), Half/main.cu:270-371 (scaled)
 // --------------------------------------------------------------------------
 template <class P>
 __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
@@ -381,7 +411,8 @@ __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
     if (z0 >= z1) return;
     const unsigned PL = a.plane, W = (unsigned)a.pitch;
     unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z0 * PL;
-    const R H = a.h, dt = a.dt;
+    const Deriv<P> D(a.h);
+    const R dt = a.dt;
 
     const int xs = slab_index(ix, a.nx, a.lo[0], a.hi[0]);
     const int ys = slab_index(iy, a.ny, a.lo[1], a.hi[1]);
@@ -412,11 +443,11 @@ __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
         const R vwy_c = P::ld(a.f[PW_VWY], o), vwy_h = P::ld(a.f[PW_VWY], o - W);
 
         // Float/main.cu:234-249
-        R uxx = (vux_c - vux_l) / H, uyy = (vuy_c - vuy_h) / H, uzz = (vuz_c - vuz_m) / H;
-        R wx = (vwx_c - vwx_l) / H, wy = (vwy_c - vwy_h) / H, wz = (vwz_c - vwz_m) / H;
-        R uxy = (vux_q - vux_c) / H, uyx = (vuy_r - vuy_c) / H;
-        R uxz = (vux_n - vux_c) / H, uzx = (vuz_r - vuz_c) / H;
-        R uyz = (vuy_n - vuy_c) / H, uzy = (vuz_q - vuz_c) / H;
+        R uxx = D(vux_c - vux_l), uyy = D(vuy_c - vuy_h), uzz = D(vuz_c - vuz_m);
+        R wx = D(vwx_c - vwx_l), wy = D(vwy_c - vwy_h), wz = D(vwz_c - vwz_m);
+        R uxy = D(vux_q - vux_c), uyx = D(vuy_r - vuy_c);
+        R uxz = D(vux_n - vux_c), uzx = D(vuz_r - vuz_c);
+        R uyz = D(vuy_n - vuy_c), uzy = D(vuz_q - vuz_c);
 
         if (xs >= 0) {
             const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs;
@@ -479,6 +510,282 @@ __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
         P::st(a.f[PW_TXY], o, xy1); P::st(a.f[PW_TYZ], o, yz1); P::st(a.f[PW_TXZ], o, xz1);
 
         vuz_m = vuz_c; vwz_m = vwz_c; vux_c = vux_n; vuy_c = vuy_n;
+    }
+}
+
+// --------------------------------------------------------------------------
+// Fused step kernel: velocity and stress update of one time step in one launch.
+//
+// The state is double-buffered (read time n from s.f, write time n+1 to g), so
+// each wavefield is read once and written once per step instead of read twice
+// (39 -> 26 touches).  A block owns a 64 x 4 column tile and marches up its z
+// chunk: at plane k it computes the new velocities of plane k+1 (tile cells: all
+// three axes, stored; the one-cell ring around the tile: only the axis
+// components the stress stencil reads, not stored), then the stress update of
+// plane k from registers (own column, planes k-1..k+1) and LDS (x/y neighbours).
+// Ring cells belong to neighbouring tiles and are recomputed here from the same
+// time-n inputs, so they come out bit-identical to the owner's values.  The CPML
+// memory variables are double-buffered likewise (read m, write mo).
+// --------------------------------------------------------------------------
+template <class P> struct FusedArgs {
+    StepArgs<P> s;                        // time n (read): fields, coefficients, memory variables, geometry
+    typename P::Field g[PW_NFIELDS];      // time n+1 (write)
+    typename P::Mem* mo[3][kMemPerAxis];  // memory variables written
+};
+
+template <class P>
+__device__ __forceinline__ typename P::R cpml2(const typename P::Mem* in, typename P::Mem* out, unsigned k,
+                                               typename P::R c, typename P::R e, typename P::R der, bool write)
+{
+    using R = typename P::R;
+    const R psi = P::ldm(in, k) - c * der;
+    if (write) P::stm(out, k, e * (psi - c * der));
+    return der + psi;
+}
+
+// Velocity update of one axis (AX 0: vux/vwx, 1: vuy/vwy, 2: vuz/vwz) at an
+// updated cell; own: this block owns the cell and writes its memory variables.
+template <class P, int AX>
+__device__ __forceinline__ void v_axis(const FusedArgs<P>& A, int ix, int iy, int iz, bool own, typename P::R& vu1,
+                                       typename P::R& vw1)
+{
+    using R = typename P::R;
+    const StepArgs<P>& a = A.s;
+    const unsigned PL = a.plane, W = (unsigned)a.pitch;
+    const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)iz * PL;
+    const Deriv<P> D(a.h);
+    const R dt = a.dt;
+    const unsigned st = AX == 0 ? 1u : (AX == 1 ? W : PL);
+    // d1: normal stress along AX (forward), d2/d3: shear stresses (backward),
+    // ds: pressure along AX (forward)  (Float/main.cu:115-128)
+    R d1, d2, d3;
+    const R ds = D(P::ld(a.f[PW_P], o + st) - P::ld(a.f[PW_P], o));
+    if (AX == 0) {
+        d1 = D(P::ld(a.f[PW_TXX], o + 1) - P::ld(a.f[PW_TXX], o));
+        d2 = D(P::ld(a.f[PW_TXY], o) - P::ld(a.f[PW_TXY], o - W));
+        d3 = D(P::ld(a.f[PW_TXZ], o) - P::ld(a.f[PW_TXZ], o - PL));
+    } else if (AX == 1) {
+        d1 = D(P::ld(a.f[PW_TYY], o + W) - P::ld(a.f[PW_TYY], o));
+        d2 = D(P::ld(a.f[PW_TXY], o) - P::ld(a.f[PW_TXY], o - 1));
+        d3 = D(P::ld(a.f[PW_TYZ], o) - P::ld(a.f[PW_TYZ], o - PL));
+    } else {
+        d1 = D(P::ld(a.f[PW_TZZ], o + PL) - P::ld(a.f[PW_TZZ], o));
+        d2 = D(P::ld(a.f[PW_TXZ], o) - P::ld(a.f[PW_TXZ], o - 1));
+        d3 = D(P::ld(a.f[PW_TYZ], o) - P::ld(a.f[PW_TYZ], o - W));
+    }
+    R dsv = ds;
+    // CPML: d1 and ds are damped along AX (half-point profile, terms 0 and 1);
+    // d2 along x (AX 1, 2) or y (AX 0); d3 along z (AX 0, 1) or y (AX 2).
+    const int xs = slab_index(ix, a.nx, a.lo[0], a.hi[0]);
+    const int ys = slab_index(iy, a.ny, a.lo[1], a.hi[1]);
+    const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
+    const unsigned kx = xs >= 0 ? ((unsigned)iz * a.ny + iy) * (a.lo[0] + a.hi[0]) + xs : 0u;
+    const unsigned ky = ys >= 0 ? ((unsigned)iz * (a.lo[1] + a.hi[1]) + ys) * W + ix : 0u;
+    const unsigned kz = zs >= 0 ? ((unsigned)zs * a.ny + iy) * W + ix : 0u;
+    const int sa = AX == 0 ? xs : (AX == 1 ? ys : zs);
+    if (sa >= 0) {
+        const Tab4<R> t = a.tab[AX][AX == 0 ? ix : (AX == 1 ? iy : iz)];
+        const unsigned k = AX == 0 ? kx : (AX == 1 ? ky : kz);
+        d1 = cpml2<P>(a.m[AX][0], A.mo[AX][0], k, t.ch, t.eh, d1, own);
+        dsv = cpml2<P>(a.m[AX][1], A.mo[AX][1], k, t.ch, t.eh, dsv, own);
+    }
+    if (AX == 0) {
+        if (ys >= 0) { const Tab4<R> t = a.tab[1][iy]; d2 = cpml2<P>(a.m[1][2], A.mo[1][2], ky, t.cf, t.ef, d2, own); }
+        if (zs >= 0) { const Tab4<R> t = a.tab[2][iz]; d3 = cpml2<P>(a.m[2][2], A.mo[2][2], kz, t.cf, t.ef, d3, own); }
+    } else if (AX == 1) {
+        if (xs >= 0) { const Tab4<R> t = a.tab[0][ix]; d2 = cpml2<P>(a.m[0][2], A.mo[0][2], kx, t.cf, t.ef, d2, own); }
+        if (zs >= 0) { const Tab4<R> t = a.tab[2][iz]; d3 = cpml2<P>(a.m[2][3], A.mo[2][3], kz, t.cf, t.ef, d3, own); }
+    } else {
+        if (xs >= 0) { const Tab4<R> t = a.tab[0][ix]; d2 = cpml2<P>(a.m[0][3], A.mo[0][3], kx, t.cf, t.ef, d2, own); }
+        if (ys >= 0) { const Tab4<R> t = a.tab[1][iy]; d3 = cpml2<P>(a.m[1][3], A.mo[1][3], ky, t.cf, t.ef, d3, own); }
+    }
+    const R S = d1 + d2 + d3;
+    const R vw0 = P::ld(a.f[PW_VWX + AX], o), vu0 = P::ld(a.f[PW_VUX + AX], o);
+    if constexpr (!P::scaled) {
+        const R ri = inv_mean(P::ldc(a.c[PC_RHO], o), P::ldc(a.c[PC_RHO], o + st));
+        const R c1 = half_sum(P::ldc(a.c[PC_C1], o), P::ldc(a.c[PC_C1], o + st));
+        const R c2 = half_sum(P::ldc(a.c[PC_C2], o), P::ldc(a.c[PC_C2], o + st));
+        const R rf = P::ldc(a.c[PC_RHOF], o);
+        R cc1, cc2;
+        cn_pair(c1, c2, rf, ri, dt, cc1, cc2);
+        vw1 = c1 == (R)0 ? (R)0 : (vw0 * cc2 - dt * (rf * ri * S + dsv)) / cc1;
+        vu1 = vu0 + dt * ri * S - rf * ri * (vw1 - vw0);
+    } else {
+        const R aw = P::ldc(a.c[SC_AWX + AX], o), bp = P::ldc(a.c[SC_BPX + AX], o), bt = P::ldc(a.c[SC_BTX + AX], o);
+        vw1 = (aw == 0.0f && bp == 0.0f && bt == 0.0f) ? 0.0f : aw * vw0 - bt * S - bp * dsv;
+        vu1 = vu0 + P::ldc(a.c[SC_RITX + AX], o) * S - P::ldc(a.c[SC_FRX + AX], o) * (vw1 - vw0);
+    }
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void k_fused(const FusedArgs<P> A)
+{
+    using R = typename P::R;
+    const StepArgs<P>& a = A.s;
+    // velocity of the current plane for the tile and its ring: vux, vuy, vuz, vwx, vwy
+    __shared__ R lds[2][5][6 * 66];
+    const Tile tl = tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int ix = tl.bx * 64 + tx, iy = tl.by * 4 + ty;
+    const int z0 = a.zb + tl.bz * a.zc;
+    const int z1 = min(z0 + a.zc, a.ze);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const unsigned PL = a.plane, W = (unsigned)a.pitch;
+    // no early return below: every thread takes part in the barriers
+    const bool in_xy = ix <= nx - 1 && iy <= ny - 1;
+    const bool up_xy = ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2;
+    const int lt = (ty + 1) * 66 + (tx + 1);
+    // ring cell of this thread: 0 = row y-1 (vuy, vwy), 1 = row y+4 (vux, vuz),
+    // 2 = column x-1 (vux, vwx), 3 = column x+64 (vuy, vuz)
+    int hk = -1, hx = 0, hy = 0;
+    if (tid < 64) { hk = 0; hx = tl.bx * 64 + tid; hy = tl.by * 4 - 1; }
+    else if (tid < 128) { hk = 1; hx = tl.bx * 64 + tid - 64; hy = tl.by * 4 + 4; }
+    else if (tid < 132) { hk = 2; hx = tl.bx * 64 - 1; hy = tl.by * 4 + tid - 128; }
+    else if (tid < 136) { hk = 3; hx = tl.bx * 64 + 64; hy = tl.by * 4 + tid - 132; }
+    const bool h_in = hk >= 0 && hx >= 0 && hx <= nx - 1 && hy >= 0 && hy <= ny - 1;
+    const bool h_up = hk >= 0 && hx >= 1 && hx <= nx - 2 && hy >= 1 && hy <= ny - 2;
+    const int lh = hk >= 0 ? (hy - tl.by * 4 + 1) * 66 + (hx - tl.bx * 64 + 1) : 0;
+
+    // new velocities of the tile cell at plane z for the axes in mask (1 x, 2 y, 4 z);
+    // cells that are never updated keep their stored value
+    auto v_tile = [&](int z, bool own, int mask, R* u) {
+        if (z0 >= z1) return;
+        if (up_xy && z >= 1 && z <= nz - 2) {
+            if (mask & 1) v_axis<P, 0>(A, ix, iy, z, own, u[0], u[3]);
+            if (mask & 2) v_axis<P, 1>(A, ix, iy, z, own, u[1], u[4]);
+            if (mask & 4) v_axis<P, 2>(A, ix, iy, z, own, u[2], u[5]);
+            // the stress update reads the velocities as stored (Half: encoded)
+            for (int c = 0; c < 6; ++c) u[c] = P::rt(u[c]);
+        } else if (in_xy && z >= 0 && z <= nz - 1) {
+            const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z * PL;
+            for (int c = 0; c < 3; ++c)
+                if (mask & (1 << c)) { u[c] = P::ld(a.f[PW_VUX + c], o); u[3 + c] = P::ld(a.f[PW_VWX + c], o); }
+        }
+    };
+    auto store_tile = [&](int z, const R* u) {
+        if (z0 >= z1 || !up_xy || z < 1 || z > nz - 2) return;
+        const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z * PL;
+        for (int c = 0; c < 6; ++c) P::st(A.g[PW_VUX + c], o, u[c]);
+    };
+    auto put_tile = [&](int buf, const R* u) {
+        lds[buf][0][lt] = u[0]; lds[buf][1][lt] = u[1]; lds[buf][2][lt] = u[2];
+        lds[buf][3][lt] = u[3]; lds[buf][4][lt] = u[4];
+    };
+    auto ring = [&](int z, int buf) {
+        if (hk < 0 || z0 >= z1) return;
+        R vu[3] = {0, 0, 0}, vw[3] = {0, 0, 0};
+        const int need = hk == 0 ? 2 : (hk == 1 ? 5 : (hk == 2 ? 1 : 6));
+        if (h_up && z >= 1 && z <= nz - 2) {
+            if (need & 1) v_axis<P, 0>(A, hx, hy, z, false, vu[0], vw[0]);
+            if (need & 2) v_axis<P, 1>(A, hx, hy, z, false, vu[1], vw[1]);
+            if (need & 4) v_axis<P, 2>(A, hx, hy, z, false, vu[2], vw[2]);
+            for (int c = 0; c < 3; ++c) { vu[c] = P::rt(vu[c]); vw[c] = P::rt(vw[c]); }
+        } else if (h_in && z >= 0 && z <= nz - 1) {
+            const unsigned o = (unsigned)hx + (unsigned)hy * W + (unsigned)z * PL;
+            for (int c = 0; c < 3; ++c)
+                if (need & (1 << c)) { vu[c] = P::ld(a.f[PW_VUX + c], o); vw[c] = P::ld(a.f[PW_VWX + c], o); }
+        }
+        if (hk == 0) { lds[buf][1][lh] = vu[1]; lds[buf][4][lh] = vw[1]; }
+        else if (hk == 1) { lds[buf][0][lh] = vu[0]; lds[buf][2][lh] = vu[2]; }
+        else if (hk == 2) { lds[buf][0][lh] = vu[0]; lds[buf][3][lh] = vw[0]; }
+        else { lds[buf][1][lh] = vu[1]; lds[buf][2][lh] = vu[2]; }
+    };
+
+    R prev[6] = {0, 0, 0, 0, 0, 0}, cur[6] = {0, 0, 0, 0, 0, 0}, nxt[6] = {0, 0, 0, 0, 0, 0};
+    v_tile(z0 - 1, false, 4, prev);  // vuz, vwz below the chunk (owned by the block below)
+    v_tile(z0, true, 7, cur);
+    store_tile(z0, cur);
+    put_tile(0, cur);
+    ring(z0, 0);
+    __syncthreads();
+
+    for (int k = z0; k < z1; ++k) {
+        const int cb = (k - z0) & 1, nb = cb ^ 1;
+        if (k + 1 < z1) {
+            v_tile(k + 1, true, 7, nxt);
+            store_tile(k + 1, nxt);
+            put_tile(nb, nxt);
+            ring(k + 1, nb);
+        } else {
+            v_tile(k + 1, false, 3, nxt);  // vux, vuy above the chunk (owned by the block above)
+        }
+        // stress update of plane k (Float/main.cu:234-293 / Half/main.cu:297-370)
+        if (up_xy) {
+            const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)k * PL;
+            const Deriv<P> D(a.h);
+    const R dt = a.dt;
+            const R vux_l = lds[cb][0][lt - 1], vux_q = lds[cb][0][lt + 66];
+            const R vuy_h = lds[cb][1][lt - 66], vuy_r = lds[cb][1][lt + 1];
+            const R vuz_r = lds[cb][2][lt + 1], vuz_q = lds[cb][2][lt + 66];
+            const R vwx_l = lds[cb][3][lt - 1], vwy_h = lds[cb][4][lt - 66];
+            R uxx = D(cur[0] - vux_l), uyy = D(cur[1] - vuy_h), uzz = D(cur[2] - prev[2]);
+            R wx = D(cur[3] - vwx_l), wy = D(cur[4] - vwy_h), wz = D(cur[5] - prev[5]);
+            R uxy = D(vux_q - cur[0]), uyx = D(vuy_r - cur[1]);
+            R uxz = D(nxt[0] - cur[0]), uzx = D(vuz_r - cur[2]);
+            R uyz = D(nxt[1] - cur[1]), uzy = D(vuz_q - cur[2]);
+            const int xs = slab_index(ix, nx, a.lo[0], a.hi[0]);
+            const int ys = slab_index(iy, ny, a.lo[1], a.hi[1]);
+            const int zs = slab_index(k, nz, a.lo[2], a.hi[2]);
+            if (xs >= 0) {
+                const Tab4<R> t = a.tab[0][ix];
+                const unsigned q = ((unsigned)k * ny + iy) * (a.lo[0] + a.hi[0]) + xs;
+                uxx = cpml2<P>(a.m[0][4], A.mo[0][4], q, t.cf, t.ef, uxx, true);
+                wx = cpml2<P>(a.m[0][5], A.mo[0][5], q, t.cf, t.ef, wx, true);
+                uyx = cpml2<P>(a.m[0][6], A.mo[0][6], q, t.ch, t.eh, uyx, true);
+                uzx = cpml2<P>(a.m[0][7], A.mo[0][7], q, t.ch, t.eh, uzx, true);
+            }
+            if (ys >= 0) {
+                const Tab4<R> t = a.tab[1][iy];
+                const unsigned q = ((unsigned)k * (a.lo[1] + a.hi[1]) + ys) * W + ix;
+                uyy = cpml2<P>(a.m[1][4], A.mo[1][4], q, t.cf, t.ef, uyy, true);
+                wy = cpml2<P>(a.m[1][5], A.mo[1][5], q, t.cf, t.ef, wy, true);
+                uxy = cpml2<P>(a.m[1][6], A.mo[1][6], q, t.ch, t.eh, uxy, true);
+                uzy = cpml2<P>(a.m[1][7], A.mo[1][7], q, t.ch, t.eh, uzy, true);
+            }
+            if (zs >= 0) {
+                const Tab4<R> t = a.tab[2][k];
+                const unsigned q = ((unsigned)zs * ny + iy) * W + ix;
+                uzz = cpml2<P>(a.m[2][4], A.mo[2][4], q, t.cf, t.ef, uzz, true);
+                wz = cpml2<P>(a.m[2][5], A.mo[2][5], q, t.cf, t.ef, wz, true);
+                uxz = cpml2<P>(a.m[2][6], A.mo[2][6], q, t.ch, t.eh, uxz, true);
+                uyz = cpml2<P>(a.m[2][7], A.mo[2][7], q, t.ch, t.eh, uyz, true);
+            }
+            const R du = uxx + uyy + uzz, dw = wx + wy + wz;
+            R p1, xx1, yy1, zz1, xy1, yz1, xz1;
+            if constexpr (!P::scaled) {
+                const R mu_c = P::ldc(a.c[PC_MU], o), mu_x = P::ldc(a.c[PC_MU], o + 1),
+                        mu_y = P::ldc(a.c[PC_MU], o + W), mu_n = P::ldc(a.c[PC_MU], o + PL),
+                        mu_xn = P::ldc(a.c[PC_MU], o + PL + 1), mu_yn = P::ldc(a.c[PC_MU], o + PL + W),
+                        mu_xy = P::ldc(a.c[PC_MU], o + 1 + W);
+                const R muxz = harm4(mu_c, mu_x, mu_n, mu_xn);
+                const R muyz = harm4(mu_c, mu_y, mu_n, mu_yn);
+                const R muxy = harm4(mu_c, mu_x, mu_y, mu_xy);
+                const R C = P::ldc(a.c[PC_C], o), M = P::ldc(a.c[PC_M], o);
+                const R HH = P::ldc(a.c[PC_HH], o), H2u = P::ldc(a.c[PC_H2U], o);
+                p1 = P::ld(a.f[PW_P], o) - dt * (C * du + M * dw);
+                xx1 = P::ld(a.f[PW_TXX], o) + dt * (H2u * (uyy + uzz) + HH * uxx + C * dw);
+                yy1 = P::ld(a.f[PW_TYY], o) + dt * (H2u * (uxx + uzz) + HH * uyy + C * dw);
+                zz1 = P::ld(a.f[PW_TZZ], o) + dt * (H2u * (uxx + uyy) + HH * uzz + C * dw);
+                xy1 = P::ld(a.f[PW_TXY], o) + muxy * dt * (uxy + uyx);
+                yz1 = P::ld(a.f[PW_TYZ], o) + muyz * dt * (uyz + uzy);
+                xz1 = P::ld(a.f[PW_TXZ], o) + muxz * dt * (uxz + uzx);
+            } else {
+                const R C = P::ldc(a.c[SC_C], o), M = P::ldc(a.c[SC_M], o);
+                const R HH = P::ldc(a.c[SC_HH], o), H2u = P::ldc(a.c[SC_H2U], o);
+                p1 = P::ld(a.f[PW_P], o) - C * du - M * dw;
+                xx1 = P::ld(a.f[PW_TXX], o) + H2u * (uyy + uzz) + HH * uxx + C * dw;
+                yy1 = P::ld(a.f[PW_TYY], o) + H2u * (uxx + uzz) + HH * uyy + C * dw;
+                zz1 = P::ld(a.f[PW_TZZ], o) + H2u * (uxx + uyy) + HH * uzz + C * dw;
+                xy1 = P::ld(a.f[PW_TXY], o) + P::ldc(a.c[SC_MUXY], o) * (uxy + uyx);
+                yz1 = P::ld(a.f[PW_TYZ], o) + P::ldc(a.c[SC_MUYZ], o) * (uyz + uzy);
+                xz1 = P::ld(a.f[PW_TXZ], o) + P::ldc(a.c[SC_MUXZ], o) * (uxz + uzx);
+            }
+            P::st(A.g[PW_P], o, p1);
+            P::st(A.g[PW_TXX], o, xx1); P::st(A.g[PW_TYY], o, yy1); P::st(A.g[PW_TZZ], o, zz1);
+            P::st(A.g[PW_TXY], o, xy1); P::st(A.g[PW_TYZ], o, yz1); P::st(A.g[PW_TXZ], o, xz1);
+        }
+        __syncthreads();
+        for (int c = 0; c < 6; ++c) { prev[c] = cur[c]; cur[c] = nxt[c]; }
     }
 }
 
@@ -625,6 +932,25 @@ hipError_t launch_stress(const DevBuffers& b, hipStream_t st, int zb, int ze)
         using P = decltype(pol);
         const StepArgs<P> a = range_args<P>(b, zb, ze);
         k_stress<P><<<a.nblocks, dim3(64, 4), 0, st>>>(a);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_fused(const DevBuffers& b, hipStream_t st)
+{
+    return dispatch(b.prec, [&](auto pol) {
+        using P = decltype(pol);
+        FusedArgs<P> A;
+        A.s = make_args<P>(b);
+        for (int k = 0; k < PW_NFIELDS; ++k) {
+            DevBuffers w = b;
+            w.f[k] = b.fo[k];
+            w.lo[k] = b.loo[k];
+            A.g[k] = make_field<P>(w, k);
+        }
+        for (int ax = 0; ax < 3; ++ax)
+            for (int k = 0; k < kMemPerAxis; ++k) A.mo[ax][k] = (typename P::Mem*)b.memo[ax][k];
+        k_fused<P><<<A.s.nblocks, dim3(64, 4), 0, st>>>(A);
         return hipGetLastError();
     });
 }

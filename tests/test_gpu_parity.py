@@ -187,16 +187,23 @@ def test_snapshots_match_fields(borehole):
         assert rel_l2(gpu["snaps"][k], ref["snaps"][k]) <= 1e-5
 
 
-def test_slab_schedule_is_bit_identical(layered, monkeypatch):
-    """The z-slab interleaved schedule (PW_SLAB) updates every cell with the same
-    arithmetic as the two full-grid launches: results are bit-identical."""
+def test_step_schedules_are_bit_identical(layered, monkeypatch):
+    """The two-kernel step (default), the fused one-launch step (PW_FUSED=1) and the
+    z-slab interleaved schedule (PW_SLAB) apply the same per-cell arithmetic:
+    results are bit-identical, in every precision."""
     grid, spec, nt, recs, gpu32, base = layered
-    outs = []
-    for slab in ("0", "3", "8"):
-        monkeypatch.setenv("PW_SLAB", slab)
-        g, _ = cases.run_gpu(grid, nt, "fp32", recs, spec=spec, fields=("vux", "vwz", "txz", "p"))
-        outs.append(g)
-    for g in outs[1:]:
-        assert np.array_equal(g["seis"], outs[0]["seis"])
-        for f in ("vux", "vwz", "txz", "p"):
-            assert np.array_equal(g["fields"][f], outs[0]["fields"][f])
+    schedules = (("0", "0"), ("1", "0"), ("0", "3"), ("0", "8"))
+    bad = []
+    for prec in ("fp32", "comp16", "bf16", "fp64"):
+        outs = []
+        for fused, slab in schedules:
+            monkeypatch.setenv("PW_FUSED", fused)
+            monkeypatch.setenv("PW_SLAB", slab)
+            g, _ = cases.run_gpu(grid, nt, prec, recs, spec=spec, fields=("vux", "vwz", "txz", "p"))
+            outs.append(g)
+        for sch, g in zip(schedules[1:], outs[1:]):
+            same = np.array_equal(g["seis"], outs[0]["seis"]) and all(
+                np.array_equal(g["fields"][f], outs[0]["fields"][f]) for f in ("vux", "vwz", "txz", "p"))
+            if not same:
+                bad.append((prec, "fused=%s slab=%s" % sch, rel_l2(g["seis"], outs[0]["seis"])))
+    assert not bad, f"schedules differing from the two-kernel step: {bad}"
