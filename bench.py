@@ -67,6 +67,115 @@ def cpu_baseline(n, np_, spec_base, dt, h, steps):
                       f"({total:.1f} s with set-up)"}
 
 
+def serving_trace(nshots, rate, seed, n_vol, np_):
+    """Shot requests: Poisson arrivals (mean rate per second), total grid N uniform
+    in {96, 128, 192, 256}, source uniform in the non-CPML interior, 500-1500
+    steps, FP32 or comp16 with equal odds, medium cropped from the interior of
+    the 512^3 two-layer model at a random origin, 64 receivers x (txx, vux) on
+    a line along x at iz = np + 10, iy = N / 2."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, nshots))
+    t -= t[0]
+    vol_i = n_vol - 2 * np_
+    out = []
+    for k in range(nshots):
+        n = int(rng.choice([96, 128, 192, 256]))
+        ni = n - 2 * np_
+        crop = tuple(int(v) for v in rng.integers(0, vol_i - ni + 1, 3))
+        src = tuple(int(v) for v in rng.integers(np_, n - np_, 3))
+        nt = int(rng.integers(500, 1501))
+        prec = "fp32" if rng.random() < 0.5 else "comp16"
+        xs = np.linspace(np_, n - 1 - np_, 64).round().astype(np.int64)
+        iz, iy = np_ + 10, n // 2
+        fields = ["txx"] * 64 + ["vux"] * 64
+        idx = [int(x) + iy * n + iz * n * n for x in xs] * 2
+        out.append(dict(t=float(t[k]), n=n, crop=crop, src=src, nt=nt, prec=prec, fields=fields, idx=idx))
+    return out
+
+
+def run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=False, **server_kw):
+    """Replays the trace against one server: each request is submitted at its
+    arrival time; returns per-shot timings and the server's statistics."""
+    import threading
+    import numpy as np
+    import mixedprecisionfaracoustic_amd as pw
+    srv = pw.Server(**server_kw)
+    vol_i = n_vol - 2 * np_
+    srv.set_medium_layered(spec, vol_i, vol_i, vol_i, np_)
+    ids = [None] * len(trace)
+    t0 = srv.clock()
+
+    def feeder():
+        for k, q in enumerate(trace):
+            delay = q["t"] - (srv.clock() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            prec = "fp32" if force_fp32 else q["prec"]
+            ids[k] = srv.submit(q["n"], q["n"], q["n"], np_, h, dt, q["nt"], prec, q["crop"], q["src"], 25.0,
+                                q["fields"], q["idx"], weights=(1.0, 1.0, 1.0, -1.0), cpml_r=1e-4, cpml_v0=4000.0)
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    done = 0
+    for k in range(len(trace)):
+        while ids[k] is None:
+            time.sleep(0.001)
+        srv.wait(ids[k])
+        done += 1
+        if done % 32 == 0:
+            print(f"  {done}/{len(trace)} shots done at {srv.clock() - t0:.1f} s", file=sys.stderr, flush=True)
+    th.join()
+    st = [srv.status(i) for i in ids]
+    seis = [srv.result(i) for i in ids]
+    stats = srv.stats()
+    srv.close()
+    lat = np.array([s["t_done"] - s["t_submit"] for s in st])
+    span = max(s["t_done"] for s in st) - min(s["t_submit"] for s in st)
+    work = sum(float(q["n"]) ** 3 * q["nt"] for q in trace)
+    finite = all(bool(np.isfinite(x).all()) for x in seis)
+    return {"shots": len(trace), "makespan_s": round(span, 3), "shots_per_s": round(len(trace) / span, 4),
+            "gcell_steps_per_s": round(work / span / 1e9, 3), "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "mean_latency_s": round(float(lat.mean()), 3),
+            "busy_s": round(stats["busy_s"], 3), "setup_s": round(stats["setup_s"], 3),
+            "graph_s": round(stats["graph_s"], 3), "retire_s": round(stats["retire_s"], 3),
+            "rounds": stats["rounds"], "graphs_built": stats["graphs_built"],
+            "peak_active": stats["peak_active"], "peak_gb": round(stats["peak_bytes"] / 1e9, 2),
+            "finite": finite}, seis
+
+
+def serving_main(args):
+    n_vol, np_, h, dt = 512, 20, 5.0, 0.5e-3
+    spec = workload(n_vol, np_)
+    trace = serving_trace(args.shots, args.rate, args.seed, n_vol, np_)
+    print(f"serving: {len(trace)} shots over {trace[-1]['t']:.1f} s of arrivals", file=sys.stderr, flush=True)
+    batched, seis_b = run_serving(trace, n_vol, np_, h, dt, spec, max_active=args.slots, round_steps=args.round_steps,
+                                  use_graphs=not args.no_graphs, batched=True)
+    print("  batched:", json.dumps(batched), file=sys.stderr, flush=True)
+    seq, seis_s = run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=True, max_active=1, round_steps=64,
+                              use_graphs=False, batched=False)
+    print("  sequential fp32:", json.dumps(seq), file=sys.stderr, flush=True)
+    # accuracy of the mixed-precision serving run against the FP32 run, per shot
+    import numpy as np
+    err = []
+    for q, a, b in zip(trace, seis_b, seis_s):
+        if q["prec"] == "fp32":
+            err.append(float(np.max(np.abs(a - b))))
+    out = {"metric": "shot-serving throughput and p50/p99 latency vs sequential FP32 FCFS",
+           "value": batched["gcell_steps_per_s"], "unit": "Gcell-steps/s", "n_gpus": 1,
+           "higher_is_better": True, "data": "synthetic (Poisson trace, seeded)",
+           "config": {"workload": "shot_serving_trace", "shots": args.shots, "rate_per_s": args.rate,
+                      "seed": args.seed, "grids": [96, 128, 192, 256], "steps": [500, 1500],
+                      "precisions": "fp32/comp16 50/50", "slots": args.slots, "round_steps": args.round_steps,
+                      "graphs": not args.no_graphs},
+           "serving": batched, "sequential_fp32": seq,
+           "throughput_ratio": round(batched["gcell_steps_per_s"] / seq["gcell_steps_per_s"], 3),
+           "p50_ratio": round(seq["p50_latency_s"] / batched["p50_latency_s"], 3),
+           "fp32_shots_identical_to_sequential": all(e == 0.0 for e in err)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,10 +187,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--prof-steps", type=int, default=40)
+    ap.add_argument("--serving", action="store_true", help="replay the shot-serving trace instead")
+    ap.add_argument("--shots", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--slots", type=int, default=32)
+    ap.add_argument("--round-steps", type=int, default=32)
+    ap.add_argument("--no-graphs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         return 2
+    if args.serving:
+        return serving_main(args)
 
     import numpy as np
     import torch

@@ -42,7 +42,8 @@ typedef enum {
     PW_ERR_STATE = 4,         /* call out of order (e.g. run before set_medium) */
     PW_ERR_UNSTABLE = 5,      /* Courant condition violated by the medium */
     PW_ERR_UNSUPPORTED = 6,   /* valid request this build does not implement */
-    PW_ERR_IO = 7             /* file could not be written */
+    PW_ERR_IO = 7,            /* file could not be written */
+    PW_ERR_TIMEOUT = 8        /* pw_server_wait: the shot did not finish in time */
 } pw_status;
 
 /* Storage precision of wavefields and CPML memory variables.  Arithmetic is FP32
@@ -238,6 +239,107 @@ pw_status pw_set_medium_layered(pw_solver* s, const pw_layered* spec);
  * PW_NBASE arrays of nxi*nyi*nzi doubles (for feeding the same medium to other
  * tools). */
 pw_status pw_get_base_fields(pw_solver* s, double* out);
+
+/* ---------------------------------------------------------------------------
+ * Shot server.
+ *
+ * No counterpart in the reference, which runs one shot per process: main()
+ * sets up, steps and writes .dat files for a single source
+ * (Float/main.cu:300-1477).  The server replaces a queue of such runs.  It
+ * admits shot requests first come, first served while the device-memory budget
+ * and the slot count allow; it keeps every admitted shot resident; and it
+ * advances all of them together, in rounds of steps:
+ *  - one launch per step kernel covers every active shot of a precision;
+ *  - a round that repeats is captured once as a hipGraph and replayed.
+ * Shots join and leave between rounds.  A shot's seismogram is bit-identical
+ * to the same shot run alone through pw_run, because the per-cell code is the
+ * same.  All calls are thread-safe; one worker thread owns the device.
+ * ------------------------------------------------------------------------- */
+typedef struct pw_server pw_server;
+
+typedef struct pw_server_config {
+    int device;
+    void* stream;           /* hipStream_t for all device work; NULL: the server creates one */
+    long long hbm_budget;   /* device bytes the admitted shots may hold; 0: 90 % of the memory
+                               free at creation, less the medium */
+    int max_active;         /* shots advanced together (slots), >= 1 */
+    int round_steps;        /* steps per scheduling round, >= 1 */
+    int use_graphs;         /* 1: a round that repeats (same shots, same length) is captured
+                               once as a hipGraph and replayed */
+    int batched;            /* 1: one launch per step kernel for all active shots of a
+                               precision; 0: each shot stepped by pw_run in turn */
+} pw_server_config;
+
+typedef struct pw_shot {
+    int nx, ny, nz, np;     /* total grid and CPML width, as in pw_config */
+    double h, dt;
+    int nt;
+    int precision;          /* pw_precision */
+    double cpml_r, cpml_v0;
+    int crop[3];            /* origin (x, y, z) of the shot's interior inside the server's
+                               medium volume, in interior cells */
+    int sx, sy, sz;         /* source cell, total grid */
+    double f0;              /* Ricker peak frequency [Hz] */
+    double w[4];            /* source weights into txx, tyy, tzz, p */
+    int nrec;
+    const int* rec_field;   /* nrec field ids; copied by pw_server_submit */
+    const long long* rec_index; /* nrec flat total-grid indices; copied */
+} pw_shot;
+
+typedef enum {
+    PW_SHOT_QUEUED = 0,
+    PW_SHOT_RUNNING = 1,
+    PW_SHOT_DONE = 2,
+    PW_SHOT_FAILED = 3
+} pw_shot_state;
+
+typedef struct pw_shot_status {
+    int state;                 /* pw_shot_state */
+    int precision, nt, nrec;
+    long long cells;           /* nx*ny*nz */
+    long long device_bytes;    /* device memory the shot holds while admitted */
+    double t_submit, t_admit, t_done; /* seconds on the server clock (-1: not yet) */
+    pw_status error;           /* PW_OK unless state == PW_SHOT_FAILED */
+    char error_text[256];
+} pw_shot_status;
+
+typedef struct pw_server_stats {
+    long long submitted, completed, failed;
+    long long rounds, graphs_built, launches;
+    double busy_s;             /* time spent in rounds (device work issued and waited for) */
+    double setup_s;            /* time spent admitting shots (allocation, medium, tables) */
+    double graph_s;            /* time spent capturing and instantiating round graphs */
+    double retire_s;           /* time spent copying results out and freeing finished shots */
+    double cell_steps;         /* sum of nx*ny*nz*nt over completed shots */
+    long long peak_bytes;      /* largest device memory held by admitted shots */
+    int peak_active;           /* largest number of shots advanced together */
+} pw_server_stats;
+
+/* max_active 16, round_steps 32, graphs on, batched, budget 0 (automatic). */
+void pw_server_config_default(pw_server_config* cfg);
+pw_status pw_server_create(const pw_server_config* cfg, pw_server** out);
+/* Finishes the admitted shots, fails the queued ones, stops the worker. */
+void pw_server_destroy(pw_server* srv);
+/* The medium volume shots are cropped from: PW_NBASE host arrays of nxi*nyi*nzi
+ * doubles (interior cells, x fastest), kept on the device.  Only while no shot
+ * is queued or admitted. */
+pw_status pw_server_set_medium(pw_server* srv, const double* const* base, int nxi, int nyi, int nzi);
+/* The same from a layered specification (pw_set_medium_layered's generator),
+ * generated for an interior of nxi*nyi*nzi cells of a grid with CPML width np. */
+pw_status pw_server_set_medium_layered(pw_server* srv, const pw_layered* spec, int nxi, int nyi, int nzi, int np);
+/* Queues a shot (validated here); returns its id.  Non-blocking. */
+pw_status pw_server_submit(pw_server* srv, const pw_shot* shot, long long* id);
+/* Blocks until the shot is done or failed (its error), or timeout_s passes
+ * (PW_ERR_TIMEOUT; timeout_s < 0 waits without limit). */
+pw_status pw_server_wait(pw_server* srv, long long id, double timeout_s);
+pw_status pw_server_status(pw_server* srv, long long id, pw_shot_status* out);
+/* Copies a finished shot's seismogram, [nrec][nt] floats in physical units. */
+pw_status pw_server_result(pw_server* srv, long long id, float* seis);
+/* Forgets a finished or failed shot and its host-side result. */
+pw_status pw_server_release(pw_server* srv, long long id);
+pw_status pw_server_get_stats(pw_server* srv, pw_server_stats* out);
+/* Seconds since pw_server_create (the clock of pw_shot_status). */
+double pw_server_clock(pw_server* srv);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,8 @@ PRECISIONS = {"fp32": 0, "comp16": 1, "bf16": 2, "fp64": 3}
 # The reference's seismogram channels (Float/main.cu:1085-1093, 1213-1222).
 REFERENCE_CHANNELS = [("sisx", "txx"), ("sisz", "tzz"), ("sisvu", "vux"), ("sisvw", "vwx"), ("sisp", "p")]
 STATUS = {0: "PW_OK", 1: "PW_ERR_INVALID", 2: "PW_ERR_HIP", 3: "PW_ERR_OUT_OF_MEMORY", 4: "PW_ERR_STATE",
-          5: "PW_ERR_UNSTABLE", 6: "PW_ERR_UNSUPPORTED", 7: "PW_ERR_IO"}
+          5: "PW_ERR_UNSTABLE", 6: "PW_ERR_UNSUPPORTED", 7: "PW_ERR_IO", 8: "PW_ERR_TIMEOUT"}
+SHOT_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 MAX_LAYERS = 16
 
 
@@ -61,6 +62,34 @@ class Layered(C.Structure):
                 ("seed", C.c_ulonglong)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("stream", C.c_void_p), ("hbm_budget", C.c_longlong),
+                ("max_active", C.c_int), ("round_steps", C.c_int), ("use_graphs", C.c_int), ("batched", C.c_int)]
+
+
+class ShotSpec(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("np", C.c_int),
+                ("h", C.c_double), ("dt", C.c_double), ("nt", C.c_int), ("precision", C.c_int),
+                ("cpml_r", C.c_double), ("cpml_v0", C.c_double), ("crop", C.c_int * 3),
+                ("sx", C.c_int), ("sy", C.c_int), ("sz", C.c_int), ("f0", C.c_double), ("w", C.c_double * 4),
+                ("nrec", C.c_int), ("rec_field", C.POINTER(C.c_int)), ("rec_index", C.POINTER(C.c_longlong))]
+
+
+class ShotStatus(C.Structure):
+    _fields_ = [("state", C.c_int), ("precision", C.c_int), ("nt", C.c_int), ("nrec", C.c_int),
+                ("cells", C.c_longlong), ("device_bytes", C.c_longlong),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("error", C.c_int), ("error_text", C.c_char * 256)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("failed", C.c_longlong),
+                ("rounds", C.c_longlong), ("graphs_built", C.c_longlong), ("launches", C.c_longlong),
+                ("busy_s", C.c_double), ("setup_s", C.c_double), ("graph_s", C.c_double),
+                ("retire_s", C.c_double), ("cell_steps", C.c_double), ("peak_bytes", C.c_longlong),
+                ("peak_active", C.c_int)]
+
+
 # symbol -> (restype, argtypes); the same list the header declares.
 _SIGS = {
     "pw_config_default": (None, [C.POINTER(Config)]),
@@ -88,6 +117,19 @@ _SIGS = {
     "pw_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "pw_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                       C.POINTER(C.c_longlong)]),
+    "pw_server_config_default": (None, [C.POINTER(ServerConfig)]),
+    "pw_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "pw_server_destroy": (None, [C.c_void_p]),
+    "pw_server_set_medium": (C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.c_int, C.c_int, C.c_int]),
+    "pw_server_set_medium_layered": (C.c_int, [C.c_void_p, C.POINTER(Layered), C.c_int, C.c_int, C.c_int,
+                                               C.c_int]),
+    "pw_server_submit": (C.c_int, [C.c_void_p, C.POINTER(ShotSpec), C.POINTER(C.c_longlong)]),
+    "pw_server_wait": (C.c_int, [C.c_void_p, C.c_longlong, C.c_double]),
+    "pw_server_status": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(ShotStatus)]),
+    "pw_server_result": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(C.c_float)]),
+    "pw_server_release": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "pw_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "pw_server_clock": (C.c_double, [C.c_void_p]),
 }
 
 _lib = None
@@ -325,3 +367,110 @@ def reference_solver(precision="fp32", nt=3500, **kw):
     s.set_source(n // 2, n // 2, 20 + 32, 3000.0, (1.0, 1.0, 1.0, 0.0))
     s.set_receivers_reference(n // 2, n // 2)
     return s
+
+
+class Server:
+    """Shot server (a pw_server handle): admits shots first come, first served by
+    device-memory budget and slot count, keeps them resident and advances all
+    active shots together (one launch per step kernel per precision, rounds of
+    steps replayed as hipGraphs).  Shots crop their medium from one volume set
+    with set_medium / set_medium_layered."""
+
+    def __init__(self, device=0, max_active=16, round_steps=32, use_graphs=True, batched=True, hbm_budget=0,
+                 stream=None):
+        L = lib()
+        cfg = ServerConfig()
+        L.pw_server_config_default(C.byref(cfg))
+        cfg.device, cfg.max_active, cfg.round_steps = int(device), int(max_active), int(round_steps)
+        cfg.use_graphs, cfg.batched, cfg.hbm_budget = int(bool(use_graphs)), int(bool(batched)), int(hbm_budget)
+        cfg.stream = stream
+        h = C.c_void_p()
+        _check(L.pw_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.cfg = cfg
+        self._nrec = {}
+        self._nt = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pw_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_medium(self, base):
+        """base: [9, nzi, nyi, nxi] float64 interior fields (the volume shots are cropped from)."""
+        b = np.ascontiguousarray(base, np.float64)
+        if b.ndim != 4 or b.shape[0] != 9:
+            raise ValueError("base must be [9, nzi, nyi, nxi]")
+        ptrs = (C.POINTER(C.c_double) * 9)(*[_dp(b[k]) for k in range(9)])
+        _check(lib().pw_server_set_medium(self._h, ptrs, b.shape[3], b.shape[2], b.shape[1]))
+
+    def set_medium_layered(self, spec, nxi, nyi, nzi, np_):
+        _check(lib().pw_server_set_medium_layered(self._h, C.byref(spec), int(nxi), int(nyi), int(nzi), int(np_)))
+
+    def submit(self, nx, ny, nz, np_, h, dt, nt, precision, crop, source, f0, fields, indices,
+               weights=(1.0, 1.0, 1.0, 0.0), cpml_r=1e-4, cpml_v0=4000.0):
+        q = ShotSpec()
+        q.nx, q.ny, q.nz, q.np = int(nx), int(ny), int(nz), int(np_)
+        q.h, q.dt, q.nt = float(h), float(dt), int(nt)
+        q.precision = PRECISIONS[precision]
+        q.cpml_r, q.cpml_v0 = float(cpml_r), float(cpml_v0)
+        for a in range(3):
+            q.crop[a] = int(crop[a])
+        q.sx, q.sy, q.sz = (int(v) for v in source)
+        q.f0 = float(f0)
+        for k in range(4):
+            q.w[k] = float(weights[k])
+        f = np.ascontiguousarray([FIELD_ID[x] if isinstance(x, str) else int(x) for x in fields], np.int32)
+        i = np.ascontiguousarray(indices, np.int64)
+        if f.shape != i.shape:
+            raise ValueError("fields and indices must have the same length")
+        q.nrec = len(f)
+        q.rec_field = f.ctypes.data_as(C.POINTER(C.c_int))
+        q.rec_index = i.ctypes.data_as(C.POINTER(C.c_longlong))
+        sid = C.c_longlong()
+        _check(lib().pw_server_submit(self._h, C.byref(q), C.byref(sid)))
+        self._nrec[sid.value] = len(f)
+        self._nt[sid.value] = int(nt)
+        return sid.value
+
+    def wait(self, sid, timeout=None):
+        _check(lib().pw_server_wait(self._h, int(sid), -1.0 if timeout is None else float(timeout)))
+
+    def status(self, sid):
+        st = ShotStatus()
+        _check(lib().pw_server_status(self._h, int(sid), C.byref(st)))
+        return {"state": SHOT_STATES[st.state], "precision": st.precision, "nt": st.nt, "nrec": st.nrec,
+                "cells": st.cells, "device_bytes": st.device_bytes, "t_submit": st.t_submit,
+                "t_admit": st.t_admit, "t_done": st.t_done, "error": st.error,
+                "error_text": st.error_text.decode(errors="replace")}
+
+    def result(self, sid):
+        n = self._nrec[sid]
+        out = np.zeros((max(n, 1), self._nt[sid]), np.float32)
+        _check(lib().pw_server_result(self._h, int(sid), _fp(out)))
+        return out[:n]
+
+    def release(self, sid):
+        _check(lib().pw_server_release(self._h, int(sid)))
+        self._nrec.pop(sid, None)
+        self._nt.pop(sid, None)
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().pw_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def clock(self):
+        return lib().pw_server_clock(self._h)

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "pw_device.h"
+#include "pw_solver.h"
 
 namespace pw {
 
@@ -27,45 +28,6 @@ pw_status fail(pw_status st, const std::string& msg)
 
 using namespace pw;
 
-struct pw_solver {
-    pw_config cfg;
-    int prec = PW_FP32;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    DevBuffers b{};
-    void* pool = nullptr;
-    size_t pool_bytes = 0;
-    size_t state_bytes_total = 0;  // bytes of the state + memory-variable region (zeroed on reset)
-    // medium
-    bool has_medium = false;
-    int em = -23, epml = 0;
-    float inv_scale_pml = 1.0f;
-    double vp_max = 0.0, courant = 0.0;
-    bool layered = false;
-    pw_layered spec{};
-    // comp16 / bf16 dynamic scale (Half/main.cu:800-807, 1326-1364)
-    float dyn_scale = 1.0f, stress_base = 1.0f, vel_base = 1.0f, source_scale = 1.0f;
-    // source
-    bool has_source = false;
-    unsigned src_off = 0;
-    double f0 = 0.0, w[4] = {1, 1, 1, 0};
-    // receivers
-    int nrec = 0;
-    int* d_rfield = nullptr;
-    unsigned* d_roff = nullptr;
-    float* d_seis = nullptr;
-    // snapshots
-    int nsnap = 0, snap_y = 0;
-    std::vector<int> snap_step, snap_field;
-    float* d_snaps = nullptr;
-    int step = 0;
-    // per-kernel timing (pw_set_profiling)
-    bool profiling = false;
-    std::vector<hipEvent_t> ev_pool;     // created lazily, reused
-    std::vector<hipEvent_t> ev_pending;  // (start, stop) pairs around step-kernel launches
-    std::vector<char> ev_kind;           // per pair: 0 velocity, 1 stress
-    long long prof_steps = 0;
-};
 
 namespace {
 
@@ -170,21 +132,117 @@ extern "C" void pw_config_default(pw_config* c)
 
 extern "C" const char* pw_last_error(void) { return g_last_error.c_str(); }
 
-extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
+namespace {
+
+pw_status check_config(const pw_config& c)
 {
-    if (!cfg || !out) return fail(PW_ERR_INVALID, "pw_create: null argument");
-    *out = nullptr;
-    const pw_config& c = *cfg;
     if (c.precision < PW_FP32 || c.precision > PW_FP64) return fail(PW_ERR_INVALID, "pw_create: unknown precision");
     if (c.np < 1 || c.nx < 2 * c.np + 3 || c.ny < 2 * c.np + 3 || c.nz < 2 * c.np + 3)
         return fail(PW_ERR_INVALID, "pw_create: grid must be at least 2*np+3 cells per axis");
     if (!(c.h > 0) || !(c.dt > 0) || c.nt < 1) return fail(PW_ERR_INVALID, "pw_create: h, dt and nt must be positive");
     if (!(c.cpml_r > 0 && c.cpml_r < 1) || !(c.cpml_v0 > 0))
         return fail(PW_ERR_INVALID, "pw_create: cpml_r must be in (0,1) and cpml_v0 positive");
-    const int pitch = (c.nx + 63) / 64 * 64;
-    const unsigned long long cells = (unsigned long long)pitch * c.ny * c.nz;
-    if (cells >= (1ull << 32) - (1ull << 20))
+    const unsigned long long pitch = (unsigned long long)(c.nx + 63) / 64 * 64;
+    if (pitch * c.ny * c.nz >= (1ull << 32) - (1ull << 20))
         return fail(PW_ERR_UNSUPPORTED, "pw_create: grid exceeds 2^32 padded cells (32-bit offsets)");
+    return PW_OK;
+}
+
+// Geometry, CPML tables and the byte layout of the pooled allocation:
+// [state | memory variables] (zeroed together; twice for the fused step) then
+// [coefficients | tables].
+struct Layout {
+    Geom g{};
+    std::vector<double> tab[3];
+    int epml = 0;
+    float isp = 1.0f;
+    std::vector<size_t> off;
+    size_t zero_bytes = 0, total = 0;
+};
+
+void plan_layout(const pw_config& c, int fused, Layout& L)
+{
+    const int prec = c.precision;
+    Geom& g = L.g;
+    const int pitch = (c.nx + 63) / 64 * 64;
+    g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
+    g.pitch = pitch;
+    g.plane = (long long)pitch * c.ny;
+    g.cells = (long long)pitch * c.ny * c.nz;
+    const size_t cells = (size_t)g.cells;
+    // CPML tables (scaled forms need epml over all three axes first).
+    const int n3[3] = {c.nx, c.ny, c.nz};
+    if (is_scaled(prec)) {
+        float pmax = 0.0f, pm;
+        for (int ax = 0; ax < 3; ++ax) {
+            build_tables(c, prec, n3[ax], L.tab[ax], &L.epml, &L.isp, &pm);
+            pmax = std::max(pmax, pm);
+        }
+        // Half/main.cu:833-839
+        if (pmax > 0.0f) {
+            L.epml = (int)std::floor(std::log2(2048.0f / pmax));
+            L.epml = std::min(20, std::max(-20, L.epml));
+        }
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+        float pm;
+        build_tables(c, prec, n3[ax], L.tab[ax], &L.epml, &L.isp, &pm);
+        slab_widths(L.tab[ax], n3[ax], &g.lo[ax], &g.hi[ax]);
+    }
+    const long long W = pitch;
+    g.slab_cells[0] = (long long)c.nz * c.ny * (g.lo[0] + g.hi[0]);
+    g.slab_cells[1] = (long long)c.nz * (g.lo[1] + g.hi[1]) * W;
+    g.slab_cells[2] = (long long)(g.lo[2] + g.hi[2]) * c.ny * W;
+
+    const size_t sb = (size_t)state_bytes(prec), cb = (size_t)coef_bytes(prec), mb = (size_t)mem_bytes(prec);
+    const size_t rb = prec == PW_FP64 ? 8 : 4;
+    L.off.clear();
+    L.total = 0;
+    auto take = [&](size_t bytes) {
+        L.off.push_back(L.total);
+        L.total += align256(bytes);
+    };
+    const bool c16 = prec == PW_COMP16;
+    const int nbuf = fused ? 2 : 1;  // the fused step reads time n and writes time n+1
+    for (int b2 = 0; b2 < nbuf; ++b2) {
+        for (int k = 0; k < PW_NFIELDS; ++k) take(cells * (c16 ? 2 : sb));
+        if (c16)
+            for (int k = 0; k < PW_NFIELDS; ++k) take(cells * 1);
+        for (int ax = 0; ax < 3; ++ax)
+            for (int k = 0; k < kMemPerAxis; ++k) take((size_t)g.slab_cells[ax] * mb);
+    }
+    L.zero_bytes = L.total;
+    const int nc = ncoef(prec);
+    for (int k = 0; k < nc; ++k) take(cells * cb);
+    for (int ax = 0; ax < 3; ++ax) take((size_t)n3[ax] * 4 * rb);
+}
+
+}  // namespace
+
+size_t pw::solver_pool_bytes(const pw_config& c, int fused)
+{
+    Layout L;
+    plan_layout(c, fused, L);
+    return L.total;
+}
+
+extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
+{
+    // The fused one-launch step (double-buffered) and the z-slab interleave are
+    // bit-identical to the two kernels but slower on MI355X; both are opt-in.
+    int fused = 0, slab = 0;
+    env_int("PW_FUSED", &fused);
+    env_int("PW_SLAB", &slab);
+    return solver_create(cfg, fused, slab, out);
+}
+
+pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver** out)
+{
+    if (!cfg || !out) return fail(PW_ERR_INVALID, "pw_create: null argument");
+    *out = nullptr;
+    const pw_config& c = *cfg;
+    pw_status cst = check_config(c);
+    if (cst != PW_OK) return cst;
 
     int ndev = 0;
     PW_HIP_TRY(hipGetDeviceCount(&ndev));
@@ -194,68 +252,19 @@ extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
     pw_solver* s = new pw_solver();
     s->cfg = c;
     s->prec = c.precision;
-    Geom& g = s->b.g;
-    g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
-    g.pitch = pitch;
-    g.plane = (long long)pitch * c.ny;
-    g.cells = (long long)cells;
-
-    // CPML tables (scaled forms need epml over all three axes first).
+    Layout L;
+    plan_layout(c, fused, L);
+    s->b.g = L.g;
+    const Geom& g = s->b.g;
+    std::vector<double>* tab = L.tab;
     const int n3[3] = {c.nx, c.ny, c.nz};
-    std::vector<double> tab[3];
-    int epml = 0;
-    float isp = 1.0f;
-    if (is_scaled(s->prec)) {
-        float pmax = 0.0f, pm;
-        for (int ax = 0; ax < 3; ++ax) {
-            build_tables(c, s->prec, n3[ax], tab[ax], &epml, &isp, &pm);
-            pmax = std::max(pmax, pm);
-        }
-        // Half/main.cu:833-839
-        if (pmax > 0.0f) {
-            epml = (int)std::floor(std::log2(2048.0f / pmax));
-            epml = std::min(20, std::max(-20, epml));
-        }
-    }
-    for (int ax = 0; ax < 3; ++ax) {
-        float pm;
-        build_tables(c, s->prec, n3[ax], tab[ax], &epml, &isp, &pm);
-        slab_widths(tab[ax], n3[ax], &g.lo[ax], &g.hi[ax]);
-    }
-    s->epml = epml;
-    s->inv_scale_pml = isp;
-    const long long W = pitch;
-    g.slab_cells[0] = (long long)c.nz * c.ny * (g.lo[0] + g.hi[0]);
-    g.slab_cells[1] = (long long)c.nz * (g.lo[1] + g.hi[1]) * W;
-    g.slab_cells[2] = (long long)(g.lo[2] + g.hi[2]) * c.ny * W;
-
-    // One pooled allocation: [state | memory variables] (zeroed together) then
-    // [coefficients | tables].
-    const size_t sb = (size_t)state_bytes(s->prec), cb = (size_t)coef_bytes(s->prec), mb = (size_t)mem_bytes(s->prec);
-    const size_t rb = s->prec == PW_FP64 ? 8 : 4;
-    std::vector<size_t> off;
-    size_t total = 0;
-    auto take = [&](size_t bytes) {
-        off.push_back(total);
-        total += align256(bytes);
-    };
+    s->epml = L.epml;
+    s->inv_scale_pml = L.isp;
+    const std::vector<size_t>& off = L.off;
+    const size_t total = L.total;
     const bool c16 = s->prec == PW_COMP16;
-    // The fused one-launch step (double-buffered) is bit-identical to the two
-    // kernels but currently slower on MI355X (see the design notes); opt-in.
-    int fused = 0;
-    env_int("PW_FUSED", &fused);
-    const int nbuf = fused ? 2 : 1;  // the fused step reads time n and writes time n+1
-    for (int b2 = 0; b2 < nbuf; ++b2) {
-        for (int k = 0; k < PW_NFIELDS; ++k) take(cells * (c16 ? 2 : sb));
-        if (c16)
-            for (int k = 0; k < PW_NFIELDS; ++k) take(cells * 1);
-        for (int ax = 0; ax < 3; ++ax)
-            for (int k = 0; k < kMemPerAxis; ++k) take((size_t)g.slab_cells[ax] * mb);
-    }
-    const size_t zero_bytes = total;
     const int nc = ncoef(s->prec);
-    for (int k = 0; k < nc; ++k) take(cells * cb);
-    for (int ax = 0; ax < 3; ++ax) take((size_t)n3[ax] * 4 * rb);
+    (void)g;
 
     hipError_t e = hipMalloc(&s->pool, total);
     if (e != hipSuccess) {
@@ -265,7 +274,7 @@ extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
                         hipGetErrorString(e));
     }
     s->pool_bytes = total;
-    s->state_bytes_total = zero_bytes;
+    s->state_bytes_total = L.zero_bytes;
     char* base = (char*)s->pool;
     size_t i = 0;
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.f[k] = base + off[i++];
@@ -288,8 +297,6 @@ extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
     int xcd = 1;
     env_int("PW_XCD_MAP", &xcd);
     s->b.xcd_map = xcd ? 1 : 0;
-    int slab = 0;
-    env_int("PW_SLAB", &slab);
     s->b.slab = std::max(0, slab);
 
     if (c.stream) {
@@ -297,7 +304,7 @@ extern "C" pw_status pw_create(const pw_config* cfg, pw_solver** out)
     } else {
         e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
- (void)hipFree(s->pool);
+            (void)hipFree(s->pool);
             delete s;
             return fail(PW_ERR_HIP, std::string("pw_create: stream: ") + hipGetErrorString(e));
         }
@@ -338,24 +345,23 @@ extern "C" void pw_destroy(pw_solver* s)
     if (!s) return;
     (void)hipSetDevice(s->cfg.device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
- (void)hipFree(s->d_rfield);
- (void)hipFree(s->d_roff);
- (void)hipFree(s->d_seis);
- (void)hipFree(s->d_snaps);
- (void)hipFree(s->pool);
+    (void)hipFree(s->d_rfield);
+    (void)hipFree(s->d_roff);
+    (void)hipFree(s->d_seis);
+    (void)hipFree(s->d_snaps);
+    (void)hipFree(s->pool);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
-namespace {
-
-// Installs a medium given device-resident base fields (interior, FP64).
-pw_status install_medium(pw_solver* s, const double* d_base, int nxi, int nyi, int nzi)
+pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi, int nyi, int nzi,
+                             long long row_stride, long long plane_stride)
 {
     const pw_config& c = s->cfg;
-    MediumArgs m;
-    const size_t ni = (size_t)nxi * nyi * nzi;
-    for (int k = 0; k < PW_NBASE; ++k) m.base[k] = d_base + k * ni;
+    MediumArgs m{};
+    for (int k = 0; k < PW_NBASE; ++k) m.base[k] = d_base[k];
+    m.row_stride = row_stride;
+    m.plane_stride = plane_stride;
     m.nxi = nxi; m.nyi = nyi; m.nzi = nzi; m.np = c.np;
     m.nx = c.nx; m.ny = c.ny; m.nz = c.nz;
     m.dt = c.dt;
@@ -365,7 +371,7 @@ pw_status install_medium(pw_solver* s, const double* d_base, int nxi, int nyi, i
     hipError_t e = medium_reduce(m, d_res, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(res, d_res, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
- (void)hipFree(d_res);
+    (void)hipFree(d_res);
     if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_set_medium: reduction: ") + hipGetErrorString(e));
     if (!(res[0] > 0) || !std::isfinite(res[0]) || !std::isfinite(res[1]) || !std::isfinite(res[2]))
         return fail(PW_ERR_INVALID, "pw_set_medium: medium has no positive finite modulus");
@@ -397,6 +403,17 @@ pw_status install_medium(pw_solver* s, const double* d_base, int nxi, int nyi, i
     return pw_reset(s);
 }
 
+namespace {
+
+// Field pointers of a dense interior block of PW_NBASE fields.
+struct DenseBase {
+    const double* p[PW_NBASE];
+    DenseBase(const double* d, size_t ni)
+    {
+        for (int k = 0; k < PW_NBASE; ++k) p[k] = d + k * ni;
+    }
+};
+
 }  // namespace
 
 extern "C" pw_status pw_set_medium(pw_solver* s, const double* const* base, int nxi, int nyi, int nzi)
@@ -418,11 +435,11 @@ extern "C" pw_status pw_set_medium(pw_solver* s, const double* const* base, int 
         e = hipMemcpyAsync(d_base + k * ni, base[k], ni * sizeof(double), hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
- (void)hipFree(d_base);
+        (void)hipFree(d_base);
         return fail(PW_ERR_HIP, std::string("pw_set_medium: upload: ") + hipGetErrorString(e));
     }
-    st = install_medium(s, d_base, nxi, nyi, nzi);
- (void)hipFree(d_base);
+    st = install_medium(s, DenseBase(d_base, ni).p, nxi, nyi, nzi, 0, 0);
+    (void)hipFree(d_base);
     if (st == PW_OK) s->layered = false;
     return st;
 }
@@ -440,10 +457,10 @@ pw_status generate_into(pw_solver* s, const pw_layered* spec, double** d_base_ou
     if (e == hipSuccess) e = hipMalloc(&d_part, 2 * 4096 * sizeof(double));
     if (e == hipSuccess) e = generate_layered(*spec, nxi, nyi, nzi, c.np, d_base, d_work, d_part, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
- (void)hipFree(d_work);
- (void)hipFree(d_part);
+    (void)hipFree(d_work);
+    (void)hipFree(d_part);
     if (e != hipSuccess) {
- (void)hipFree(d_base);
+        (void)hipFree(d_base);
         return fail(e == hipErrorOutOfMemory ? PW_ERR_OUT_OF_MEMORY : PW_ERR_HIP,
                     std::string("pw_set_medium_layered: generator: ") + hipGetErrorString(e));
     }
@@ -479,8 +496,9 @@ extern "C" pw_status pw_set_medium_layered(pw_solver* s, const pw_layered* spec)
     double* d_base = nullptr;
     if ((st = generate_into(s, spec, &d_base)) != PW_OK) return st;
     const pw_config& c = s->cfg;
-    st = install_medium(s, d_base, c.nx - 2 * c.np, c.ny - 2 * c.np, c.nz - 2 * c.np);
- (void)hipFree(d_base);
+    const int nxi = c.nx - 2 * c.np, nyi = c.ny - 2 * c.np, nzi = c.nz - 2 * c.np;
+    st = install_medium(s, DenseBase(d_base, (size_t)nxi * nyi * nzi).p, nxi, nyi, nzi, 0, 0);
+    (void)hipFree(d_base);
     if (st == PW_OK) {
         s->layered = true;
         s->spec = *spec;
@@ -500,7 +518,7 @@ extern "C" pw_status pw_get_base_fields(pw_solver* s, double* out)
     const pw_config& c = s->cfg;
     const size_t ni = (size_t)(c.nx - 2 * c.np) * (c.ny - 2 * c.np) * (c.nz - 2 * c.np);
     hipError_t e = hipMemcpy(out, d_base, PW_NBASE * ni * sizeof(double), hipMemcpyDeviceToHost);
- (void)hipFree(d_base);
+    (void)hipFree(d_base);
     if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_get_base_fields: ") + hipGetErrorString(e));
     return PW_OK;
 }
@@ -537,7 +555,7 @@ extern "C" pw_status pw_set_receivers(pw_solver* s, int n, const int* field, con
     }
     PW_HIP_TRY(hipSetDevice(c.device));
     PW_HIP_TRY(hipStreamSynchronize(s->stream));
- (void)hipFree(s->d_rfield); (void)hipFree(s->d_roff); (void)hipFree(s->d_seis);
+    (void)hipFree(s->d_rfield); (void)hipFree(s->d_roff); (void)hipFree(s->d_seis);
     s->d_rfield = nullptr; s->d_roff = nullptr; s->d_seis = nullptr;
     s->nrec = 0;
     if (n == 0) return PW_OK;
@@ -582,7 +600,7 @@ extern "C" pw_status pw_set_snapshots(pw_solver* s, int n, const int* step, cons
             return fail(PW_ERR_INVALID, "pw_set_snapshots: snapshot " + std::to_string(k) + " out of range");
     PW_HIP_TRY(hipSetDevice(c.device));
     PW_HIP_TRY(hipStreamSynchronize(s->stream));
- (void)hipFree(s->d_snaps);
+    (void)hipFree(s->d_snaps);
     s->d_snaps = nullptr;
     s->nsnap = 0;
     s->snap_step.assign(step, step + n);
@@ -624,6 +642,32 @@ float dynamic_scale_for_step(int it, int nt, float end)
 
 }  // namespace
 
+StepPlan pw::plan_step(const pw_solver* s, int it, float* dyn)
+{
+    const pw_config& c = s->cfg;
+    const bool scaled = is_scaled(s->prec);
+    StepPlan p{0.0f, 0.0, 1.0f, 1.0f};
+    if (scaled && (it == 0 || (c.half_dyn_interval > 0 && it % c.half_dyn_interval == 0))) {
+        const float dyn_end = c.half_dyn_end > 0 ? (float)c.half_dyn_end : 16.0f;
+        const float target = dynamic_scale_for_step(it, c.nt, dyn_end);
+        const float ratio = target / *dyn;
+        if (std::fabs(ratio - 1.0f) > 1.0e-6f) {
+            p.ratio = ratio;
+            *dyn = target;
+        }
+    }
+    if (s->has_source) {
+        if (s->prec == PW_FP64) p.amp = ricker_f64(it, c.dt, s->f0);
+        else if (scaled) p.amp = *dyn * s->source_scale * ricker_f32(it, (float)c.dt, (float)s->f0);
+        else p.amp = ricker_f32(it, (float)c.dt, (float)s->f0);
+    }
+    if (scaled) {
+        p.s_stress = s->stress_base / *dyn;
+        p.s_vel = s->vel_base / *dyn;
+    }
+    return p;
+}
+
 extern "C" pw_status pw_run(pw_solver* s, int nsteps)
 {
     pw_status st = check_handle(s, "pw_run");
@@ -633,25 +677,11 @@ extern "C" pw_status pw_run(pw_solver* s, int nsteps)
     if (nsteps < 0 || s->step + nsteps > c.nt)
         return fail(PW_ERR_INVALID, "pw_run: steps beyond the configured nt");
     PW_HIP_TRY(hipSetDevice(c.device));
-    const bool scaled = is_scaled(s->prec);
-    const float dyn_end = c.half_dyn_end > 0 ? (float)c.half_dyn_end : 16.0f;
     for (int k = 0; k < nsteps; ++k) {
         const int it = s->step;
-        if (scaled && (it == 0 || (c.half_dyn_interval > 0 && it % c.half_dyn_interval == 0))) {
-            const float target = dynamic_scale_for_step(it, c.nt, dyn_end);
-            const float ratio = target / s->dyn_scale;
-            if (std::fabs(ratio - 1.0f) > 1.0e-6f) {
-                PW_HIP_TRY(launch_rescale(s->b, s->stream, ratio));
-                s->dyn_scale = target;
-            }
-        }
-        if (s->has_source) {
-            double amp;
-            if (s->prec == PW_FP64) amp = ricker_f64(it, c.dt, s->f0);
-            else if (scaled) amp = s->dyn_scale * s->source_scale * ricker_f32(it, (float)c.dt, (float)s->f0);
-            else amp = ricker_f32(it, (float)c.dt, (float)s->f0);
-            PW_HIP_TRY(launch_source(s->b, s->stream, s->src_off, s->w, amp));
-        }
+        const StepPlan plan = plan_step(s, it, &s->dyn_scale);
+        if (plan.ratio != 0.0f) PW_HIP_TRY(launch_rescale(s->b, s->stream, plan.ratio));
+        if (s->has_source) PW_HIP_TRY(launch_source(s->b, s->stream, s->src_off, s->w, plan.amp));
         const int nz = c.nz;
         // one step kernel launch, bracketed by events when profiling
         auto step_kernel = [&](int kind, int zb, int ze) -> pw_status {
@@ -710,8 +740,7 @@ extern "C" pw_status pw_run(pw_solver* s, int nsteps)
             }
         }
         if (s->profiling) s->prof_steps++;
-        const float ss = scaled ? s->stress_base / s->dyn_scale : 1.0f;
-        const float sv = scaled ? s->vel_base / s->dyn_scale : 1.0f;
+        const float ss = plan.s_stress, sv = plan.s_vel;
         if (s->nrec > 0)
             PW_HIP_TRY(launch_gather(s->b, s->stream, s->d_rfield, s->d_roff, s->nrec, s->d_seis, c.nt, it, ss, sv));
         for (int q = 0; q < s->nsnap; ++q)
@@ -776,7 +805,7 @@ extern "C" pw_status pw_get_field(pw_solver* s, int field, float* host)
     hipError_t e = launch_decode(s->b, s->stream, field, d, sc);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e == hipSuccess) e = hipMemcpy(host, d, bytes, hipMemcpyDeviceToHost);
- (void)hipFree(d);
+    (void)hipFree(d);
     if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_get_field: ") + hipGetErrorString(e));
     return PW_OK;
 }

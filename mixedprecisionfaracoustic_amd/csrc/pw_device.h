@@ -65,10 +65,52 @@ hipError_t launch_decode(const DevBuffers& b, hipStream_t st, int field, float* 
 hipError_t launch_rescale(const DevBuffers& b, hipStream_t st, float ratio);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);
 
+// ---- multi-shot step (shot server) ----
+// One member of a batch: the solver buffers of one shot plus its per-step
+// tables, all device pointers.  amp is R-typed (float, double for FP64).
+struct BatchShot {
+    const DevBuffers* b;
+    const void* amp;        // [nt] source amplitude
+    const float* oscale;    // [nt][2] receiver scale: stress, velocity
+    const float* ratio;     // [nt] dynamic rescale ratio, 0 = none (null for FP32/FP64)
+    const int* rfield;
+    const unsigned* roff;
+    float* seis;            // [nrec][nt]
+    int nrec, nt;
+    unsigned src_off;
+    int has_src;
+    double w[4];
+};
+// Device tables of one batch: the shots of one precision, advanced together.
+// The step of shot s during a launch is it[s] + k, k = step within the round;
+// shots at their last step are skipped, so every launch can be captured once
+// and replayed.
+struct BatchGroup {
+    int prec = -1;
+    int n = 0;
+    void* d_descs = nullptr;     // per-shot descriptors (policy-typed)
+    unsigned* d_bstart = nullptr;  // [n+1] step-kernel block prefix
+    unsigned* d_xstart = nullptr;  // [8][n+1] per-XCD block prefix (XCD-aware map)
+    unsigned* d_rstart = nullptr;  // [n+1] receiver prefix
+    int* d_it = nullptr;         // [n] step of each shot at the start of the round
+    int* d_nt = nullptr;         // [n] steps of each shot
+    unsigned nblocks = 0, nrec = 0;
+    int xcd_map = 1;
+    size_t bytes = 0;
+};
+// (Re)builds g for the given shots (all of precision prec) at their current steps.
+hipError_t batch_build(BatchGroup& g, int prec, const BatchShot* shots, int n, const int* it, hipStream_t st);
+void batch_free(BatchGroup& g);
+// One step (rescale, source, velocity, stress, receivers) of every shot of g at
+// round offset k, and the end-of-round advance it[s] += K.
+hipError_t batch_step(const BatchGroup& g, hipStream_t st, int k);
+hipError_t batch_advance(const BatchGroup& g, hipStream_t st, int K);
+
 // ---- medium precompute (pw_medium_gpu.hip), all FP64 on the device ----
 struct MediumArgs {
-    const double* base[PW_NBASE];  // interior fields nxi*nyi*nzi
+    const double* base[PW_NBASE];  // interior fields, x fastest
     int nxi, nyi, nzi, np;
+    long long row_stride, plane_stride;  // elements; 0 = dense nxi, nxi*nyi (a crop of a larger volume otherwise)
     int nx, ny, nz;                // total grid
     double dt;
 };

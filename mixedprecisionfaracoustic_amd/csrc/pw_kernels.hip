@@ -27,7 +27,9 @@
 // except the FP64 policy.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <vector>
 
 #include "pw_device.h"
 
@@ -201,21 +203,25 @@ __device__ __forceinline__ int slab_index(int i, int n, int lo, int hi)
 // then z, so the x and y neighbour tiles whose halo rows a tile re-reads were
 // just fetched into the same L2.
 struct Tile { int bx, by, bz; };
-__device__ __forceinline__ Tile tile_of_block(int gx, int gy, int nblocks, int xcd_map)
+__device__ __forceinline__ unsigned xcd_remap(unsigned L, unsigned nblocks, int xcd_map)
 {
-    const unsigned L = blockIdx.x;
-    unsigned t = L;
-    if (xcd_map) {
-        const unsigned q = (unsigned)nblocks / 8u, r = (unsigned)nblocks % 8u;
-        const unsigned k = L % 8u, j = L / 8u;
-        t = k * q + (k < r ? k : r) + j;
-    }
+    if (!xcd_map) return L;
+    const unsigned q = nblocks / 8u, r = nblocks % 8u;
+    const unsigned k = L % 8u, j = L / 8u;
+    return k * q + (k < r ? k : r) + j;
+}
+__device__ __forceinline__ Tile tile_of_index(unsigned t, int gx, int gy)
+{
     Tile tl;
     tl.bx = (int)(t % (unsigned)gx);
     const unsigned row = t / (unsigned)gx;
     tl.by = (int)(row % (unsigned)gy);
     tl.bz = (int)(row / (unsigned)gy);
     return tl;
+}
+__device__ __forceinline__ Tile tile_of_block(int gx, int gy, int nblocks, int xcd_map)
+{
+    return tile_of_index(xcd_remap(blockIdx.x, (unsigned)nblocks, xcd_map), gx, gy);
 }
 
 // One CPML term in single-variable form (see header comment).
@@ -274,10 +280,9 @@ template <class P> struct Deriv {
 // Velocity kernel: Float/main.cu:64-192 (physical), Half/main.cu:156-268 (scaled)
 // --------------------------------------------------------------------------
 template <class P>
-__global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
+__device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile tl)
 {
     using R = typename P::R;
-    const Tile tl = tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map);
     const int ix = tl.bx * 64 + threadIdx.x;
     const int iy = tl.by * 4 + threadIdx.y;
     if (ix < 1 || ix > a.nx - 2 || iy < 1 || iy > a.ny - 2) return;
@@ -399,10 +404,9 @@ __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
 // Stress kernel: Float/main.cu:194-295 (physical), Half/main.cu:270-371 (scaled)
 // --------------------------------------------------------------------------
 template <class P>
-__global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
+__device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
 {
     using R = typename P::R;
-    const Tile tl = tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map);
     const int ix = tl.bx * 64 + threadIdx.x;
     const int iy = tl.by * 4 + threadIdx.y;
     if (ix < 1 || ix > a.nx - 2 || iy < 1 || iy > a.ny - 2) return;
@@ -511,6 +515,15 @@ __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
 
         vuz_m = vuz_c; vwz_m = vwz_c; vux_c = vux_n; vuy_c = vuy_n;
     }
+}
+
+template <class P> __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
+{
+    velocity_tile<P>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
+}
+template <class P> __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
+{
+    stress_tile<P>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
 }
 
 // --------------------------------------------------------------------------
@@ -891,6 +904,172 @@ __global__ void k_zero(uint4* __restrict__ p, unsigned long long n16)
 }
 
 // --------------------------------------------------------------------------
+// Multi-shot step: every shot of one precision advanced by one launch per
+// kernel.  A descriptor table holds each shot's arguments; the launch's blocks
+// are the concatenation of the shots' tile grids (block prefix bstart), dealt to
+// the XCDs as one grid.  Each shot's current step is it[s] + k (k = step within
+// the round), read on the device, so a round of launches can be captured once
+// and replayed; a shot past its last step skips the work.  The per-cell code is
+// the single-shot code (velocity_tile / stress_tile): results are bit-identical
+// to a shot run alone.
+// --------------------------------------------------------------------------
+template <class P> struct ShotDesc {
+    StepArgs<P> a;
+    const typename P::R* amp;
+    const float* oscale;
+    const float* ratio;
+    const int* rfield;
+    const unsigned* roff;
+    float* seis;
+    int nrec, nt;
+    unsigned src_off;
+    int has_src;
+    typename P::R w[4];
+    unsigned long long ncell, nmem[3];
+};
+
+struct BatchArgs {
+    const void* descs;
+    const unsigned* bstart;  // [n+1] block prefix (without the XCD map)
+    const unsigned* xstart;  // [8][n+1] per-XCD block prefix (with the XCD map)
+    const unsigned* rstart;
+    int* it;
+    int n;
+    unsigned nblocks, nrec;
+    int xcd_map;
+};
+
+// largest s with start[s] <= t (start ascending, start[0] = 0)
+__device__ __forceinline__ int find_shot(const unsigned* __restrict__ start, int n, unsigned t)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Block -> (shot, tile index within the shot).  With the XCD map, XCD x (the
+// blocks L with L % 8 == x) gets the x-th eighth of EVERY shot's tiles, each
+// eighth contiguous in the shot's x-y-z tile order: the XCDs share the work
+// evenly whatever the mix of grid sizes (blocks of large shots march more
+// planes), and each XCD's L2 still sees neighbouring tiles of one shot in turn.
+// Returns false for the spare blocks of the XCDs with fewer tiles.
+template <class P> __device__ __forceinline__ bool batch_block(const BatchArgs& c, int& s, unsigned& local)
+{
+    const ShotDesc<P>* D = (const ShotDesc<P>*)c.descs;
+    if (!c.xcd_map) {
+        const unsigned t = blockIdx.x;
+        s = find_shot(c.bstart, c.n, t);
+        local = t - c.bstart[s];
+        return true;
+    }
+    const unsigned x = blockIdx.x % 8u, j = blockIdx.x / 8u;
+    const unsigned* xs = c.xstart + x * (unsigned)(c.n + 1);
+    if (j >= xs[c.n]) return false;
+    s = find_shot(xs, c.n, j);
+    const unsigned nb = (unsigned)D[s].a.nblocks, q = nb / 8u, r = nb % 8u;
+    local = x * q + (x < r ? x : r) + (j - xs[s]);
+    return true;
+}
+
+template <class P> __global__ __launch_bounds__(256) void k_velocity_batch(const BatchArgs c, int k)
+{
+    int s;
+    unsigned t;
+    if (!batch_block<P>(c, s, t)) return;
+    const ShotDesc<P>* d = (const ShotDesc<P>*)c.descs + s;
+    if (c.it[s] + k >= d->nt) return;
+    const StepArgs<P> a = d->a;
+    velocity_tile<P>(a, tile_of_index(t, a.gx, a.gy));
+}
+
+template <class P> __global__ __launch_bounds__(256) void k_stress_batch(const BatchArgs c, int k)
+{
+    int s;
+    unsigned t;
+    if (!batch_block<P>(c, s, t)) return;
+    const ShotDesc<P>* d = (const ShotDesc<P>*)c.descs + s;
+    if (c.it[s] + k >= d->nt) return;
+    const StepArgs<P> a = d->a;
+    stress_tile<P>(a, tile_of_index(t, a.gx, a.gy));
+}
+
+// one thread per (shot, source channel); the arithmetic of k_source
+template <class P> __global__ void k_source_batch(const BatchArgs c, int k)
+{
+    using R = typename P::R;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = g >> 2, ch = g & 3;
+    if (s >= c.n) return;
+    const ShotDesc<P>& d = ((const ShotDesc<P>*)c.descs)[s];
+    const int it = c.it[s] + k;
+    if (it >= d.nt || !d.has_src || d.w[ch] == (R)0) return;
+    const int fid[4] = {PW_TXX, PW_TYY, PW_TZZ, PW_P};
+    const typename P::Field f = d.a.f[fid[ch]];
+    const R amp = d.amp[it];
+    P::st(f, d.src_off, P::ld(f, d.src_off) + d.w[ch] * amp);
+}
+
+// Multiplies array arr (0..12 wavefields, then the memory variables) of one
+// shot by ratio over the indices i0, i0 + stride, ...  (k_rescale's arithmetic)
+template <class P>
+__device__ __forceinline__ void rescale_array(const StepArgs<P>& a, int arr, unsigned long long i0,
+                                              unsigned long long stride, unsigned long long ncell,
+                                              const unsigned long long nmem[3], float ratio)
+{
+    if (arr < PW_NFIELDS) {
+        const typename P::Field f = a.f[arr];
+        for (unsigned long long i = i0; i < ncell; i += stride) P::st(f, (unsigned)i, P::ld(f, (unsigned)i) * ratio);
+    } else {
+        const int q = arr - PW_NFIELDS;
+        const int ax = q / kMemPerAxis, t = q % kMemPerAxis;
+        typename P::Mem* m = a.m[ax][t];
+        if (!m) return;
+        for (unsigned long long i = i0; i < nmem[ax]; i += stride) P::stm(m, (unsigned)i, P::ldm(m, (unsigned)i) * ratio);
+    }
+}
+
+// grid (blocks per shot, shots); idle unless the shot rescales at this step
+template <class P> __global__ void k_rescale_batch(const BatchArgs c, int k)
+{
+    const int s = blockIdx.y;
+    const ShotDesc<P>& d = ((const ShotDesc<P>*)c.descs)[s];
+    const int it = c.it[s] + k;
+    if (it >= d.nt || !d.ratio) return;
+    const float ratio = d.ratio[it];
+    if (ratio == 0.0f) return;
+    const StepArgs<P> a = d.a;
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    const unsigned long long i0 = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int arr = 0; arr < PW_NFIELDS + 3 * kMemPerAxis; ++arr) rescale_array<P>(a, arr, i0, stride, d.ncell, d.nmem, ratio);
+}
+
+// one thread per receiver of the batch; the arithmetic of k_gather
+template <class P> __global__ void k_gather_batch(const BatchArgs c, int k)
+{
+    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= c.nrec) return;
+    const int s = find_shot(c.rstart, c.n, r);
+    const ShotDesc<P>& d = ((const ShotDesc<P>*)c.descs)[s];
+    const int it = c.it[s] + k;
+    if (it >= d.nt) return;
+    const unsigned q = r - c.rstart[s];
+    const int fid = d.rfield[q];
+    const float v = (float)ld_any<P>(d.a, fid, d.roff[q]);
+    const float s_stress = d.oscale[2 * it], s_vel = d.oscale[2 * it + 1];
+    d.seis[(size_t)q * d.nt + it] = (fid <= PW_VWZ ? s_vel : s_stress) * v;
+}
+
+__global__ void k_advance(int* it, const int* __restrict__ nt, int n, int K)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) it[s] = min(it[s] + K, nt[s]);
+}
+
+// --------------------------------------------------------------------------
 // Launchers
 // --------------------------------------------------------------------------
 // Restricts a step launch to planes [zb, ze).
@@ -1018,6 +1197,129 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t st)
     unsigned blocks = (unsigned)((n16 + 255) / 256);
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, st, (uint4*)p, n16);
+    return hipGetLastError();
+}
+
+// ---- multi-shot launchers ----
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+void batch_free(BatchGroup& g)
+{
+    if (g.d_descs) (void)hipFree(g.d_descs);
+    g = BatchGroup{};
+}
+
+hipError_t batch_build(BatchGroup& g, int prec, const BatchShot* shots, int n, const int* it, hipStream_t st)
+{
+    if (n <= 0) {
+        g.n = 0;
+        g.prec = prec;
+        return hipSuccess;
+    }
+    return dispatch(prec, [&](auto pol) -> hipError_t {
+        using P = decltype(pol);
+        using R = typename P::R;
+        std::vector<ShotDesc<P>> d((size_t)n);
+        std::vector<unsigned> bstart((size_t)n + 1, 0u), rstart((size_t)n + 1, 0u);
+        std::vector<int> ntv((size_t)n), itv(it, it + n);
+        for (int s = 0; s < n; ++s) {
+            const BatchShot& b = shots[s];
+            if (b.b->prec != prec || b.b->fused) return hipErrorInvalidValue;
+            ShotDesc<P>& q = d[(size_t)s];
+            q.a = make_args<P>(*b.b);
+            q.amp = (const R*)b.amp;
+            q.oscale = b.oscale;
+            q.ratio = b.ratio;
+            q.rfield = b.rfield;
+            q.roff = b.roff;
+            q.seis = b.seis;
+            q.nrec = b.nrec;
+            q.nt = b.nt;
+            q.src_off = b.src_off;
+            q.has_src = b.has_src;
+            for (int c = 0; c < 4; ++c) q.w[c] = (R)b.w[c];
+            q.ncell = (unsigned long long)b.b->g.cells;
+            for (int ax = 0; ax < 3; ++ax) q.nmem[ax] = (unsigned long long)b.b->g.slab_cells[ax];
+            bstart[(size_t)s + 1] = bstart[(size_t)s] + (unsigned)q.a.nblocks;
+            rstart[(size_t)s + 1] = rstart[(size_t)s] + (unsigned)b.nrec;
+            ntv[(size_t)s] = b.nt;
+        }
+        // per-XCD prefix: XCD x holds q + (x < r) tiles of a shot with q*8 + r tiles
+        std::vector<unsigned> xstart((size_t)8 * (n + 1), 0u);
+        unsigned xmax = 0;
+        for (int x = 0; x < 8; ++x) {
+            unsigned* xs = xstart.data() + (size_t)x * (n + 1);
+            for (int s = 0; s < n; ++s) {
+                const unsigned nb = (unsigned)d[(size_t)s].a.nblocks;
+                xs[s + 1] = xs[s] + nb / 8u + ((unsigned)x < nb % 8u ? 1u : 0u);
+            }
+            xmax = std::max(xmax, xs[n]);
+        }
+        const size_t o_desc = 0, o_b = al256(sizeof(ShotDesc<P>) * n), o_x = o_b + al256(4 * (n + 1)),
+                     o_r = o_x + al256(4 * 8 * (n + 1)), o_it = o_r + al256(4 * (n + 1)), o_nt = o_it + al256(4 * n),
+                     total = o_nt + al256(4 * n);
+        if (g.bytes < total) {
+            batch_free(g);
+            hipError_t e = hipMalloc(&g.d_descs, total);
+            if (e != hipSuccess) return e;
+            g.bytes = total;
+        }
+        char* base = (char*)g.d_descs;
+        g.prec = prec;
+        g.n = n;
+        g.d_bstart = (unsigned*)(base + o_b);
+        g.d_xstart = (unsigned*)(base + o_x);
+        g.d_rstart = (unsigned*)(base + o_r);
+        g.d_it = (int*)(base + o_it);
+        g.d_nt = (int*)(base + o_nt);
+        g.xcd_map = shots[0].b->xcd_map;
+        g.nblocks = g.xcd_map ? 8u * xmax : bstart[(size_t)n];
+        g.nrec = rstart[(size_t)n];
+        hipError_t e = hipMemcpyAsync(base + o_desc, d.data(), sizeof(ShotDesc<P>) * n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(g.d_bstart, bstart.data(), 4 * (n + 1), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(g.d_xstart, xstart.data(), 4 * 8 * (n + 1), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(g.d_rstart, rstart.data(), 4 * (n + 1), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(g.d_it, itv.data(), 4 * n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(g.d_nt, ntv.data(), 4 * n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);  // the host tables go out of scope
+        return e;
+    });
+}
+
+static BatchArgs batch_args(const BatchGroup& g)
+{
+    BatchArgs c;
+    c.descs = g.d_descs;
+    c.bstart = g.d_bstart;
+    c.xstart = g.d_xstart;
+    c.rstart = g.d_rstart;
+    c.it = g.d_it;
+    c.n = g.n;
+    c.nblocks = g.nblocks;
+    c.nrec = g.nrec;
+    c.xcd_map = g.xcd_map;
+    return c;
+}
+
+hipError_t batch_step(const BatchGroup& g, hipStream_t st, int k)
+{
+    if (g.n <= 0) return hipSuccess;
+    const BatchArgs c = batch_args(g);
+    return dispatch(g.prec, [&](auto pol) -> hipError_t {
+        using P = decltype(pol);
+        if (P::scaled) k_rescale_batch<P><<<dim3(64, (unsigned)g.n), 256, 0, st>>>(c, k);
+        k_source_batch<P><<<(unsigned)((4 * g.n + 255) / 256), 256, 0, st>>>(c, k);
+        k_velocity_batch<P><<<g.nblocks, dim3(64, 4), 0, st>>>(c, k);
+        k_stress_batch<P><<<g.nblocks, dim3(64, 4), 0, st>>>(c, k);
+        if (g.nrec > 0) k_gather_batch<P><<<(g.nrec + 255) / 256, 256, 0, st>>>(c, k);
+        return hipGetLastError();
+    });
+}
+
+hipError_t batch_advance(const BatchGroup& g, hipStream_t st, int K)
+{
+    if (g.n <= 0) return hipSuccess;
+    k_advance<<<(unsigned)((g.n + 255) / 256), 256, 0, st>>>(g.d_it, g.d_nt, g.n, K);
     return hipGetLastError();
 }
 

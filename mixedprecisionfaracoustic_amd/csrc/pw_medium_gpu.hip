@@ -21,6 +21,7 @@ typedef _Float16 f16;
 struct BaseView {
     const double* b[PW_NBASE];
     int nxi, nyi, nzi, np;
+    long long rs, ps;  // element strides of a row and a plane of the base arrays
     // padded (extmodel) value of field f at total-grid cell (ix, iy, iz)
     __device__ __forceinline__ double at(int f, int ix, int iy, int iz) const
     {
@@ -28,7 +29,7 @@ struct BaseView {
         kx = kx < 0 ? 0 : (kx >= nxi ? nxi - 1 : kx);
         ky = ky < 0 ? 0 : (ky >= nyi ? nyi - 1 : ky);
         kz = kz < 0 ? 0 : (kz >= nzi ? nzi - 1 : kz);
-        return b[f][(size_t)kx + (size_t)ky * nxi + (size_t)kz * nxi * nyi];
+        return b[f][(long long)kx + (long long)ky * rs + (long long)kz * ps];
     }
 };
 
@@ -232,6 +233,8 @@ static BaseView make_view(const MediumArgs& m)
     BaseView v;
     for (int k = 0; k < PW_NBASE; ++k) v.b[k] = m.base[k];
     v.nxi = m.nxi; v.nyi = m.nyi; v.nzi = m.nzi; v.np = m.np;
+    v.rs = m.row_stride > 0 ? m.row_stride : m.nxi;
+    v.ps = m.plane_stride > 0 ? m.plane_stride : (long long)m.nxi * m.nyi;
     return v;
 }
 

@@ -1,0 +1,181 @@
+"""Shot server on the GPU: every shot's seismogram is the same in the mix as
+alone and as through the single-shot Solver (bit-identical: the server runs the
+same per-cell code), admission follows the memory budget and the slot count,
+and a failing shot does not disturb the others."""
+import numpy as np
+import pytest
+
+import cases
+import mixedprecisionfaracoustic_amd as pw
+
+pytestmark = pytest.mark.gpu
+
+NP = 6
+H, DT = 5.0, 0.5e-3
+VOL = (48, 40, 44)  # medium volume interior (nz, ny, nx)
+
+
+@pytest.fixture(scope="module")
+def volume():
+    """A two-layer perturbed medium volume, generated on the GPU by the library."""
+    nzi, nyi, nxi = VOL
+    spec = pw.layered([0, nzi // 2], [cases.sandstone(), cases.lower_layer()], perturb=0.05, corr_cells=3.0,
+                      seed=7)
+    s = pw.Solver(nxi + 2 * NP, nyi + 2 * NP, nzi + 2 * NP, NP, H, DT, 2, cpml_r=1e-4)
+    s.set_medium_layered(spec)
+    base = s.base_fields()
+    s.close()
+    return base
+
+
+def shot_list():
+    """Shots of different sizes, precisions, lengths, crops and sources."""
+    out = []
+    specs = [((28, 26, 30), "fp32", 120, (3, 5, 7)), ((32, 32, 32), "comp16", 150, (10, 2, 4)),
+             ((36, 30, 34), "fp32", 64, (0, 0, 0)), ((30, 34, 28), "bf16", 90, (20, 6, 12)),
+             ((26, 28, 36), "fp64", 75, (8, 10, 20)), ((34, 28, 30), "comp16", 101, (5, 9, 11))]
+    for k, ((nx, ny, nz), prec, nt, crop) in enumerate(specs):
+        src = (nx // 2 - 1 + k % 3, ny // 2 + 1 - k % 2, nz // 2 - 2 + k % 4)
+        iy, iz = ny // 2 + 2, NP + 3
+        fields, idx = [], []
+        for f in ("txx", "vux", "vwz", "p", "txz"):
+            for ix in range(1, nx - 1, 2):
+                fields.append(f)
+                idx.append(ix + iy * nx + iz * nx * ny)
+        out.append(dict(nx=nx, ny=ny, nz=nz, prec=prec, nt=nt, crop=crop, src=src, f0=30.0 + 2 * k,
+                        w=(1.0, 1.0, 1.0, -1.0 if k % 2 else 0.5), fields=fields, idx=idx))
+    return out
+
+
+def submit(srv, q, nt=None, dt=DT):
+    return srv.submit(q["nx"], q["ny"], q["nz"], NP, H, dt, q["nt"] if nt is None else nt, q["prec"], q["crop"],
+                      q["src"], q["f0"], q["fields"], q["idx"], weights=q["w"], cpml_r=1e-4, cpml_v0=4000.0)
+
+
+def solver_run(volume, q):
+    nx, ny, nz = q["nx"], q["ny"], q["nz"]
+    cx, cy, cz = q["crop"]
+    crop = volume[:, cz:cz + nz - 2 * NP, cy:cy + ny - 2 * NP, cx:cx + nx - 2 * NP]
+    s = pw.Solver(nx, ny, nz, NP, H, DT, q["nt"], precision=q["prec"], cpml_r=1e-4, cpml_v0=4000.0)
+    s.set_medium(crop)
+    s.set_source(*q["src"], q["f0"], q["w"])
+    s.set_receivers(q["fields"], q["idx"])
+    s.run(q["nt"])
+    out = s.seismogram()
+    s.close()
+    return out
+
+
+def run_server(volume, shots, **kw):
+    with pw.Server(**kw) as srv:
+        srv.set_medium(volume)
+        ids = [submit(srv, q) for q in shots]
+        res = []
+        for i in ids:
+            srv.wait(i, timeout=120)
+            res.append(srv.result(i))
+        st = [srv.status(i) for i in ids]
+        stats = srv.stats()
+    return res, st, stats
+
+
+@pytest.fixture(scope="module")
+def alone(volume):
+    shots = shot_list()
+    return shots, [solver_run(volume, q) for q in shots]
+
+
+def test_mix_matches_solver_bitwise(volume, alone):
+    shots, ref = alone
+    res, st, stats = run_server(volume, shots, max_active=8, round_steps=16)
+    print("stats", stats)
+    assert stats["completed"] == len(shots) and stats["failed"] == 0
+    assert stats["peak_active"] == len(shots)
+    assert stats["graphs_built"] >= 1
+    for q, a, b in zip(shots, res, ref):
+        assert np.isfinite(a).all() and np.abs(a).max() > 0, q["prec"]
+        assert np.array_equal(a, b), (q["prec"], cases.rel_l2(a, b))
+
+
+def test_each_shot_alone_through_the_server_is_identical(volume, alone):
+    shots, ref = alone
+    with pw.Server(max_active=1, round_steps=7) as srv:
+        srv.set_medium(volume)
+        for q, b in zip(shots, ref):
+            i = submit(srv, q)
+            srv.wait(i, timeout=120)
+            assert np.array_equal(srv.result(i), b), q["prec"]
+            srv.release(i)
+
+
+@pytest.mark.parametrize("kw", [dict(use_graphs=False), dict(batched=False), dict(round_steps=1)])
+def test_round_modes_are_identical(volume, alone, kw):
+    shots, ref = alone
+    res, _, _ = run_server(volume, shots, max_active=4, **kw)
+    for q, a, b in zip(shots, res, ref):
+        assert np.array_equal(a, b), (kw, q["prec"])
+
+
+def test_continuous_batching_admits_as_slots_free(volume, alone):
+    """Three slots for six shots of different lengths: shots join while others
+    are mid-run, first come first served, and still match the Solver."""
+    shots, ref = alone
+    res, st, stats = run_server(volume, shots, max_active=3, round_steps=8)
+    assert stats["peak_active"] == 3
+    for a, b in zip(res, ref):
+        assert np.array_equal(a, b)
+    admits = [s["t_admit"] for s in st]
+    assert admits == sorted(admits)  # FCFS
+    # some shot was admitted while an earlier one was still running
+    overlap = any(st[j]["t_admit"] < st[i]["t_done"] for i in range(len(st)) for j in range(i + 1, len(st))
+                  if st[j]["t_admit"] > st[i]["t_admit"])
+    assert overlap
+
+
+def test_memory_budget_limits_admission(volume, alone):
+    """A budget that holds either of two shots but not both admits one at a time."""
+    shots, ref = alone
+    with pw.Server(max_active=8) as probe:
+        probe.set_medium(volume)
+        ids = [submit(probe, q) for q in shots[:2]]
+        for i in ids:
+            probe.wait(i, timeout=120)
+        need = [probe.status(i)["device_bytes"] for i in ids]
+    budget = int(max(need) * 1.2)
+    assert sum(need) > budget
+    with pw.Server(max_active=8, hbm_budget=budget) as srv:
+        srv.set_medium(volume)
+        ids = [submit(srv, q) for q in shots[:2]] + [submit(srv, shots[0])]
+        for i in ids:
+            srv.wait(i, timeout=120)
+        stats = srv.stats()
+        assert stats["peak_bytes"] <= budget
+        assert np.array_equal(srv.result(ids[0]), ref[0])
+        assert np.array_equal(srv.result(ids[1]), ref[1])
+        assert np.array_equal(srv.result(ids[2]), ref[0])
+    assert stats["peak_active"] == 1
+
+
+def test_failing_shots_do_not_disturb_others(volume, alone):
+    shots, ref = alone
+    with pw.Server(max_active=8, hbm_budget=1 << 30) as srv:
+        srv.set_medium(volume)
+        good = submit(srv, shots[0])
+        unstable = submit(srv, shots[2], dt=5e-3)  # Courant number above 1
+        with pytest.raises(pw.PorowaveError):  # crop outside the volume: refused at submission
+            submit(srv, dict(shots[3], nx=400, ny=400, nz=400, crop=(0, 0, 0)))
+        good2 = submit(srv, shots[3])
+        with pytest.raises(pw.PorowaveError) as ei:
+            srv.wait(unstable, timeout=120)
+        assert ei.value.status == 5  # PW_ERR_UNSTABLE
+        assert srv.status(unstable)["state"] == "failed"
+        srv.wait(good, timeout=120)
+        srv.wait(good2, timeout=120)
+        assert np.array_equal(srv.result(good), ref[0])
+        assert np.array_equal(srv.result(good2), ref[3])
+    with pw.Server(max_active=2, hbm_budget=1 << 20) as srv:  # 1 MiB: no shot fits
+        srv.set_medium(volume)
+        i = submit(srv, shots[0])
+        with pytest.raises(pw.PorowaveError) as ei:
+            srv.wait(i, timeout=60)
+        assert ei.value.status == 3  # PW_ERR_OUT_OF_MEMORY

@@ -77,3 +77,39 @@ def test_create_without_gpu_reports_error():
         pytest.skip("GPU present")
     with pytest.raises(pw.PorowaveError):
         pw.Solver(20, 20, 20, 4, 0.01, 0.9e-6, 10)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirror of every struct in include/porowave.h has the C layout
+    (size and field offsets, as gcc lays the header out)."""
+    import ctypes as C
+    import subprocess
+    structs = {"pw_config": pw.Config, "pw_info": pw.Info, "pw_rock": pw.Rock, "pw_layered": pw.Layered,
+               "pw_server_config": pw.ServerConfig, "pw_shot": pw.ShotSpec, "pw_shot_status": pw.ShotStatus,
+               "pw_server_stats": pw.ServerStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "porowave.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            a, b, v = ln.split()
+            got[(a, b)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_server_needs_a_device():
+    """Without a GPU the server refuses to start (no CPU fallback)."""
+    with pytest.raises(pw.PorowaveError):
+        pw.Server()
