@@ -49,6 +49,20 @@ def receivers(n, np_):
     return fields, idx
 
 
+def measured_traffic(precision, kernel, n):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/hbm_traffic_512.md), or None."""
+    import re
+    path = os.path.join(HERE, "profiles", "hbm_traffic_512.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"traffic (\S+) (\S+) (\d+) (\d+)\s*$", line)
+        if m and m.group(1) == precision and m.group(2) == kernel and int(m.group(3)) == n:
+            return float(m.group(4))
+    return None
+
+
 def cpu_baseline(n, np_, spec_base, dt, h, steps):
     """The oracle (OpenMP C restatement of Float/main.cu) on the same grid and medium,
     slab-CPML mode, timed on a bounded number of steps."""
@@ -278,7 +292,8 @@ def main():
                    "dx_m": h, "dt_s": dt, "f0_hz": 25.0, "receivers": len(idx), "precision": args.precision,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.precision, kname, n),
                      "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 4)},
         "step_hbm_gbs": round(step_bytes * args.steps / sec / 1e9, 1),
         "bytes_per_cell_step": round(info["bytes_per_cell_step"], 2),
