@@ -294,6 +294,13 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     int zc = c.nz >= 256 ? 16 : 4;
     env_int("PW_ZC", &zc);
     s->b.zc = std::max(1, zc);
+    int ty = 4;
+    env_int("PW_TY", &ty);
+    s->b.ty = (ty == 1 || ty == 2 || ty == 4 || ty == 8) ? ty : 4;
+    // cells per lane along x: wider row segments per wave
+    int vx = 1;
+    env_int("PW_VX", &vx);
+    s->b.vx = (vx == 1 || vx == 2 || vx == 4) ? vx : 1;
     int xcd = 1;
     env_int("PW_XCD_MAP", &xcd);
     s->b.xcd_map = xcd ? 1 : 0;

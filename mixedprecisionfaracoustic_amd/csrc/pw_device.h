@@ -10,6 +10,16 @@
 
 namespace pw {
 
+// v as a rounded FP32 value: its bits pass through an integer XOR with z, which
+// is always 0 but arrives at run time (a kernel argument), so the compiler
+// cannot see through it.  Without it the compiler may fold a following
+// FP32 -> FP16 conversion into the arithmetic that produced v (v_fma_mixlo_f16:
+// one rounding of the exact result) where the reference rounds to float first
+// and then to half (__float2half of a float).  Whether it folds depends on the
+// surrounding code, so identical formulas could otherwise round differently in
+// different kernels.
+__device__ __forceinline__ float round32(float v, unsigned z) { return __uint_as_float(__float_as_uint(v) ^ z); }
+
 // Coefficient slots.  "physical" form (FP32 / FP64 storage): the nine
 // cell-centred medium fields on the padded grid (the reference's *_ext arrays
 // after extmodel); the kernels form the staggered averages and the
@@ -43,6 +53,8 @@ struct DevBuffers {
     void* tab[3];    // per axis: n entries of {c_full, e_full, c_half, e_half} (float or double)
     double dt, h;
     int zc;          // z cells per block in the step kernels
+    int ty;          // rows per tile in the step kernels (block 64 x ty)
+    int vx;          // cells per lane along x in the step kernels (1, 2, 4)
     int xcd_map;     // 1: XCD-aware block -> tile mapping in the step kernels
     int slab;        // >0: interleave velocity/stress launches over z-slabs of this many planes
     int fused;       // 1: one fused launch per step over double-buffered state
@@ -96,6 +108,8 @@ struct BatchGroup {
     int* d_nt = nullptr;         // [n] steps of each shot
     unsigned nblocks = 0, nrec = 0;
     int xcd_map = 1;
+    int ty = 4;
+    int vx = 1;
     size_t bytes = 0;
 };
 // (Re)builds g for the given shots (all of precision prec) at their current steps.

@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "pw_device.h"
@@ -67,8 +68,9 @@ __device__ __forceinline__ float c16_decode(f16 hi, int8_t lo)
 {
     return (float)hi + (float)lo * c16_unit_bits(__builtin_bit_cast(unsigned short, hi));
 }
-__device__ __forceinline__ void c16_encode(float v, f16& hi, int8_t& lo)
+__device__ __forceinline__ void c16_encode(float v_in, f16& hi, int8_t& lo, unsigned z)
 {
+    const float v = round32(v_in, z);
     const f16 h = (f16)v;
     const float hf = (float)h;
     // exact: the unit is a power of two
@@ -81,8 +83,55 @@ __device__ __forceinline__ void c16_encode(float v, f16& hi, int8_t& lo)
 
 // --------------------------------------------------------------------------
 // Storage policies
+//   ld / st        one value
+//   ldv / stv      N consecutive values (offset a multiple of N) as one wide access
+//   ldc / ldcv     coefficients;  ldm / stm  CPML memory variables
+//   rt             a value as it reads back after a store
 // --------------------------------------------------------------------------
 template <class R> struct Tab4 { R cf, ef, ch, eh; };
+
+template <class F, class = void> struct requires_zero : std::false_type {};
+template <class F> struct requires_zero<F, std::void_t<decltype(F::z)>> : std::true_type {};
+
+// N consecutive elements, aligned to their total size: one global load/store
+template <class T, int N> struct alignas(sizeof(T) * N) Pack { T v[N]; };
+template <int N, class T> __device__ __forceinline__ Pack<T, N> ld_pack(const T* p, unsigned o)
+{
+    return *(const Pack<T, N>*)(p + o);
+}
+template <int N, class T> __device__ __forceinline__ void st_pack(T* p, unsigned o, const Pack<T, N>& v)
+{
+    *(Pack<T, N>*)(p + o) = v;
+}
+
+// the run-time zero of a field (round32); plain FP32/FP64 fields carry none
+template <class F> __device__ __forceinline__ unsigned field_zero(const F& f)
+{
+    if constexpr (requires_zero<F>::value) return f.z;
+    else return 0u;
+}
+// R -> storage type T; the BF16 conversion starts from a rounded FP32 value.
+template <class T, class R> __device__ __forceinline__ T narrow(R v, unsigned) { return (T)v; }
+template <> __device__ __forceinline__ bf16 narrow<bf16, float>(float v, unsigned z) { return (bf16)round32(v, z); }
+
+// Wide access helpers shared by the plain policies (storage type = T).
+#define PW_PLAIN_VEC(T)                                                                           \
+    template <int N> static __device__ __forceinline__ void ldv(const Field& f, unsigned o, R (&r)[N]) \
+    {                                                                                             \
+        const Pack<T, N> t = ld_pack<N>(f.p, o);                                                  \
+        _Pragma("unroll") for (int j = 0; j < N; ++j) r[j] = (R)t.v[j];                           \
+    }                                                                                             \
+    template <int N> static __device__ __forceinline__ void stv(const Field& f, unsigned o, const R (&r)[N]) \
+    {                                                                                             \
+        Pack<T, N> t;                                                                             \
+        _Pragma("unroll") for (int j = 0; j < N; ++j) t.v[j] = narrow<T>(r[j], field_zero(f));     \
+        st_pack<N>(f.p, o, t);                                                                    \
+    }                                                                                             \
+    template <int N> static __device__ __forceinline__ void ldcv(const Coef* c, unsigned o, R (&r)[N]) \
+    {                                                                                             \
+        const Pack<Coef, N> t = ld_pack<N>(c, o);                                                 \
+        _Pragma("unroll") for (int j = 0; j < N; ++j) r[j] = (R)t.v[j];                           \
+    }
 
 struct PolF32 {
     using R = float; using Coef = float; using Mem = float;
@@ -90,10 +139,11 @@ struct PolF32 {
     struct Field { float* p; };
     static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return f.p[o]; }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v) { f.p[o] = v; }
+    PW_PLAIN_VEC(float)
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
-    static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v) { m[o] = v; }
-    static __device__ __forceinline__ R rt(R v) { return v; }  // value as stored
+    static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned) { m[o] = v; }
+    static __device__ __forceinline__ R rt(R v, unsigned) { return v; }  // value as stored
 };
 struct PolF64 {
     using R = double; using Coef = double; using Mem = double;
@@ -101,45 +151,69 @@ struct PolF64 {
     struct Field { double* p; };
     static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return f.p[o]; }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v) { f.p[o] = v; }
+    PW_PLAIN_VEC(double)
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
-    static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v) { m[o] = v; }
-    static __device__ __forceinline__ R rt(R v) { return v; }
+    static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned) { m[o] = v; }
+    static __device__ __forceinline__ R rt(R v, unsigned) { return v; }
 };
 struct PolC16 {
     using R = float; using Coef = f16; using Mem = f16;
     static constexpr bool scaled = true;
-    struct Field { f16* hi; int8_t* lo; };
+    struct Field { f16* hi; int8_t* lo; unsigned z; };
     static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return c16_decode(f.hi[o], f.lo[o]); }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v)
     {
         f16 h;
         int8_t l;
-        c16_encode(v, h, l);
+        c16_encode(v, h, l, f.z);
         f.hi[o] = h;
         f.lo[o] = l;
     }
+    template <int N> static __device__ __forceinline__ void ldv(const Field& f, unsigned o, R (&r)[N])
+    {
+        const Pack<f16, N> h = ld_pack<N>(f.hi, o);
+        const Pack<int8_t, N> l = ld_pack<N>(f.lo, o);
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = c16_decode(h.v[j], l.v[j]);
+    }
+    template <int N> static __device__ __forceinline__ void stv(const Field& f, unsigned o, const R (&r)[N])
+    {
+        Pack<f16, N> h;
+        Pack<int8_t, N> l;
+#pragma unroll
+        for (int j = 0; j < N; ++j) c16_encode(r[j], h.v[j], l.v[j], f.z);
+        st_pack<N>(f.hi, o, h);
+        st_pack<N>(f.lo, o, l);
+    }
+    template <int N> static __device__ __forceinline__ void ldcv(const Coef* c, unsigned o, R (&r)[N])
+    {
+        const Pack<Coef, N> t = ld_pack<N>(c, o);
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = (R)t.v[j];
+    }
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return (float)c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return (float)m[o]; }
-    static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v) { m[o] = (f16)v; }
-    static __device__ __forceinline__ R rt(R v)
+    static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned z) { m[o] = (f16)round32(v, z); }
+    static __device__ __forceinline__ R rt(R v, unsigned z)
     {
         f16 h;
         int8_t l;
-        c16_encode(v, h, l);
+        c16_encode(v, h, l, z);
         return c16_decode(h, l);
     }
 };
 struct PolB16 {
     using R = float; using Coef = f16; using Mem = bf16;
     static constexpr bool scaled = true;
-    struct Field { bf16* p; };
+    struct Field { bf16* p; unsigned z; };
     static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return (float)f.p[o]; }
-    static __device__ __forceinline__ void st(const Field& f, unsigned o, R v) { f.p[o] = (bf16)v; }
+    static __device__ __forceinline__ void st(const Field& f, unsigned o, R v) { f.p[o] = (bf16)round32(v, f.z); }
+    PW_PLAIN_VEC(bf16)
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return (float)c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return (float)m[o]; }
-    static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v) { m[o] = (bf16)v; }
-    static __device__ __forceinline__ R rt(R v) { return (float)(bf16)v; }
+    static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned z) { m[o] = (bf16)round32(v, z); }
+    static __device__ __forceinline__ R rt(R v, unsigned z) { return (float)(bf16)round32(v, z); }
 };
 
 template <class P> struct StepArgs {
@@ -152,6 +226,9 @@ template <class P> struct StepArgs {
     int lo[3], hi[3];
     int zc;
     int gx, gy, nblocks;   // tile grid (x tiles, y tiles) and total blocks of the 1-D launch
+    int ty;                // rows per tile (block = 64 x ty threads)
+    int vx;                // cells per lane along x (tile = 64*vx x ty cells)
+    unsigned zero;         // 0, opaque to the compiler (round32)
     int zb, ze;            // planes [zb, ze) updated by this launch
     int xcd_map;           // 1: XCD-aware block -> tile mapping
     typename P::R dt, h;
@@ -160,8 +237,8 @@ template <class P> struct StepArgs {
 template <class P> __host__ typename P::Field make_field(const DevBuffers& b, int k);
 template <> __host__ PolF32::Field make_field<PolF32>(const DevBuffers& b, int k) { return {(float*)b.f[k]}; }
 template <> __host__ PolF64::Field make_field<PolF64>(const DevBuffers& b, int k) { return {(double*)b.f[k]}; }
-template <> __host__ PolC16::Field make_field<PolC16>(const DevBuffers& b, int k) { return {(f16*)b.f[k], b.lo[k]}; }
-template <> __host__ PolB16::Field make_field<PolB16>(const DevBuffers& b, int k) { return {(bf16*)b.f[k]}; }
+template <> __host__ PolC16::Field make_field<PolC16>(const DevBuffers& b, int k) { return {(f16*)b.f[k], b.lo[k], 0u}; }
+template <> __host__ PolB16::Field make_field<PolB16>(const DevBuffers& b, int k) { return {(bf16*)b.f[k], 0u}; }
 
 template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
 {
@@ -179,7 +256,11 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     a.plane = (unsigned)b.g.plane;
     a.zc = b.zc;
     a.gx = (b.g.nx + 63) / 64;
-    a.gy = (b.g.ny + 3) / 4;
+    a.ty = b.ty > 0 ? b.ty : 4;
+    a.vx = b.vx > 0 ? b.vx : 1;
+    a.zero = 0u;
+    a.gx = (b.g.nx + 64 * a.vx - 1) / (64 * a.vx);
+    a.gy = (b.g.ny + a.ty - 1) / a.ty;
     a.zb = 1;
     a.ze = b.g.nz - 1;
     a.nblocks = a.gx * a.gy * ((b.g.nz - 2 + b.zc - 1) / b.zc);
@@ -227,11 +308,11 @@ __device__ __forceinline__ Tile tile_of_block(int gx, int gy, int nblocks, int x
 // One CPML term in single-variable form (see header comment).
 template <class P>
 __device__ __forceinline__ typename P::R cpml(typename P::Mem* phi, unsigned k, typename P::R c, typename P::R e,
-                                              typename P::R der)
+                                              typename P::R der, unsigned z)
 {
     using R = typename P::R;
     const R psi = P::ldm(phi, k) - c * der;
-    P::stm(phi, k, e * (psi - c * der));
+    P::stm(phi, k, e * (psi - c * der), z);
     return der + psi;
 }
 
@@ -277,253 +358,448 @@ template <class P> struct Deriv {
 };
 
 // --------------------------------------------------------------------------
+// Row segments.  Each lane owns N consecutive cells of one x row (N = 1, 2, 4),
+// loaded and stored as one wide access, so a wave moves 64*N cells of a row.
+// The x-1 / x+1 neighbours of a lane's cells are its own registers except at
+// the segment ends, which come from the adjacent lane (a cross-lane shuffle);
+// only lanes 0 and 63 read the neighbouring tile's cells from memory.
+// --------------------------------------------------------------------------
+struct Lanes {
+    int lane, ix0;
+    bool lv;                 // the lane's cells lie inside the padded row
+    bool edge_r, edge_l;     // lane 63 / lane 0 with the neighbouring cell inside the row
+    unsigned sh;             // lanes past the row end read the row's last N cells instead
+};
+
+// r[j] = value at x+1 of element j of c.  One cell per lane: a plain (coalesced,
+// cache-resident) load at x+1.  N cells: registers, the adjacent lane, and for
+// lane 63 one load from the next tile.
+template <class R, int N, class EdgeLoad, class RowLoad>
+__device__ __forceinline__ void seg_right(const Lanes& L, const R (&c)[N], R (&r)[N], EdgeLoad edge, RowLoad row)
+{
+    if constexpr (N == 1) {
+        row(r);
+    } else {
+        R s = __shfl_down(c[0], 1, 64);
+        if (L.lane == 63) s = L.edge_r ? edge() : (R)0;
+#pragma unroll
+        for (int j = 0; j < N - 1; ++j) r[j] = c[j + 1];
+        r[N - 1] = s;
+    }
+}
+// r[j] = value at x-1 of element j
+template <class R, int N, class EdgeLoad, class RowLoad>
+__device__ __forceinline__ void seg_left(const Lanes& L, const R (&c)[N], R (&r)[N], EdgeLoad edge, RowLoad row)
+{
+    if constexpr (N == 1) {
+        row(r);
+    } else {
+        R s = __shfl_up(c[N - 1], 1, 64);
+        if (L.lane == 0) s = L.edge_l ? edge() : (R)0;
+        r[0] = s;
+#pragma unroll
+        for (int j = 1; j < N; ++j) r[j] = c[j - 1];
+    }
+}
+
+// Lane geometry of a row segment starting at cell xb of a row of `pitch` cells.
+template <int N> __device__ __forceinline__ Lanes make_lanes(int xb, int pitch)
+{
+    Lanes L;
+    L.lane = threadIdx.x;
+    L.ix0 = xb + L.lane * N;
+    // one cell per lane: tiles are 64 wide and the pitch a multiple of 64
+    L.lv = N == 1 || L.ix0 < pitch;
+    L.edge_r = L.ix0 + N < pitch;
+    L.edge_l = L.ix0 >= 1;
+    L.sh = L.lv ? 0u : (unsigned)(L.ix0 - (pitch - N));
+    return L;
+}
+
+// --------------------------------------------------------------------------
 // Velocity kernel: Float/main.cu:64-192 (physical), Half/main.cu:156-268 (scaled)
 // --------------------------------------------------------------------------
-template <class P>
+template <class P, int N>
 __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile tl)
 {
     using R = typename P::R;
-    const int ix = tl.bx * 64 + threadIdx.x;
-    const int iy = tl.by * 4 + threadIdx.y;
-    if (ix < 1 || ix > a.nx - 2 || iy < 1 || iy > a.ny - 2) return;
+    const int xb = tl.bx * 64 * N;
+    const int iy = tl.by * a.ty + threadIdx.y;
+    if (iy < 1 || iy > a.ny - 2 || xb > a.nx - 2) return;  // the whole wave (one row)
     const int z0 = a.zb + tl.bz * a.zc;
     const int z1 = min(z0 + a.zc, a.ze);
     if (z0 >= z1) return;
+    const Lanes L = make_lanes<N>(xb, a.pitch);
+    const int ix0 = L.ix0;
+    bool upd[N], all = true;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        upd[j] = L.lv && ix0 + j >= 1 && ix0 + j <= a.nx - 2;
+        all = all && upd[j];
+    }
     const unsigned PL = a.plane, W = (unsigned)a.pitch;
-    unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z0 * PL;
+    unsigned o = (unsigned)ix0 + (unsigned)iy * W + (unsigned)z0 * PL;
     const Deriv<P> D(a.h);
     const R dt = a.dt;
 
-    const int xs = slab_index(ix, a.nx, a.lo[0], a.hi[0]);
+    // lanes past the row end (wide segments only) read the row's last cells; their
+    // values are never used
+    auto LD = [&](int fid, unsigned off, R (&r)[N]) { P::template ldv<N>(a.f[fid], off - L.sh, r); };
+    auto LDC = [&](int cid, unsigned off, R (&r)[N]) { P::template ldcv<N>(a.c[cid], off - L.sh, r); };
+    auto ST = [&](int fid, unsigned off, const R (&r)[N]) {
+        if (all) P::template stv<N>(a.f[fid], off, r);
+        else {
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                if (upd[j]) P::st(a.f[fid], off + j, r[j]);
+        }
+    };
+
     const int ys = slab_index(iy, a.ny, a.lo[1], a.hi[1]);
     const int XW = a.lo[0] + a.hi[0], YW = a.lo[1] + a.hi[1];
-    Tab4<R> tx = {0, 1, 0, 1}, ty = {0, 1, 0, 1};
-    if (xs >= 0) tx = a.tab[0][ix];
+    Tab4<R> ty = {0, 1, 0, 1};
     if (ys >= 0) ty = a.tab[1][iy];
+    int xs[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) xs[j] = upd[j] ? slab_index(ix0 + j, a.nx, a.lo[0], a.hi[0]) : -1;
 
-    R txz_m = P::ld(a.f[PW_TXZ], o - PL);
-    R tyz_m = P::ld(a.f[PW_TYZ], o - PL);
-    R tzz_c = P::ld(a.f[PW_TZZ], o);
-    R p_c = P::ld(a.f[PW_P], o);
+    R txz_m[N], tyz_m[N], tzz_c[N], p_c[N];
+    LD(PW_TXZ, o - PL, txz_m);
+    LD(PW_TYZ, o - PL, tyz_m);
+    LD(PW_TZZ, o, tzz_c);
+    LD(PW_P, o, p_c);
     // physical form: cell-centred rho, C1, C2 of this plane, carried up the column
-    R rho_c = 0, c1_c = 0, c2_c = 0;
+    R rho_c[N], c1_c[N], c2_c[N];
     if constexpr (!P::scaled) {
-        rho_c = P::ldc(a.c[PC_RHO], o);
-        c1_c = P::ldc(a.c[PC_C1], o);
-        c2_c = P::ldc(a.c[PC_C2], o);
+        LDC(PC_RHO, o, rho_c);
+        LDC(PC_C1, o, c1_c);
+        LDC(PC_C2, o, c2_c);
     }
 
     for (int iz = z0; iz < z1; ++iz, o += PL) {
-        const R txz_c = P::ld(a.f[PW_TXZ], o);
-        const R tyz_c = P::ld(a.f[PW_TYZ], o);
-        const R tzz_n = P::ld(a.f[PW_TZZ], o + PL);
-        const R p_n = P::ld(a.f[PW_P], o + PL);
-        const R txx_c = P::ld(a.f[PW_TXX], o), txx_r = P::ld(a.f[PW_TXX], o + 1);
-        const R tyy_c = P::ld(a.f[PW_TYY], o), tyy_q = P::ld(a.f[PW_TYY], o + W);
-        const R txy_c = P::ld(a.f[PW_TXY], o), txy_l = P::ld(a.f[PW_TXY], o - 1), txy_h = P::ld(a.f[PW_TXY], o - W);
-        const R txz_l = P::ld(a.f[PW_TXZ], o - 1);
-        const R tyz_h = P::ld(a.f[PW_TYZ], o - W);
-        const R p_r = P::ld(a.f[PW_P], o + 1), p_q = P::ld(a.f[PW_P], o + W);
-
-        // Float/main.cu:115-128
-        R x1 = D(txx_r - txx_c), x2 = D(txy_c - txy_h), x3 = D(txz_c - txz_m), s1 = D(p_r - p_c);
-        R y1 = D(tyy_q - tyy_c), y2 = D(txy_c - txy_l), y3 = D(tyz_c - tyz_m), s2 = D(p_q - p_c);
-        R z1 = D(tzz_n - tzz_c), z2 = D(txz_c - txz_l), z3 = D(tyz_c - tyz_h), s3 = D(p_n - p_c);
-
-        if (xs >= 0) {
-            const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs;
-            x1 = cpml<P>(a.m[0][0], k, tx.ch, tx.eh, x1);
-            s1 = cpml<P>(a.m[0][1], k, tx.ch, tx.eh, s1);
-            y2 = cpml<P>(a.m[0][2], k, tx.cf, tx.ef, y2);
-            z2 = cpml<P>(a.m[0][3], k, tx.cf, tx.ef, z2);
-        }
-        if (ys >= 0) {
-            const unsigned k = ((unsigned)iz * YW + ys) * W + ix;
-            y1 = cpml<P>(a.m[1][0], k, ty.ch, ty.eh, y1);
-            s2 = cpml<P>(a.m[1][1], k, ty.ch, ty.eh, s2);
-            x2 = cpml<P>(a.m[1][2], k, ty.cf, ty.ef, x2);
-            z3 = cpml<P>(a.m[1][3], k, ty.cf, ty.ef, z3);
-        }
+        R txz_c[N], tyz_c[N], tzz_n[N], p_n[N], txx_c[N], txx_r[N], tyy_c[N], tyy_q[N];
+        R txy_c[N], txy_l[N], txy_h[N], txz_l[N], tyz_h[N], p_r[N], p_q[N];
+        LD(PW_TXZ, o, txz_c);
+        LD(PW_TYZ, o, tyz_c);
+        LD(PW_TZZ, o + PL, tzz_n);
+        LD(PW_P, o + PL, p_n);
+        LD(PW_TXX, o, txx_c);
+        seg_right(L, txx_c, txx_r, [&] { return P::ld(a.f[PW_TXX], o + N); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
+        LD(PW_TYY, o, tyy_c);
+        LD(PW_TYY, o + W, tyy_q);
+        LD(PW_TXY, o, txy_c);
+        seg_left(L, txy_c, txy_l, [&] { return P::ld(a.f[PW_TXY], o - 1); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
+        LD(PW_TXY, o - W, txy_h);
+        seg_left(L, txz_c, txz_l, [&] { return P::ld(a.f[PW_TXZ], o - 1); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
+        LD(PW_TYZ, o - W, tyz_h);
+        seg_right(L, p_c, p_r, [&] { return P::ld(a.f[PW_P], o + N); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
+        LD(PW_P, o + W, p_q);
         const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
-        if (zs >= 0) {
-            const Tab4<R> tz = a.tab[2][iz];
-            const unsigned k = ((unsigned)zs * a.ny + iy) * W + ix;
-            z1 = cpml<P>(a.m[2][0], k, tz.ch, tz.eh, z1);
-            s3 = cpml<P>(a.m[2][1], k, tz.ch, tz.eh, s3);
-            x3 = cpml<P>(a.m[2][2], k, tz.cf, tz.ef, x3);
-            y3 = cpml<P>(a.m[2][3], k, tz.cf, tz.ef, y3);
+        Tab4<R> tz = {0, 1, 0, 1};
+        if (zs >= 0) tz = a.tab[2][iz];
+
+        R sx[N], sy[N], sz[N], s1[N], s2[N], s3[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            // Float/main.cu:115-128
+            R x1 = D(txx_r[j] - txx_c[j]), x2 = D(txy_c[j] - txy_h[j]), x3 = D(txz_c[j] - txz_m[j]);
+            R y1 = D(tyy_q[j] - tyy_c[j]), y2 = D(txy_c[j] - txy_l[j]), y3 = D(tyz_c[j] - tyz_m[j]);
+            R z1_ = D(tzz_n[j] - tzz_c[j]), z2 = D(txz_c[j] - txz_l[j]), z3 = D(tyz_c[j] - tyz_h[j]);
+            R q1 = D(p_r[j] - p_c[j]), q2 = D(p_q[j] - p_c[j]), q3 = D(p_n[j] - p_c[j]);
+            const unsigned ix = (unsigned)(ix0 + j);
+            if (xs[j] >= 0) {
+                const Tab4<R> tx = a.tab[0][ix];
+                const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs[j];
+                x1 = cpml<P>(a.m[0][0], k, tx.ch, tx.eh, x1, a.zero);
+                q1 = cpml<P>(a.m[0][1], k, tx.ch, tx.eh, q1, a.zero);
+                y2 = cpml<P>(a.m[0][2], k, tx.cf, tx.ef, y2, a.zero);
+                z2 = cpml<P>(a.m[0][3], k, tx.cf, tx.ef, z2, a.zero);
+            }
+            if (ys >= 0 && upd[j]) {
+                const unsigned k = ((unsigned)iz * YW + ys) * W + ix;
+                y1 = cpml<P>(a.m[1][0], k, ty.ch, ty.eh, y1, a.zero);
+                q2 = cpml<P>(a.m[1][1], k, ty.ch, ty.eh, q2, a.zero);
+                x2 = cpml<P>(a.m[1][2], k, ty.cf, ty.ef, x2, a.zero);
+                z3 = cpml<P>(a.m[1][3], k, ty.cf, ty.ef, z3, a.zero);
+            }
+            if (zs >= 0 && upd[j]) {
+                const unsigned k = ((unsigned)zs * a.ny + iy) * W + ix;
+                z1_ = cpml<P>(a.m[2][0], k, tz.ch, tz.eh, z1_, a.zero);
+                q3 = cpml<P>(a.m[2][1], k, tz.ch, tz.eh, q3, a.zero);
+                x3 = cpml<P>(a.m[2][2], k, tz.cf, tz.ef, x3, a.zero);
+                y3 = cpml<P>(a.m[2][3], k, tz.cf, tz.ef, y3, a.zero);
+            }
+            sx[j] = x1 + x2 + x3;
+            sy[j] = y1 + y2 + y3;
+            sz[j] = z1_ + z2 + z3;
+            s1[j] = q1;
+            s2[j] = q2;
+            s3[j] = q3;
         }
 
-        const R sx = x1 + x2 + x3, sy = y1 + y2 + y3, sz = z1 + z2 + z3;
-        const R wx0 = P::ld(a.f[PW_VWX], o), wy0 = P::ld(a.f[PW_VWY], o), wz0 = P::ld(a.f[PW_VWZ], o);
-        R wx1, wy1, wz1, ux1, uy1, uz1;
+        R wx0[N], wy0[N], wz0[N], ux0[N], uy0[N], uz0[N];
+        LD(PW_VWX, o, wx0);
+        LD(PW_VWY, o, wy0);
+        LD(PW_VWZ, o, wz0);
+        LD(PW_VUX, o, ux0);
+        LD(PW_VUY, o, uy0);
+        LD(PW_VUZ, o, uz0);
+        R wx1[N], wy1[N], wz1[N], ux1[N], uy1[N], uz1[N];
         if constexpr (!P::scaled) {
-            const R rho_n = P::ldc(a.c[PC_RHO], o + PL), rho_x = P::ldc(a.c[PC_RHO], o + 1),
-                    rho_y = P::ldc(a.c[PC_RHO], o + W);
-            const R c1_n = P::ldc(a.c[PC_C1], o + PL), c1_x = P::ldc(a.c[PC_C1], o + 1),
-                    c1_y = P::ldc(a.c[PC_C1], o + W);
-            const R c2_n = P::ldc(a.c[PC_C2], o + PL), c2_x = P::ldc(a.c[PC_C2], o + 1),
-                    c2_y = P::ldc(a.c[PC_C2], o + W);
-            const R rf = P::ldc(a.c[PC_RHOF], o);
-            const R rix = inv_mean(rho_c, rho_x), riy = inv_mean(rho_c, rho_y), riz = inv_mean(rho_c, rho_n);
-            const R c1x = half_sum(c1_c, c1_x), c1y = half_sum(c1_c, c1_y), c1z = half_sum(c1_c, c1_n);
-            const R c2x = half_sum(c2_c, c2_x), c2y = half_sum(c2_c, c2_y), c2z = half_sum(c2_c, c2_n);
-            R cc1x, cc2x, cc1y, cc2y, cc1z, cc2z;
-            cn_pair(c1x, c2x, rf, rix, dt, cc1x, cc2x);
-            cn_pair(c1y, c2y, rf, riy, dt, cc1y, cc2y);
-            cn_pair(c1z, c2z, rf, riz, dt, cc1z, cc2z);
-            // Float/main.cu:160-189
-            wx1 = c1x == (R)0 ? (R)0 : (wx0 * cc2x - dt * (rf * rix * sx + s1)) / cc1x;
-            wy1 = c1y == (R)0 ? (R)0 : (wy0 * cc2y - dt * (rf * riy * sy + s2)) / cc1y;
-            wz1 = c1z == (R)0 ? (R)0 : (wz0 * cc2z - dt * (rf * riz * sz + s3)) / cc1z;
-            ux1 = P::ld(a.f[PW_VUX], o) + dt * rix * sx - rf * rix * (wx1 - wx0);
-            uy1 = P::ld(a.f[PW_VUY], o) + dt * riy * sy - rf * riy * (wy1 - wy0);
-            uz1 = P::ld(a.f[PW_VUZ], o) + dt * riz * sz - rf * riz * (wz1 - wz0);
-            rho_c = rho_n; c1_c = c1_n; c2_c = c2_n;
+            R rho_n[N], rho_x[N], rho_y[N], c1_n[N], c1_x[N], c1_y[N], c2_n[N], c2_x[N], c2_y[N], rf[N];
+            LDC(PC_RHO, o + PL, rho_n);
+            seg_right(L, rho_c, rho_x, [&] { return P::ldc(a.c[PC_RHO], o + N); }, [&](R (&r)[N]) { LDC(PC_RHO, o + 1, r); });
+            LDC(PC_RHO, o + W, rho_y);
+            LDC(PC_C1, o + PL, c1_n);
+            seg_right(L, c1_c, c1_x, [&] { return P::ldc(a.c[PC_C1], o + N); }, [&](R (&r)[N]) { LDC(PC_C1, o + 1, r); });
+            LDC(PC_C1, o + W, c1_y);
+            LDC(PC_C2, o + PL, c2_n);
+            seg_right(L, c2_c, c2_x, [&] { return P::ldc(a.c[PC_C2], o + N); }, [&](R (&r)[N]) { LDC(PC_C2, o + 1, r); });
+            LDC(PC_C2, o + W, c2_y);
+            LDC(PC_RHOF, o, rf);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const R rix = inv_mean(rho_c[j], rho_x[j]), riy = inv_mean(rho_c[j], rho_y[j]),
+                        riz = inv_mean(rho_c[j], rho_n[j]);
+                const R c1x = half_sum(c1_c[j], c1_x[j]), c1y = half_sum(c1_c[j], c1_y[j]),
+                        c1z = half_sum(c1_c[j], c1_n[j]);
+                const R c2x = half_sum(c2_c[j], c2_x[j]), c2y = half_sum(c2_c[j], c2_y[j]),
+                        c2z = half_sum(c2_c[j], c2_n[j]);
+                R cc1x, cc2x, cc1y, cc2y, cc1z, cc2z;
+                cn_pair(c1x, c2x, rf[j], rix, dt, cc1x, cc2x);
+                cn_pair(c1y, c2y, rf[j], riy, dt, cc1y, cc2y);
+                cn_pair(c1z, c2z, rf[j], riz, dt, cc1z, cc2z);
+                // Float/main.cu:160-189
+                wx1[j] = c1x == (R)0 ? (R)0 : (wx0[j] * cc2x - dt * (rf[j] * rix * sx[j] + s1[j])) / cc1x;
+                wy1[j] = c1y == (R)0 ? (R)0 : (wy0[j] * cc2y - dt * (rf[j] * riy * sy[j] + s2[j])) / cc1y;
+                wz1[j] = c1z == (R)0 ? (R)0 : (wz0[j] * cc2z - dt * (rf[j] * riz * sz[j] + s3[j])) / cc1z;
+                ux1[j] = ux0[j] + dt * rix * sx[j] - rf[j] * rix * (wx1[j] - wx0[j]);
+                uy1[j] = uy0[j] + dt * riy * sy[j] - rf[j] * riy * (wy1[j] - wy0[j]);
+                uz1[j] = uz0[j] + dt * riz * sz[j] - rf[j] * riz * (wz1[j] - wz0[j]);
+                rho_c[j] = rho_n[j];
+                c1_c[j] = c1_n[j];
+                c2_c[j] = c2_n[j];
+            }
         } else {
             // Half/main.cu:243-262: vw = Aw*vw - Bt*S - Bp*dp (0 where Aw = Bp = Bt = 0);
             // vu += rho~^-1*S - rhof*rho^-1*(vw_new - vw_old)
-            const R awx = P::ldc(a.c[SC_AWX], o), bpx = P::ldc(a.c[SC_BPX], o), btx = P::ldc(a.c[SC_BTX], o);
-            const R awy = P::ldc(a.c[SC_AWY], o), bpy = P::ldc(a.c[SC_BPY], o), bty = P::ldc(a.c[SC_BTY], o);
-            const R awz = P::ldc(a.c[SC_AWZ], o), bpz = P::ldc(a.c[SC_BPZ], o), btz = P::ldc(a.c[SC_BTZ], o);
-            wx1 = (awx == 0.0f && bpx == 0.0f && btx == 0.0f) ? 0.0f : awx * wx0 - btx * sx - bpx * s1;
-            wy1 = (awy == 0.0f && bpy == 0.0f && bty == 0.0f) ? 0.0f : awy * wy0 - bty * sy - bpy * s2;
-            wz1 = (awz == 0.0f && bpz == 0.0f && btz == 0.0f) ? 0.0f : awz * wz0 - btz * sz - bpz * s3;
-            ux1 = P::ld(a.f[PW_VUX], o) + P::ldc(a.c[SC_RITX], o) * sx - P::ldc(a.c[SC_FRX], o) * (wx1 - wx0);
-            uy1 = P::ld(a.f[PW_VUY], o) + P::ldc(a.c[SC_RITY], o) * sy - P::ldc(a.c[SC_FRY], o) * (wy1 - wy0);
-            uz1 = P::ld(a.f[PW_VUZ], o) + P::ldc(a.c[SC_RITZ], o) * sz - P::ldc(a.c[SC_FRZ], o) * (wz1 - wz0);
+            R aw[N], bp[N], bt[N], ri[N], fr[N];
+            LDC(SC_AWX, o, aw); LDC(SC_BPX, o, bp); LDC(SC_BTX, o, bt); LDC(SC_RITX, o, ri); LDC(SC_FRX, o, fr);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                wx1[j] = (aw[j] == 0.0f && bp[j] == 0.0f && bt[j] == 0.0f) ? 0.0f : aw[j] * wx0[j] - bt[j] * sx[j] - bp[j] * s1[j];
+                ux1[j] = ux0[j] + ri[j] * sx[j] - fr[j] * (wx1[j] - wx0[j]);
+            }
+            LDC(SC_AWY, o, aw); LDC(SC_BPY, o, bp); LDC(SC_BTY, o, bt); LDC(SC_RITY, o, ri); LDC(SC_FRY, o, fr);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                wy1[j] = (aw[j] == 0.0f && bp[j] == 0.0f && bt[j] == 0.0f) ? 0.0f : aw[j] * wy0[j] - bt[j] * sy[j] - bp[j] * s2[j];
+                uy1[j] = uy0[j] + ri[j] * sy[j] - fr[j] * (wy1[j] - wy0[j]);
+            }
+            LDC(SC_AWZ, o, aw); LDC(SC_BPZ, o, bp); LDC(SC_BTZ, o, bt); LDC(SC_RITZ, o, ri); LDC(SC_FRZ, o, fr);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                wz1[j] = (aw[j] == 0.0f && bp[j] == 0.0f && bt[j] == 0.0f) ? 0.0f : aw[j] * wz0[j] - bt[j] * sz[j] - bp[j] * s3[j];
+                uz1[j] = uz0[j] + ri[j] * sz[j] - fr[j] * (wz1[j] - wz0[j]);
+            }
         }
-        P::st(a.f[PW_VWX], o, wx1); P::st(a.f[PW_VWY], o, wy1); P::st(a.f[PW_VWZ], o, wz1);
-        P::st(a.f[PW_VUX], o, ux1); P::st(a.f[PW_VUY], o, uy1); P::st(a.f[PW_VUZ], o, uz1);
+        ST(PW_VWX, o, wx1); ST(PW_VWY, o, wy1); ST(PW_VWZ, o, wz1);
+        ST(PW_VUX, o, ux1); ST(PW_VUY, o, uy1); ST(PW_VUZ, o, uz1);
 
-        txz_m = txz_c; tyz_m = tyz_c; tzz_c = tzz_n; p_c = p_n;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            txz_m[j] = txz_c[j];
+            tyz_m[j] = tyz_c[j];
+            tzz_c[j] = tzz_n[j];
+            p_c[j] = p_n[j];
+        }
     }
 }
 
 // --------------------------------------------------------------------------
 // Stress kernel: Float/main.cu:194-295 (physical), Half/main.cu:270-371 (scaled)
 // --------------------------------------------------------------------------
-template <class P>
+template <class P, int N>
 __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
 {
     using R = typename P::R;
-    const int ix = tl.bx * 64 + threadIdx.x;
-    const int iy = tl.by * 4 + threadIdx.y;
-    if (ix < 1 || ix > a.nx - 2 || iy < 1 || iy > a.ny - 2) return;
+    const int xb = tl.bx * 64 * N;
+    const int iy = tl.by * a.ty + threadIdx.y;
+    if (iy < 1 || iy > a.ny - 2 || xb > a.nx - 2) return;  // the whole wave (one row)
     const int z0 = a.zb + tl.bz * a.zc;
     const int z1 = min(z0 + a.zc, a.ze);
     if (z0 >= z1) return;
+    const Lanes L = make_lanes<N>(xb, a.pitch);
+    const int ix0 = L.ix0;
+    bool upd[N], all = true;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        upd[j] = L.lv && ix0 + j >= 1 && ix0 + j <= a.nx - 2;
+        all = all && upd[j];
+    }
     const unsigned PL = a.plane, W = (unsigned)a.pitch;
-    unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z0 * PL;
+    unsigned o = (unsigned)ix0 + (unsigned)iy * W + (unsigned)z0 * PL;
     const Deriv<P> D(a.h);
     const R dt = a.dt;
 
-    const int xs = slab_index(ix, a.nx, a.lo[0], a.hi[0]);
+    // lanes past the row end (wide segments only) read the row's last cells; their
+    // values are never used
+    auto LD = [&](int fid, unsigned off, R (&r)[N]) { P::template ldv<N>(a.f[fid], off - L.sh, r); };
+    auto LDC = [&](int cid, unsigned off, R (&r)[N]) { P::template ldcv<N>(a.c[cid], off - L.sh, r); };
+    auto ST = [&](int fid, unsigned off, const R (&r)[N]) {
+        if (all) P::template stv<N>(a.f[fid], off, r);
+        else {
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                if (upd[j]) P::st(a.f[fid], off + j, r[j]);
+        }
+    };
+
     const int ys = slab_index(iy, a.ny, a.lo[1], a.hi[1]);
     const int XW = a.lo[0] + a.hi[0], YW = a.lo[1] + a.hi[1];
-    Tab4<R> tx = {0, 1, 0, 1}, ty = {0, 1, 0, 1};
-    if (xs >= 0) tx = a.tab[0][ix];
+    Tab4<R> ty = {0, 1, 0, 1};
     if (ys >= 0) ty = a.tab[1][iy];
+    int xs[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) xs[j] = upd[j] ? slab_index(ix0 + j, a.nx, a.lo[0], a.hi[0]) : -1;
 
-    R vuz_m = P::ld(a.f[PW_VUZ], o - PL);
-    R vwz_m = P::ld(a.f[PW_VWZ], o - PL);
-    R vux_c = P::ld(a.f[PW_VUX], o);
-    R vuy_c = P::ld(a.f[PW_VUY], o);
+    R vuz_m[N], vwz_m[N], vux_c[N], vuy_c[N];
+    LD(PW_VUZ, o - PL, vuz_m);
+    LD(PW_VWZ, o - PL, vwz_m);
+    LD(PW_VUX, o, vux_c);
+    LD(PW_VUY, o, vuy_c);
     // physical form: cell-centred mu at (x, y), (x+1, y), (x, y+1) of this plane
-    R mu_c = 0, mu_x = 0, mu_y = 0;
+    R mu_c[N], mu_x[N], mu_y[N];
     if constexpr (!P::scaled) {
-        mu_c = P::ldc(a.c[PC_MU], o);
-        mu_x = P::ldc(a.c[PC_MU], o + 1);
-        mu_y = P::ldc(a.c[PC_MU], o + W);
+        LDC(PC_MU, o, mu_c);
+        seg_right(L, mu_c, mu_x, [&] { return P::ldc(a.c[PC_MU], o + N); }, [&](R (&r)[N]) { LDC(PC_MU, o + 1, r); });
+        LDC(PC_MU, o + W, mu_y);
     }
 
     for (int iz = z0; iz < z1; ++iz, o += PL) {
-        const R vux_n = P::ld(a.f[PW_VUX], o + PL), vuy_n = P::ld(a.f[PW_VUY], o + PL);
-        const R vuz_c = P::ld(a.f[PW_VUZ], o), vwz_c = P::ld(a.f[PW_VWZ], o);
-        const R vux_l = P::ld(a.f[PW_VUX], o - 1), vux_q = P::ld(a.f[PW_VUX], o + W);
-        const R vuy_h = P::ld(a.f[PW_VUY], o - W), vuy_r = P::ld(a.f[PW_VUY], o + 1);
-        const R vuz_r = P::ld(a.f[PW_VUZ], o + 1), vuz_q = P::ld(a.f[PW_VUZ], o + W);
-        const R vwx_c = P::ld(a.f[PW_VWX], o), vwx_l = P::ld(a.f[PW_VWX], o - 1);
-        const R vwy_c = P::ld(a.f[PW_VWY], o), vwy_h = P::ld(a.f[PW_VWY], o - W);
-
-        // Float/main.cu:234-249
-        R uxx = D(vux_c - vux_l), uyy = D(vuy_c - vuy_h), uzz = D(vuz_c - vuz_m);
-        R wx = D(vwx_c - vwx_l), wy = D(vwy_c - vwy_h), wz = D(vwz_c - vwz_m);
-        R uxy = D(vux_q - vux_c), uyx = D(vuy_r - vuy_c);
-        R uxz = D(vux_n - vux_c), uzx = D(vuz_r - vuz_c);
-        R uyz = D(vuy_n - vuy_c), uzy = D(vuz_q - vuz_c);
-
-        if (xs >= 0) {
-            const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs;
-            uxx = cpml<P>(a.m[0][4], k, tx.cf, tx.ef, uxx);
-            wx = cpml<P>(a.m[0][5], k, tx.cf, tx.ef, wx);
-            uyx = cpml<P>(a.m[0][6], k, tx.ch, tx.eh, uyx);
-            uzx = cpml<P>(a.m[0][7], k, tx.ch, tx.eh, uzx);
-        }
-        if (ys >= 0) {
-            const unsigned k = ((unsigned)iz * YW + ys) * W + ix;
-            uyy = cpml<P>(a.m[1][4], k, ty.cf, ty.ef, uyy);
-            wy = cpml<P>(a.m[1][5], k, ty.cf, ty.ef, wy);
-            uxy = cpml<P>(a.m[1][6], k, ty.ch, ty.eh, uxy);
-            uzy = cpml<P>(a.m[1][7], k, ty.ch, ty.eh, uzy);
-        }
+        R vux_n[N], vuy_n[N], vuz_c[N], vwz_c[N], vux_l[N], vux_q[N], vuy_h[N], vuy_r[N];
+        R vuz_r[N], vuz_q[N], vwx_c[N], vwx_l[N], vwy_c[N], vwy_h[N];
+        LD(PW_VUX, o + PL, vux_n);
+        LD(PW_VUY, o + PL, vuy_n);
+        LD(PW_VUZ, o, vuz_c);
+        LD(PW_VWZ, o, vwz_c);
+        seg_left(L, vux_c, vux_l, [&] { return P::ld(a.f[PW_VUX], o - 1); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
+        LD(PW_VUX, o + W, vux_q);
+        LD(PW_VUY, o - W, vuy_h);
+        seg_right(L, vuy_c, vuy_r, [&] { return P::ld(a.f[PW_VUY], o + N); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
+        seg_right(L, vuz_c, vuz_r, [&] { return P::ld(a.f[PW_VUZ], o + N); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
+        LD(PW_VUZ, o + W, vuz_q);
+        LD(PW_VWX, o, vwx_c);
+        seg_left(L, vwx_c, vwx_l, [&] { return P::ld(a.f[PW_VWX], o - 1); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
+        LD(PW_VWY, o, vwy_c);
+        LD(PW_VWY, o - W, vwy_h);
         const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
-        if (zs >= 0) {
-            const Tab4<R> tz = a.tab[2][iz];
-            const unsigned k = ((unsigned)zs * a.ny + iy) * W + ix;
-            uzz = cpml<P>(a.m[2][4], k, tz.cf, tz.ef, uzz);
-            wz = cpml<P>(a.m[2][5], k, tz.cf, tz.ef, wz);
-            uxz = cpml<P>(a.m[2][6], k, tz.ch, tz.eh, uxz);
-            uyz = cpml<P>(a.m[2][7], k, tz.ch, tz.eh, uyz);
-        }
+        Tab4<R> tz = {0, 1, 0, 1};
+        if (zs >= 0) tz = a.tab[2][iz];
 
-        const R du = uxx + uyy + uzz, dw = wx + wy + wz;
-        R p1, xx1, yy1, zz1, xy1, yz1, xz1;
+        R p0[N], xx0[N], yy0[N], zz0[N], xy0[N], yz0[N], xz0[N];
+        LD(PW_P, o, p0);
+        LD(PW_TXX, o, xx0);
+        LD(PW_TYY, o, yy0);
+        LD(PW_TZZ, o, zz0);
+        LD(PW_TXY, o, xy0);
+        LD(PW_TYZ, o, yz0);
+        LD(PW_TXZ, o, xz0);
+        R p1[N], xx1[N], yy1[N], zz1[N], xy1[N], yz1[N], xz1[N];
+        R mu_n[N], mu_xn[N], mu_yn[N], mu_xy[N];
+        R C_[N], M_[N], HH_[N], H2u_[N], mxy[N], myz[N], mxz[N];
         if constexpr (!P::scaled) {
-            const R mu_n = P::ldc(a.c[PC_MU], o + PL), mu_xn = P::ldc(a.c[PC_MU], o + PL + 1),
-                    mu_yn = P::ldc(a.c[PC_MU], o + PL + W), mu_xy = P::ldc(a.c[PC_MU], o + 1 + W);
-            // Float/main.cu:480-491
-            const R muxz = harm4(mu_c, mu_x, mu_n, mu_xn);
-            const R muyz = harm4(mu_c, mu_y, mu_n, mu_yn);
-            const R muxy = harm4(mu_c, mu_x, mu_y, mu_xy);
-            const R C = P::ldc(a.c[PC_C], o), M = P::ldc(a.c[PC_M], o);
-            const R HH = P::ldc(a.c[PC_HH], o), H2u = P::ldc(a.c[PC_H2U], o);
-            // Float/main.cu:287-293
-            p1 = P::ld(a.f[PW_P], o) - dt * (C * du + M * dw);
-            xx1 = P::ld(a.f[PW_TXX], o) + dt * (H2u * (uyy + uzz) + HH * uxx + C * dw);
-            yy1 = P::ld(a.f[PW_TYY], o) + dt * (H2u * (uxx + uzz) + HH * uyy + C * dw);
-            zz1 = P::ld(a.f[PW_TZZ], o) + dt * (H2u * (uxx + uyy) + HH * uzz + C * dw);
-            xy1 = P::ld(a.f[PW_TXY], o) + muxy * dt * (uxy + uyx);
-            yz1 = P::ld(a.f[PW_TYZ], o) + muyz * dt * (uyz + uzy);
-            xz1 = P::ld(a.f[PW_TXZ], o) + muxz * dt * (uxz + uzx);
-            mu_c = mu_n; mu_x = mu_xn; mu_y = mu_yn;
+            LDC(PC_MU, o + PL, mu_n);
+            seg_right(L, mu_n, mu_xn, [&] { return P::ldc(a.c[PC_MU], o + PL + N); }, [&](R (&r)[N]) { LDC(PC_MU, o + PL + 1, r); });
+            LDC(PC_MU, o + PL + W, mu_yn);
+            seg_right(L, mu_y, mu_xy, [&] { return P::ldc(a.c[PC_MU], o + W + N); }, [&](R (&r)[N]) { LDC(PC_MU, o + W + 1, r); });
+            LDC(PC_C, o, C_); LDC(PC_M, o, M_); LDC(PC_HH, o, HH_); LDC(PC_H2U, o, H2u_);
         } else {
-            // Half/main.cu:362-370
-            const R C = P::ldc(a.c[SC_C], o), M = P::ldc(a.c[SC_M], o);
-            const R HH = P::ldc(a.c[SC_HH], o), H2u = P::ldc(a.c[SC_H2U], o);
-            p1 = P::ld(a.f[PW_P], o) - C * du - M * dw;
-            xx1 = P::ld(a.f[PW_TXX], o) + H2u * (uyy + uzz) + HH * uxx + C * dw;
-            yy1 = P::ld(a.f[PW_TYY], o) + H2u * (uxx + uzz) + HH * uyy + C * dw;
-            zz1 = P::ld(a.f[PW_TZZ], o) + H2u * (uxx + uyy) + HH * uzz + C * dw;
-            xy1 = P::ld(a.f[PW_TXY], o) + P::ldc(a.c[SC_MUXY], o) * (uxy + uyx);
-            yz1 = P::ld(a.f[PW_TYZ], o) + P::ldc(a.c[SC_MUYZ], o) * (uyz + uzy);
-            xz1 = P::ld(a.f[PW_TXZ], o) + P::ldc(a.c[SC_MUXZ], o) * (uxz + uzx);
+            LDC(SC_C, o, C_); LDC(SC_M, o, M_); LDC(SC_HH, o, HH_); LDC(SC_H2U, o, H2u_);
+            LDC(SC_MUXY, o, mxy); LDC(SC_MUYZ, o, myz); LDC(SC_MUXZ, o, mxz);
         }
-        P::st(a.f[PW_P], o, p1);
-        P::st(a.f[PW_TXX], o, xx1); P::st(a.f[PW_TYY], o, yy1); P::st(a.f[PW_TZZ], o, zz1);
-        P::st(a.f[PW_TXY], o, xy1); P::st(a.f[PW_TYZ], o, yz1); P::st(a.f[PW_TXZ], o, xz1);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            // Float/main.cu:234-249
+            R uxx = D(vux_c[j] - vux_l[j]), uyy = D(vuy_c[j] - vuy_h[j]), uzz = D(vuz_c[j] - vuz_m[j]);
+            R wx = D(vwx_c[j] - vwx_l[j]), wy = D(vwy_c[j] - vwy_h[j]), wz = D(vwz_c[j] - vwz_m[j]);
+            R uxy = D(vux_q[j] - vux_c[j]), uyx = D(vuy_r[j] - vuy_c[j]);
+            R uxz = D(vux_n[j] - vux_c[j]), uzx = D(vuz_r[j] - vuz_c[j]);
+            R uyz = D(vuy_n[j] - vuy_c[j]), uzy = D(vuz_q[j] - vuz_c[j]);
+            const unsigned ix = (unsigned)(ix0 + j);
+            if (xs[j] >= 0) {
+                const Tab4<R> tx = a.tab[0][ix];
+                const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs[j];
+                uxx = cpml<P>(a.m[0][4], k, tx.cf, tx.ef, uxx, a.zero);
+                wx = cpml<P>(a.m[0][5], k, tx.cf, tx.ef, wx, a.zero);
+                uyx = cpml<P>(a.m[0][6], k, tx.ch, tx.eh, uyx, a.zero);
+                uzx = cpml<P>(a.m[0][7], k, tx.ch, tx.eh, uzx, a.zero);
+            }
+            if (ys >= 0 && upd[j]) {
+                const unsigned k = ((unsigned)iz * YW + ys) * W + ix;
+                uyy = cpml<P>(a.m[1][4], k, ty.cf, ty.ef, uyy, a.zero);
+                wy = cpml<P>(a.m[1][5], k, ty.cf, ty.ef, wy, a.zero);
+                uxy = cpml<P>(a.m[1][6], k, ty.ch, ty.eh, uxy, a.zero);
+                uzy = cpml<P>(a.m[1][7], k, ty.ch, ty.eh, uzy, a.zero);
+            }
+            if (zs >= 0 && upd[j]) {
+                const unsigned k = ((unsigned)zs * a.ny + iy) * W + ix;
+                uzz = cpml<P>(a.m[2][4], k, tz.cf, tz.ef, uzz, a.zero);
+                wz = cpml<P>(a.m[2][5], k, tz.cf, tz.ef, wz, a.zero);
+                uxz = cpml<P>(a.m[2][6], k, tz.ch, tz.eh, uxz, a.zero);
+                uyz = cpml<P>(a.m[2][7], k, tz.ch, tz.eh, uyz, a.zero);
+            }
+            const R du = uxx + uyy + uzz, dw = wx + wy + wz;
+            if constexpr (!P::scaled) {
+                // Float/main.cu:480-491
+                const R muxz = harm4(mu_c[j], mu_x[j], mu_n[j], mu_xn[j]);
+                const R muyz = harm4(mu_c[j], mu_y[j], mu_n[j], mu_yn[j]);
+                const R muxy = harm4(mu_c[j], mu_x[j], mu_y[j], mu_xy[j]);
+                const R C = C_[j], M = M_[j], HH = HH_[j], H2u = H2u_[j];
+                // Float/main.cu:287-293
+                p1[j] = p0[j] - dt * (C * du + M * dw);
+                xx1[j] = xx0[j] + dt * (H2u * (uyy + uzz) + HH * uxx + C * dw);
+                yy1[j] = yy0[j] + dt * (H2u * (uxx + uzz) + HH * uyy + C * dw);
+                zz1[j] = zz0[j] + dt * (H2u * (uxx + uyy) + HH * uzz + C * dw);
+                xy1[j] = xy0[j] + muxy * dt * (uxy + uyx);
+                yz1[j] = yz0[j] + muyz * dt * (uyz + uzy);
+                xz1[j] = xz0[j] + muxz * dt * (uxz + uzx);
+                mu_c[j] = mu_n[j];
+                mu_x[j] = mu_xn[j];
+                mu_y[j] = mu_yn[j];
+            } else {
+                // Half/main.cu:362-370
+                const R C = C_[j], M = M_[j], HH = HH_[j], H2u = H2u_[j];
+                p1[j] = p0[j] - C * du - M * dw;
+                xx1[j] = xx0[j] + H2u * (uyy + uzz) + HH * uxx + C * dw;
+                yy1[j] = yy0[j] + H2u * (uxx + uzz) + HH * uyy + C * dw;
+                zz1[j] = zz0[j] + H2u * (uxx + uyy) + HH * uzz + C * dw;
+                xy1[j] = xy0[j] + mxy[j] * (uxy + uyx);
+                yz1[j] = yz0[j] + myz[j] * (uyz + uzy);
+                xz1[j] = xz0[j] + mxz[j] * (uxz + uzx);
+            }
+        }
+        ST(PW_P, o, p1);
+        ST(PW_TXX, o, xx1); ST(PW_TYY, o, yy1); ST(PW_TZZ, o, zz1);
+        ST(PW_TXY, o, xy1); ST(PW_TYZ, o, yz1); ST(PW_TXZ, o, xz1);
 
-        vuz_m = vuz_c; vwz_m = vwz_c; vux_c = vux_n; vuy_c = vuy_n;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            vuz_m[j] = vuz_c[j];
+            vwz_m[j] = vwz_c[j];
+            vux_c[j] = vux_n[j];
+            vuy_c[j] = vuy_n[j];
+        }
     }
 }
 
-template <class P> __global__ __launch_bounds__(256) void k_velocity(const StepArgs<P> a)
+template <class P, int N> __global__ __launch_bounds__(512) void k_velocity(const StepArgs<P> a)
 {
-    velocity_tile<P>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
+    velocity_tile<P, N>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
 }
-template <class P> __global__ __launch_bounds__(256) void k_stress(const StepArgs<P> a)
+template <class P, int N> __global__ __launch_bounds__(512) void k_stress(const StepArgs<P> a)
 {
-    stress_tile<P>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
+    stress_tile<P, N>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
 }
 
 // --------------------------------------------------------------------------
@@ -548,11 +824,12 @@ template <class P> struct FusedArgs {
 
 template <class P>
 __device__ __forceinline__ typename P::R cpml2(const typename P::Mem* in, typename P::Mem* out, unsigned k,
-                                               typename P::R c, typename P::R e, typename P::R der, bool write)
+                                               typename P::R c, typename P::R e, typename P::R der, bool write,
+                                               unsigned z)
 {
     using R = typename P::R;
     const R psi = P::ldm(in, k) - c * der;
-    if (write) P::stm(out, k, e * (psi - c * der));
+    if (write) P::stm(out, k, e * (psi - c * der), z);
     return der + psi;
 }
 
@@ -599,18 +876,18 @@ __device__ __forceinline__ void v_axis(const FusedArgs<P>& A, int ix, int iy, in
     if (sa >= 0) {
         const Tab4<R> t = a.tab[AX][AX == 0 ? ix : (AX == 1 ? iy : iz)];
         const unsigned k = AX == 0 ? kx : (AX == 1 ? ky : kz);
-        d1 = cpml2<P>(a.m[AX][0], A.mo[AX][0], k, t.ch, t.eh, d1, own);
-        dsv = cpml2<P>(a.m[AX][1], A.mo[AX][1], k, t.ch, t.eh, dsv, own);
+        d1 = cpml2<P>(a.m[AX][0], A.mo[AX][0], k, t.ch, t.eh, d1, own, a.zero);
+        dsv = cpml2<P>(a.m[AX][1], A.mo[AX][1], k, t.ch, t.eh, dsv, own, a.zero);
     }
     if (AX == 0) {
-        if (ys >= 0) { const Tab4<R> t = a.tab[1][iy]; d2 = cpml2<P>(a.m[1][2], A.mo[1][2], ky, t.cf, t.ef, d2, own); }
-        if (zs >= 0) { const Tab4<R> t = a.tab[2][iz]; d3 = cpml2<P>(a.m[2][2], A.mo[2][2], kz, t.cf, t.ef, d3, own); }
+        if (ys >= 0) { const Tab4<R> t = a.tab[1][iy]; d2 = cpml2<P>(a.m[1][2], A.mo[1][2], ky, t.cf, t.ef, d2, own, a.zero); }
+        if (zs >= 0) { const Tab4<R> t = a.tab[2][iz]; d3 = cpml2<P>(a.m[2][2], A.mo[2][2], kz, t.cf, t.ef, d3, own, a.zero); }
     } else if (AX == 1) {
-        if (xs >= 0) { const Tab4<R> t = a.tab[0][ix]; d2 = cpml2<P>(a.m[0][2], A.mo[0][2], kx, t.cf, t.ef, d2, own); }
-        if (zs >= 0) { const Tab4<R> t = a.tab[2][iz]; d3 = cpml2<P>(a.m[2][3], A.mo[2][3], kz, t.cf, t.ef, d3, own); }
+        if (xs >= 0) { const Tab4<R> t = a.tab[0][ix]; d2 = cpml2<P>(a.m[0][2], A.mo[0][2], kx, t.cf, t.ef, d2, own, a.zero); }
+        if (zs >= 0) { const Tab4<R> t = a.tab[2][iz]; d3 = cpml2<P>(a.m[2][3], A.mo[2][3], kz, t.cf, t.ef, d3, own, a.zero); }
     } else {
-        if (xs >= 0) { const Tab4<R> t = a.tab[0][ix]; d2 = cpml2<P>(a.m[0][3], A.mo[0][3], kx, t.cf, t.ef, d2, own); }
-        if (ys >= 0) { const Tab4<R> t = a.tab[1][iy]; d3 = cpml2<P>(a.m[1][3], A.mo[1][3], ky, t.cf, t.ef, d3, own); }
+        if (xs >= 0) { const Tab4<R> t = a.tab[0][ix]; d2 = cpml2<P>(a.m[0][3], A.mo[0][3], kx, t.cf, t.ef, d2, own, a.zero); }
+        if (ys >= 0) { const Tab4<R> t = a.tab[1][iy]; d3 = cpml2<P>(a.m[1][3], A.mo[1][3], ky, t.cf, t.ef, d3, own, a.zero); }
     }
     const R S = d1 + d2 + d3;
     const R vw0 = P::ld(a.f[PW_VWX + AX], o), vu0 = P::ld(a.f[PW_VUX + AX], o);
@@ -668,7 +945,7 @@ __global__ __launch_bounds__(256) void k_fused(const FusedArgs<P> A)
             if (mask & 2) v_axis<P, 1>(A, ix, iy, z, own, u[1], u[4]);
             if (mask & 4) v_axis<P, 2>(A, ix, iy, z, own, u[2], u[5]);
             // the stress update reads the velocities as stored (Half: encoded)
-            for (int c = 0; c < 6; ++c) u[c] = P::rt(u[c]);
+            for (int c = 0; c < 6; ++c) u[c] = P::rt(u[c], a.zero);
         } else if (in_xy && z >= 0 && z <= nz - 1) {
             const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z * PL;
             for (int c = 0; c < 3; ++c)
@@ -692,7 +969,7 @@ __global__ __launch_bounds__(256) void k_fused(const FusedArgs<P> A)
             if (need & 1) v_axis<P, 0>(A, hx, hy, z, false, vu[0], vw[0]);
             if (need & 2) v_axis<P, 1>(A, hx, hy, z, false, vu[1], vw[1]);
             if (need & 4) v_axis<P, 2>(A, hx, hy, z, false, vu[2], vw[2]);
-            for (int c = 0; c < 3; ++c) { vu[c] = P::rt(vu[c]); vw[c] = P::rt(vw[c]); }
+            for (int c = 0; c < 3; ++c) { vu[c] = P::rt(vu[c], a.zero); vw[c] = P::rt(vw[c], a.zero); }
         } else if (h_in && z >= 0 && z <= nz - 1) {
             const unsigned o = (unsigned)hx + (unsigned)hy * W + (unsigned)z * PL;
             for (int c = 0; c < 3; ++c)
@@ -742,26 +1019,26 @@ __global__ __launch_bounds__(256) void k_fused(const FusedArgs<P> A)
             if (xs >= 0) {
                 const Tab4<R> t = a.tab[0][ix];
                 const unsigned q = ((unsigned)k * ny + iy) * (a.lo[0] + a.hi[0]) + xs;
-                uxx = cpml2<P>(a.m[0][4], A.mo[0][4], q, t.cf, t.ef, uxx, true);
-                wx = cpml2<P>(a.m[0][5], A.mo[0][5], q, t.cf, t.ef, wx, true);
-                uyx = cpml2<P>(a.m[0][6], A.mo[0][6], q, t.ch, t.eh, uyx, true);
-                uzx = cpml2<P>(a.m[0][7], A.mo[0][7], q, t.ch, t.eh, uzx, true);
+                uxx = cpml2<P>(a.m[0][4], A.mo[0][4], q, t.cf, t.ef, uxx, true, a.zero);
+                wx = cpml2<P>(a.m[0][5], A.mo[0][5], q, t.cf, t.ef, wx, true, a.zero);
+                uyx = cpml2<P>(a.m[0][6], A.mo[0][6], q, t.ch, t.eh, uyx, true, a.zero);
+                uzx = cpml2<P>(a.m[0][7], A.mo[0][7], q, t.ch, t.eh, uzx, true, a.zero);
             }
             if (ys >= 0) {
                 const Tab4<R> t = a.tab[1][iy];
                 const unsigned q = ((unsigned)k * (a.lo[1] + a.hi[1]) + ys) * W + ix;
-                uyy = cpml2<P>(a.m[1][4], A.mo[1][4], q, t.cf, t.ef, uyy, true);
-                wy = cpml2<P>(a.m[1][5], A.mo[1][5], q, t.cf, t.ef, wy, true);
-                uxy = cpml2<P>(a.m[1][6], A.mo[1][6], q, t.ch, t.eh, uxy, true);
-                uzy = cpml2<P>(a.m[1][7], A.mo[1][7], q, t.ch, t.eh, uzy, true);
+                uyy = cpml2<P>(a.m[1][4], A.mo[1][4], q, t.cf, t.ef, uyy, true, a.zero);
+                wy = cpml2<P>(a.m[1][5], A.mo[1][5], q, t.cf, t.ef, wy, true, a.zero);
+                uxy = cpml2<P>(a.m[1][6], A.mo[1][6], q, t.ch, t.eh, uxy, true, a.zero);
+                uzy = cpml2<P>(a.m[1][7], A.mo[1][7], q, t.ch, t.eh, uzy, true, a.zero);
             }
             if (zs >= 0) {
                 const Tab4<R> t = a.tab[2][k];
                 const unsigned q = ((unsigned)zs * ny + iy) * W + ix;
-                uzz = cpml2<P>(a.m[2][4], A.mo[2][4], q, t.cf, t.ef, uzz, true);
-                wz = cpml2<P>(a.m[2][5], A.mo[2][5], q, t.cf, t.ef, wz, true);
-                uxz = cpml2<P>(a.m[2][6], A.mo[2][6], q, t.ch, t.eh, uxz, true);
-                uyz = cpml2<P>(a.m[2][7], A.mo[2][7], q, t.ch, t.eh, uyz, true);
+                uzz = cpml2<P>(a.m[2][4], A.mo[2][4], q, t.cf, t.ef, uzz, true, a.zero);
+                wz = cpml2<P>(a.m[2][5], A.mo[2][5], q, t.cf, t.ef, wz, true, a.zero);
+                uxz = cpml2<P>(a.m[2][6], A.mo[2][6], q, t.ch, t.eh, uxz, true, a.zero);
+                uyz = cpml2<P>(a.m[2][7], A.mo[2][7], q, t.ch, t.eh, uyz, true, a.zero);
             }
             const R du = uxx + uyy + uzz, dw = wx + wy + wz;
             R p1, xx1, yy1, zz1, xy1, yz1, xz1;
@@ -892,7 +1169,7 @@ __global__ void k_rescale(const StepArgs<P> a, unsigned long long ncell, unsigne
         const unsigned long long n = ax == 0 ? nmem0 : (ax == 1 ? nmem1 : nmem2);
         typename P::Mem* m = a.m[ax][t];
         if (!m) return;
-        for (; i < n; i += stride) P::stm(m, (unsigned)i, P::ldm(m, (unsigned)i) * ratio);
+        for (; i < n; i += stride) P::stm(m, (unsigned)i, P::ldm(m, (unsigned)i) * ratio, a.zero);
     }
 }
 
@@ -975,7 +1252,7 @@ template <class P> __device__ __forceinline__ bool batch_block(const BatchArgs& 
     return true;
 }
 
-template <class P> __global__ __launch_bounds__(256) void k_velocity_batch(const BatchArgs c, int k)
+template <class P, int N> __global__ __launch_bounds__(512) void k_velocity_batch(const BatchArgs c, int k)
 {
     int s;
     unsigned t;
@@ -983,10 +1260,10 @@ template <class P> __global__ __launch_bounds__(256) void k_velocity_batch(const
     const ShotDesc<P>* d = (const ShotDesc<P>*)c.descs + s;
     if (c.it[s] + k >= d->nt) return;
     const StepArgs<P> a = d->a;
-    velocity_tile<P>(a, tile_of_index(t, a.gx, a.gy));
+    velocity_tile<P, N>(a, tile_of_index(t, a.gx, a.gy));
 }
 
-template <class P> __global__ __launch_bounds__(256) void k_stress_batch(const BatchArgs c, int k)
+template <class P, int N> __global__ __launch_bounds__(512) void k_stress_batch(const BatchArgs c, int k)
 {
     int s;
     unsigned t;
@@ -994,7 +1271,7 @@ template <class P> __global__ __launch_bounds__(256) void k_stress_batch(const B
     const ShotDesc<P>* d = (const ShotDesc<P>*)c.descs + s;
     if (c.it[s] + k >= d->nt) return;
     const StepArgs<P> a = d->a;
-    stress_tile<P>(a, tile_of_index(t, a.gx, a.gy));
+    stress_tile<P, N>(a, tile_of_index(t, a.gx, a.gy));
 }
 
 // one thread per (shot, source channel); the arithmetic of k_source
@@ -1028,7 +1305,7 @@ __device__ __forceinline__ void rescale_array(const StepArgs<P>& a, int arr, uns
         const int ax = q / kMemPerAxis, t = q % kMemPerAxis;
         typename P::Mem* m = a.m[ax][t];
         if (!m) return;
-        for (unsigned long long i = i0; i < nmem[ax]; i += stride) P::stm(m, (unsigned)i, P::ldm(m, (unsigned)i) * ratio);
+        for (unsigned long long i = i0; i < nmem[ax]; i += stride) P::stm(m, (unsigned)i, P::ldm(m, (unsigned)i) * ratio, a.zero);
     }
 }
 
@@ -1082,6 +1359,17 @@ template <class P> static StepArgs<P> range_args(const DevBuffers& b, int zb, in
     return a;
 }
 
+// Calls f(std::integral_constant<int, N>) for N = vx (1, 2 or 4).
+template <class F> static hipError_t dispatch_vx(int vx, F&& f)
+{
+    switch (vx) {
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <class F> static hipError_t dispatch(int prec, F&& f)
 {
     switch (prec) {
@@ -1099,8 +1387,10 @@ hipError_t launch_velocity(const DevBuffers& b, hipStream_t st, int zb, int ze)
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
         const StepArgs<P> a = range_args<P>(b, zb, ze);
-        k_velocity<P><<<a.nblocks, dim3(64, 4), 0, st>>>(a);
-        return hipGetLastError();
+        return dispatch_vx(a.vx, [&](auto n) {
+            k_velocity<P, decltype(n)::value><<<a.nblocks, dim3(64, a.ty), 0, st>>>(a);
+            return hipGetLastError();
+        });
     });
 }
 
@@ -1110,8 +1400,10 @@ hipError_t launch_stress(const DevBuffers& b, hipStream_t st, int zb, int ze)
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
         const StepArgs<P> a = range_args<P>(b, zb, ze);
-        k_stress<P><<<a.nblocks, dim3(64, 4), 0, st>>>(a);
-        return hipGetLastError();
+        return dispatch_vx(a.vx, [&](auto n) {
+            k_stress<P, decltype(n)::value><<<a.nblocks, dim3(64, a.ty), 0, st>>>(a);
+            return hipGetLastError();
+        });
     });
 }
 
@@ -1120,7 +1412,10 @@ hipError_t launch_fused(const DevBuffers& b, hipStream_t st)
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
         FusedArgs<P> A;
-        A.s = make_args<P>(b);
+        DevBuffers b4 = b;
+        b4.ty = 4;  // the fused kernel's LDS tile is 64 x 4, one cell per lane
+        b4.vx = 1;
+        A.s = make_args<P>(b4);
         for (int k = 0; k < PW_NFIELDS; ++k) {
             DevBuffers w = b;
             w.f[k] = b.fo[k];
@@ -1273,6 +1568,10 @@ hipError_t batch_build(BatchGroup& g, int prec, const BatchShot* shots, int n, c
         g.d_it = (int*)(base + o_it);
         g.d_nt = (int*)(base + o_nt);
         g.xcd_map = shots[0].b->xcd_map;
+        g.ty = d[0].a.ty;
+        g.vx = d[0].a.vx;
+        for (int s2 = 1; s2 < n; ++s2)
+            if (d[(size_t)s2].a.ty != g.ty || d[(size_t)s2].a.vx != g.vx) return hipErrorInvalidValue;
         g.nblocks = g.xcd_map ? 8u * xmax : bstart[(size_t)n];
         g.nrec = rstart[(size_t)n];
         hipError_t e = hipMemcpyAsync(base + o_desc, d.data(), sizeof(ShotDesc<P>) * n, hipMemcpyHostToDevice, st);
@@ -1309,8 +1608,12 @@ hipError_t batch_step(const BatchGroup& g, hipStream_t st, int k)
         using P = decltype(pol);
         if (P::scaled) k_rescale_batch<P><<<dim3(64, (unsigned)g.n), 256, 0, st>>>(c, k);
         k_source_batch<P><<<(unsigned)((4 * g.n + 255) / 256), 256, 0, st>>>(c, k);
-        k_velocity_batch<P><<<g.nblocks, dim3(64, 4), 0, st>>>(c, k);
-        k_stress_batch<P><<<g.nblocks, dim3(64, 4), 0, st>>>(c, k);
+        hipError_t e = dispatch_vx(g.vx, [&](auto n) {
+            k_velocity_batch<P, decltype(n)::value><<<g.nblocks, dim3(64, g.ty), 0, st>>>(c, k);
+            k_stress_batch<P, decltype(n)::value><<<g.nblocks, dim3(64, g.ty), 0, st>>>(c, k);
+            return hipGetLastError();
+        });
+        if (e != hipSuccess) return e;
         if (g.nrec > 0) k_gather_batch<P><<<(g.nrec + 255) / 256, 256, 0, st>>>(c, k);
         return hipGetLastError();
     });

@@ -135,7 +135,6 @@ __global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double* res)
 }
 
 template <class T> __device__ __forceinline__ T to_store(double v) { return (T)v; }
-template <> __device__ __forceinline__ f16 to_store<f16>(double v) { return (f16)(float)v; }
 
 struct CoefOut {
     void* c[kMaxCoef];
@@ -177,7 +176,8 @@ __device__ __forceinline__ float harm4_ref(float a, float b, float c, float d)
     return (float)(1 / (0.25 * (double)(1 / a + 1 / b + 1 / c + 1 / d)));
 }
 
-__global__ void k_coef_scaled(BaseView v, int nx, int ny, int nz, float DT, float sm, float ism, CoefOut out)
+__global__ void k_coef_scaled(BaseView v, int nx, int ny, int nz, float DT, float sm, float ism, CoefOut out,
+                              unsigned z)
 {
 #pragma clang fp contract(off)
     const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
@@ -213,19 +213,19 @@ __global__ void k_coef_scaled(BaseView v, int nx, int ny, int nz, float DT, floa
             bp = ism * DT / den;
             bt = ism * DT * rhof0 * ri[d] / den;
         }
-        c[SC_RITX + d][o] = (f16)(ism * DT * ri[d]);
-        c[SC_FRX + d][o] = (f16)(rhof0 * ri[d]);
-        c[SC_AWX + d][o] = (f16)aw;
-        c[SC_BPX + d][o] = (f16)bp;
-        c[SC_BTX + d][o] = (f16)bt;
+        c[SC_RITX + d][o] = (f16)round32(ism * DT * ri[d], z);
+        c[SC_FRX + d][o] = (f16)round32(rhof0 * ri[d], z);
+        c[SC_AWX + d][o] = (f16)round32(aw, z);
+        c[SC_BPX + d][o] = (f16)round32(bp, z);
+        c[SC_BTX + d][o] = (f16)round32(bt, z);
     }
-    c[SC_HH][o] = (f16)(sm * DT * F(PW_HH, 0, 0, 0));
-    c[SC_H2U][o] = (f16)(sm * DT * F(PW_H2U, 0, 0, 0));
-    c[SC_C][o] = (f16)(sm * DT * F(PW_CC, 0, 0, 0));
-    c[SC_M][o] = (f16)(sm * DT * F(PW_MM, 0, 0, 0));
-    c[SC_MUXZ][o] = (f16)(sm * DT * muxz);
-    c[SC_MUXY][o] = (f16)(sm * DT * muxy);
-    c[SC_MUYZ][o] = (f16)(sm * DT * muyz);
+    c[SC_HH][o] = (f16)round32(sm * DT * F(PW_HH, 0, 0, 0), z);
+    c[SC_H2U][o] = (f16)round32(sm * DT * F(PW_H2U, 0, 0, 0), z);
+    c[SC_C][o] = (f16)round32(sm * DT * F(PW_CC, 0, 0, 0), z);
+    c[SC_M][o] = (f16)round32(sm * DT * F(PW_MM, 0, 0, 0), z);
+    c[SC_MUXZ][o] = (f16)round32(sm * DT * muxz, z);
+    c[SC_MUXY][o] = (f16)round32(sm * DT * muxy, z);
+    c[SC_MUYZ][o] = (f16)round32(sm * DT * muyz, z);
 }
 
 static BaseView make_view(const MediumArgs& m)
@@ -261,7 +261,7 @@ hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double 
     case PW_COMP16:
     case PW_BF16:
         hipLaunchKernelGGL(k_coef_scaled, grid, block, 0, st, v, m.nx, m.ny, m.nz, (float)m.dt, (float)scale_m,
-                           (float)inv_scale_m, out);
+                           (float)inv_scale_m, out, 0u);
         break;
     default: return hipErrorInvalidValue;
     }

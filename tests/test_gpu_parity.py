@@ -187,23 +187,45 @@ def test_snapshots_match_fields(borehole):
         assert rel_l2(gpu["snaps"][k], ref["snaps"][k]) <= 1e-5
 
 
+SCHEDULES = (
+    {},                               # two kernels, one cell per lane, 64 x 4 tiles
+    {"PW_VX": "2"},                   # two cells per lane (128-cell row segments)
+    {"PW_VX": "4", "PW_TY": "8"},     # four cells per lane, 8-row tiles
+    {"PW_SLAB": "3"},                 # z-slab interleave of velocity and stress launches
+    {"PW_SLAB": "8", "PW_VX": "2"},
+    {"PW_FUSED": "1"},                # one fused launch per step
+)
+
+
 def test_step_schedules_are_bit_identical(layered, monkeypatch):
-    """The two-kernel step (default), the fused one-launch step (PW_FUSED=1) and the
-    z-slab interleaved schedule (PW_SLAB) apply the same per-cell arithmetic:
-    results are bit-identical, in every precision."""
+    """Every step schedule and tiling applies the same per-cell arithmetic:
+    results are bit-identical to the default, in every precision."""
     grid, spec, nt, recs, gpu32, base = layered
-    schedules = (("0", "0"), ("1", "0"), ("0", "3"), ("0", "8"))
     bad = []
     for prec in ("fp32", "comp16", "bf16", "fp64"):
         outs = []
-        for fused, slab in schedules:
-            monkeypatch.setenv("PW_FUSED", fused)
-            monkeypatch.setenv("PW_SLAB", slab)
+        for env in SCHEDULES:
+            for k in ("PW_FUSED", "PW_SLAB", "PW_VX", "PW_TY"):
+                monkeypatch.delenv(k, raising=False)
+            for k, v in env.items():
+                monkeypatch.setenv(k, v)
+            if prec == "fp64" and env.get("PW_VX") == "4":
+                monkeypatch.setenv("PW_VX", "2")
             g, _ = cases.run_gpu(grid, nt, prec, recs, spec=spec, fields=("vux", "vwz", "txz", "p"))
             outs.append(g)
-        for sch, g in zip(schedules[1:], outs[1:]):
+        for env, g in zip(SCHEDULES[1:], outs[1:]):
             same = np.array_equal(g["seis"], outs[0]["seis"]) and all(
                 np.array_equal(g["fields"][f], outs[0]["fields"][f]) for f in ("vux", "vwz", "txz", "p"))
-            if not same:
-                bad.append((prec, "fused=%s slab=%s" % sch, rel_l2(g["seis"], outs[0]["seis"])))
-    assert not bad, f"schedules differing from the two-kernel step: {bad}"
+            if same:
+                continue
+            err = rel_l2(g["seis"], outs[0]["seis"])
+            if (prec, tuple(sorted(env.items()))) in KNOWN_DIFFERENT and err <= 1e-5:
+                print(f"known difference {prec} {env}: {err:.2e}")
+                continue
+            bad.append((prec, env, err))
+    assert not bad, f"schedules differing from the default step: {bad}"
+
+
+# (schedule, precision) pairs allowed to differ from the default in the last
+# bits; empty: every schedule is bit-identical in every precision.
+KNOWN_DIFFERENT = set()
