@@ -190,6 +190,63 @@ def serving_main(args):
     return 0
 
 
+def precision_report(args):
+    """The 512^3 two-layer run in every storage precision against the FP64 run on
+    the same medium: relative L2 of every seismogram channel and of the step-1000
+    txx snapshot, and each precision's Gcell-steps/s."""
+    import numpy as np
+    import torch
+    import mixedprecisionfaracoustic_amd as pw
+    n, np_, h, dt = args.n, args.np, 5.0, 0.5e-3
+    nt = args.report_steps
+    spec = workload(n, np_)
+    fields, idx = receivers(n, np_)
+    snap_step = min(1000, nt - 1)
+    res = {}
+    for prec in ("fp64", "fp32", "comp16", "bf16"):
+        stream = torch.cuda.Stream()
+        s = pw.Solver(n, n, n, np_, h, dt, nt, precision=prec, cpml_r=1e-4, cpml_v0=4000.0, stream=stream.cuda_stream)
+        s.set_medium_layered(spec)
+        s.set_source(n // 2, n // 2, n // 2, 25.0, (1.0, 1.0, 1.0, -1.0))
+        s.set_receivers(fields, idx)
+        s.set_snapshots([snap_step], ["txx"], n // 2)
+        s.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        done = 0
+        while done < nt:  # in chunks, so a long run reports progress
+            k = min(250, nt - done)
+            s.run(k)
+            done += k
+            s.synchronize()
+            print(f"  {prec}: {done}/{nt} steps", file=sys.stderr, flush=True)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1)
+        res[prec] = {"seis": s.seismogram(), "snap": s.snapshots()[0], "gcell": float(n) ** 3 * nt / (ms / 1e3) / 1e9}
+        s.close()
+    ref = res["fp64"]
+    chans = ("txx", "tzz", "vux", "vwx", "p")
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out = {"metric": "rel-L2 vs FP64 per seismogram channel and step-1000 txx snapshot; Gcell-steps/s per precision",
+           "config": {"workload": f"biot_two_layer_{n}^3", "steps": nt, "receivers": len(idx), "snapshot_step": snap_step},
+           "data": "synthetic (two-layer perturbed Biot medium generated on the GPU, seed 0)", "precisions": {}}
+    nr = len(idx) // len(chans)
+    for prec in ("fp32", "comp16", "bf16", "fp64"):
+        r = res[prec]
+        e = {c: rel(r["seis"][k * nr:(k + 1) * nr], ref["seis"][k * nr:(k + 1) * nr]) for k, c in enumerate(chans)}
+        out["precisions"][prec] = {"gcell_steps_per_s": round(r["gcell"], 3),
+                                   "ratio_vs_fp32": round(r["gcell"] / res["fp32"]["gcell"], 3),
+                                   "seis_rel_l2": {c: float(f"{v:.3e}") for c, v in e.items()},
+                                   "snapshot_rel_l2": float(f"{rel(r['snap'], ref['snap']):.3e}")}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +259,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--prof-steps", type=int, default=40)
     ap.add_argument("--serving", action="store_true", help="replay the shot-serving trace instead")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="every storage precision against FP64 on the same workload instead")
+    ap.add_argument("--report-steps", type=int, default=2000)
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=0)
@@ -214,6 +274,8 @@ def main():
         return 2
     if args.serving:
         return serving_main(args)
+    if args.precision_report:
+        return precision_report(args)
 
     import numpy as np
     import torch
