@@ -96,6 +96,10 @@ typedef struct pw_config {
     int check_cfl;          /* 1: pw_set_medium refuses media violating the Courant
                                limit (the reference only prints a warning,
                                Float/main.cu:940-942) */
+    int space_order;        /* staggered differences: 2 (the reference, Float/main.cu:115-128,
+                               234-249) or 4 (9/8, -1/24; two-cell halo, the outermost
+                               two layers are not updated; Courant limit 6/7).  4 runs
+                               the two-kernel step with one cell per lane. */
 } pw_config;
 
 /* Fills a config with the reference's values (Float/main.cu:20-23, 305-322):
@@ -168,6 +172,34 @@ pw_status pw_get_field(pw_solver* s, int field, float* host);
 /* Replaces wfile / wfile3d (Float/main.cu:1501-1516, 1563-1588): writes nr rows
  * of nc values as "%e " text, each row followed by "\n" and a blank line. */
 pw_status pw_write_dat(const char* path, const float* data, int nr, int nc);
+
+/* Binary seismogram / snapshot file (no reference counterpart: the reference
+ * writes text only).  An 80-byte header, then nr*nc little-endian FP32 values,
+ * row-major.  Values times `scale` are in physical units (the library writes 1). */
+enum { PW_BIN_SEISMOGRAM = 0, PW_BIN_SNAPSHOT = 1 };
+typedef struct pw_bin_header {
+    char magic[4];          /* "PWB1" (set by pw_write_bin) */
+    int version;            /* 1 (set by pw_write_bin) */
+    int kind;               /* PW_BIN_SEISMOGRAM ([receiver][step]) or PW_BIN_SNAPSHOT ([z][x]) */
+    int nr, nc;             /* rows, columns */
+    int field;              /* pw field id, or -1 for a mixed receiver set */
+    int step;               /* snapshot step, or -1 */
+    int reserved;
+    double dt, h;           /* time step [s] and grid step [m] of the run */
+    double scale;
+    char label[24];         /* NUL-terminated name, e.g. "sisx", "txx500" */
+} pw_bin_header;
+pw_status pw_write_bin(const char* path, const pw_bin_header* hdr, const float* data);
+/* Reads the header, and nr*nc values into data unless data is NULL. */
+pw_status pw_read_bin(const char* path, pw_bin_header* hdr, float* data);
+
+/* Writes the reference program's output set from a handle into directory dir
+ * (Float/main.cu:1187-1222): with pw_set_receivers_reference, sisx, sisz, sisvu,
+ * sisvw, sisp ([nz][nt] each); otherwise one "seismogram" file ([nrec][nt]); and
+ * one file per snapshot named field + step ("txx500", ...).  format: PW_FORMAT_DAT
+ * (the reference's text, ".dat") or PW_FORMAT_BIN (".bin"). */
+enum { PW_FORMAT_DAT = 0, PW_FORMAT_BIN = 1 };
+pw_status pw_write_outputs(pw_solver* s, const char* dir, int format);
 
 typedef struct pw_info {
     int precision;

@@ -39,7 +39,7 @@ class Config(C.Structure):
                 ("h", C.c_double), ("dt", C.c_double), ("nt", C.c_int), ("precision", C.c_int),
                 ("cpml_r", C.c_double), ("cpml_v0", C.c_double),
                 ("half_es", C.c_int), ("half_dyn_end", C.c_double), ("half_dyn_interval", C.c_int),
-                ("device", C.c_int), ("stream", C.c_void_p), ("check_cfl", C.c_int)]
+                ("device", C.c_int), ("stream", C.c_void_p), ("check_cfl", C.c_int), ("space_order", C.c_int)]
 
 
 class Info(C.Structure):
@@ -90,6 +90,14 @@ class ServerStats(C.Structure):
                 ("peak_active", C.c_int)]
 
 
+class BinHeader(C.Structure):
+    _fields_ = [("magic", C.c_char * 4), ("version", C.c_int), ("kind", C.c_int), ("nr", C.c_int), ("nc", C.c_int),
+                ("field", C.c_int), ("step", C.c_int), ("reserved", C.c_int), ("dt", C.c_double), ("h", C.c_double),
+                ("scale", C.c_double), ("label", C.c_char * 24)]
+
+
+FORMATS = {"dat": 0, "bin": 1}
+
 # symbol -> (restype, argtypes); the same list the header declares.
 _SIGS = {
     "pw_config_default": (None, [C.POINTER(Config)]),
@@ -117,6 +125,9 @@ _SIGS = {
     "pw_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "pw_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                       C.POINTER(C.c_longlong)]),
+    "pw_write_bin": (C.c_int, [C.c_char_p, C.POINTER(BinHeader), C.POINTER(C.c_float)]),
+    "pw_read_bin": (C.c_int, [C.c_char_p, C.POINTER(BinHeader), C.POINTER(C.c_float)]),
+    "pw_write_outputs": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "pw_server_config_default": (None, [C.POINTER(ServerConfig)]),
     "pw_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "pw_server_destroy": (None, [C.c_void_p]),
@@ -204,6 +215,30 @@ def write_dat(path, data):
     _check(lib().pw_write_dat(os.fsencode(path), _fp(a), a.shape[0], a.shape[1]))
 
 
+def write_bin(path, data, kind=0, field=-1, step=-1, dt=0.0, h=0.0, label=""):
+    """Binary seismogram (kind 0) or snapshot (kind 1) file: 80-byte header + FP32 rows."""
+    a = np.ascontiguousarray(data, np.float32)
+    if a.ndim != 2:
+        raise ValueError("write_bin expects a 2-D array")
+    hd = BinHeader()
+    hd.kind, hd.nr, hd.nc, hd.field, hd.step = int(kind), a.shape[0], a.shape[1], int(field), int(step)
+    hd.dt, hd.h, hd.scale = float(dt), float(h), 1.0
+    hd.label = label.encode()[:23]
+    _check(lib().pw_write_bin(os.fsencode(path), C.byref(hd), _fp(a)))
+
+
+def read_bin(path):
+    """(header dict, [nr, nc] float32 array) of a binary file written by write_bin / write_outputs."""
+    hd = BinHeader()
+    _check(lib().pw_read_bin(os.fsencode(path), C.byref(hd), None))
+    out = np.zeros((hd.nr, hd.nc), np.float32)
+    _check(lib().pw_read_bin(os.fsencode(path), C.byref(hd), _fp(out) if out.size else None))
+    head = {k: getattr(hd, k) for k, _ in BinHeader._fields_}
+    head["magic"] = head["magic"].decode()
+    head["label"] = head["label"].decode()
+    return head, out
+
+
 class Solver:
     """One shot on one GPU (a pw_solver handle).
 
@@ -213,7 +248,8 @@ class Solver:
     """
 
     def __init__(self, nx, ny, nz, np_, h, dt, nt, precision="fp32", cpml_r=1e-6, cpml_v0=4000.0,
-                 device=0, check_cfl=True, half_es=0, half_dyn_end=16.0, half_dyn_interval=100, stream=None):
+                 device=0, check_cfl=True, half_es=0, half_dyn_end=16.0, half_dyn_interval=100, stream=None,
+                 space_order=2):
         L = lib()
         cfg = Config()
         L.pw_config_default(C.byref(cfg))
@@ -225,6 +261,7 @@ class Solver:
         cfg.device = int(device)
         cfg.stream = stream
         cfg.check_cfl = 1 if check_cfl else 0
+        cfg.space_order = int(space_order)
         h_ = C.c_void_p()
         _check(L.pw_create(C.byref(cfg), C.byref(h_)))
         self._h = h_
@@ -338,6 +375,11 @@ class Solver:
         out = np.zeros(self.shape, np.float32)
         _check(lib().pw_get_field(self._h, FIELD_ID[name] if isinstance(name, str) else int(name), _fp(out)))
         return out
+
+    def write_outputs(self, directory, fmt="dat"):
+        """The reference's output files (sisx/sisz/sisvu/sisvw/sisp, txx500, ...) as
+        .dat text or .bin binary into directory."""
+        _check(lib().pw_write_outputs(self._h, os.fsencode(directory), FORMATS[fmt]))
 
     def set_profiling(self, enable=True):
         _check(lib().pw_set_profiling(self._h, 1 if enable else 0))

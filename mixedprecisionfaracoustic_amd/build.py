@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libporowave.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["pw_api.cpp", "pw_medium.cpp", "pw_server.cpp", "pw_kernels.hip", "pw_medium_gpu.hip"]
+SOURCES = ["pw_api.cpp", "pw_medium.cpp", "pw_server.cpp", "pw_output.cpp", "pw_kernels.hip", "pw_medium_gpu.hip"]
 ARCH = os.environ.get("PW_OFFLOAD_ARCH", "gfx950")
 
 
@@ -50,7 +50,7 @@ def _compile(src, verbose):
 
 def build(verbose=False):
     os.makedirs(OBJDIR, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=5) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs

@@ -128,6 +128,7 @@ extern "C" void pw_config_default(pw_config* c)
     c->device = 0;
     c->stream = nullptr;
     c->check_cfl = 1;
+    c->space_order = 2;
 }
 
 extern "C" const char* pw_last_error(void) { return g_last_error.c_str(); }
@@ -142,6 +143,10 @@ pw_status check_config(const pw_config& c)
     if (!(c.h > 0) || !(c.dt > 0) || c.nt < 1) return fail(PW_ERR_INVALID, "pw_create: h, dt and nt must be positive");
     if (!(c.cpml_r > 0 && c.cpml_r < 1) || !(c.cpml_v0 > 0))
         return fail(PW_ERR_INVALID, "pw_create: cpml_r must be in (0,1) and cpml_v0 positive");
+    if (c.space_order != 0 && c.space_order != 2 && c.space_order != 4)
+        return fail(PW_ERR_INVALID, "pw_create: space_order must be 2 or 4");
+    if (c.space_order == 4 && (c.nx < 2 * c.np + 5 || c.ny < 2 * c.np + 5 || c.nz < 2 * c.np + 5))
+        return fail(PW_ERR_INVALID, "pw_create: 4th order needs at least 2*np+5 cells per axis");
     const unsigned long long pitch = (unsigned long long)(c.nx + 63) / 64 * 64;
     if (pitch * c.ny * c.nz >= (1ull << 32) - (1ull << 20))
         return fail(PW_ERR_UNSUPPORTED, "pw_create: grid exceeds 2^32 padded cells (32-bit offsets)");
@@ -243,6 +248,8 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     const pw_config& c = *cfg;
     pw_status cst = check_config(c);
     if (cst != PW_OK) return cst;
+    const int order = c.space_order == 4 ? 4 : 2;
+    if (order == 4 && fused) return fail(PW_ERR_UNSUPPORTED, "pw_create: the fused step is 2nd order only");
 
     int ndev = 0;
     PW_HIP_TRY(hipGetDeviceCount(&ndev));
@@ -304,7 +311,8 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     int xcd = 1;
     env_int("PW_XCD_MAP", &xcd);
     s->b.xcd_map = xcd ? 1 : 0;
-    s->b.slab = std::max(0, slab);
+    s->b.slab = order == 4 ? 0 : std::max(0, slab);  // the z-slab interleave assumes a one-plane halo
+    s->b.order = order;
 
     if (c.stream) {
         s->stream = (hipStream_t)c.stream;
@@ -383,7 +391,9 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
     if (!(res[0] > 0) || !std::isfinite(res[0]) || !std::isfinite(res[1]) || !std::isfinite(res[2]))
         return fail(PW_ERR_INVALID, "pw_set_medium: medium has no positive finite modulus");
     s->vp_max = std::sqrt(res[2]);
-    s->courant = s->vp_max * c.dt / c.h * std::sqrt(3.0);
+    // von Neumann limit of the staggered leapfrog: vp dt / h * sqrt(3) * sum|c_k| <= 1,
+    // sum|c_k| = 1 (2nd order) or 9/8 + 1/24 = 7/6 (4th order)
+    s->courant = s->vp_max * c.dt / c.h * std::sqrt(3.0) * (s->b.order == 4 ? 7.0 / 6.0 : 1.0);
     if (c.check_cfl && !(s->courant <= 1.0))
         return fail(PW_ERR_UNSTABLE, "pw_set_medium: Courant number " + std::to_string(s->courant) +
                                          " > 1 (vp_max " + std::to_string(s->vp_max) + " m/s)");
@@ -565,6 +575,7 @@ extern "C" pw_status pw_set_receivers(pw_solver* s, int n, const int* field, con
     (void)hipFree(s->d_rfield); (void)hipFree(s->d_roff); (void)hipFree(s->d_seis);
     s->d_rfield = nullptr; s->d_roff = nullptr; s->d_seis = nullptr;
     s->nrec = 0;
+    s->ref_receivers = false;
     if (n == 0) return PW_OK;
     PW_HIP_TRY(hipMalloc(&s->d_rfield, n * sizeof(int)));
     PW_HIP_TRY(hipMalloc(&s->d_roff, n * sizeof(unsigned)));
@@ -592,7 +603,9 @@ extern "C" pw_status pw_set_receivers_reference(pw_solver* s, int sx, int sy)
             f.push_back(ch[k]);
             idx.push_back((long long)sx + (long long)sy * c.nx + (long long)iz * c.nx * c.ny);
         }
-    return pw_set_receivers(s, (int)f.size(), f.data(), idx.data());
+    st = pw_set_receivers(s, (int)f.size(), f.data(), idx.data());
+    if (st == PW_OK) s->ref_receivers = true;
+    return st;
 }
 
 extern "C" pw_status pw_set_snapshots(pw_solver* s, int n, const int* step, const int* field, int plane_y)
@@ -690,6 +703,7 @@ extern "C" pw_status pw_run(pw_solver* s, int nsteps)
         if (plan.ratio != 0.0f) PW_HIP_TRY(launch_rescale(s->b, s->stream, plan.ratio));
         if (s->has_source) PW_HIP_TRY(launch_source(s->b, s->stream, s->src_off, s->w, plan.amp));
         const int nz = c.nz;
+        const int halo = s->b.order == 4 ? 2 : 1;  // planes [halo, nz - halo) are updated
         // one step kernel launch, bracketed by events when profiling
         auto step_kernel = [&](int kind, int zb, int ze) -> pw_status {
             if (ze <= zb) return PW_OK;
@@ -716,7 +730,7 @@ extern "C" pw_status pw_run(pw_solver* s, int nsteps)
         };
         pw_status ks = PW_OK;
         if (s->b.fused) {
-            if ((ks = step_kernel(2, 1, nz - 1)) != PW_OK) return ks;
+            if ((ks = step_kernel(2, halo, nz - halo)) != PW_OK) return ks;
             // time n+1 becomes the current state
             for (int q = 0; q < PW_NFIELDS; ++q) {
                 std::swap(s->b.f[q], s->b.fo[q]);
@@ -725,8 +739,8 @@ extern "C" pw_status pw_run(pw_solver* s, int nsteps)
             for (int ax = 0; ax < 3; ++ax)
                 for (int q = 0; q < kMemPerAxis; ++q) std::swap(s->b.mem[ax][q], s->b.memo[ax][q]);
         } else if (s->b.slab <= 0) {
-            if ((ks = step_kernel(0, 1, nz - 1)) != PW_OK) return ks;
-            if ((ks = step_kernel(1, 1, nz - 1)) != PW_OK) return ks;
+            if ((ks = step_kernel(0, halo, nz - halo)) != PW_OK) return ks;
+            if ((ks = step_kernel(1, halo, nz - halo)) != PW_OK) return ks;
         } else {
             // z-slab interleave: velocity on [zv, ve), then stress on the planes
             // whose inputs are complete.  Stress at plane z needs the new velocity at

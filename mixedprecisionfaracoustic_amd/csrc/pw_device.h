@@ -55,6 +55,7 @@ struct DevBuffers {
     int zc;          // z cells per block in the step kernels
     int ty;          // rows per tile in the step kernels (block 64 x ty)
     int vx;          // cells per lane along x in the step kernels (1, 2, 4)
+    int order;       // spatial order of the staggered differences: 2 or 4
     int xcd_map;     // 1: XCD-aware block -> tile mapping in the step kernels
     int slab;        // >0: interleave velocity/stress launches over z-slabs of this many planes
     int fused;       // 1: one fused launch per step over double-buffered state

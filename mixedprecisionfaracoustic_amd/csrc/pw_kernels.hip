@@ -257,13 +257,14 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     a.zc = b.zc;
     a.gx = (b.g.nx + 63) / 64;
     a.ty = b.ty > 0 ? b.ty : 4;
-    a.vx = b.vx > 0 ? b.vx : 1;
+    a.vx = (b.vx > 0 && b.order != 4) ? b.vx : 1;  // 4th order: one cell per lane
     a.zero = 0u;
     a.gx = (b.g.nx + 64 * a.vx - 1) / (64 * a.vx);
     a.gy = (b.g.ny + a.ty - 1) / a.ty;
-    a.zb = 1;
-    a.ze = b.g.nz - 1;
-    a.nblocks = a.gx * a.gy * ((b.g.nz - 2 + b.zc - 1) / b.zc);
+    const int halo = b.order == 4 ? 2 : 1;  // the outermost layers that are never updated
+    a.zb = halo;
+    a.ze = b.g.nz - halo;
+    a.nblocks = a.gx * a.gy * ((a.ze - a.zb + b.zc - 1) / b.zc);
     a.xcd_map = b.xcd_map;
     a.dt = (R)b.dt;
     a.h = (R)b.h;
@@ -419,13 +420,15 @@ template <int N> __device__ __forceinline__ Lanes make_lanes(int xb, int pitch)
 // --------------------------------------------------------------------------
 // Velocity kernel: Float/main.cu:64-192 (physical), Half/main.cu:156-268 (scaled)
 // --------------------------------------------------------------------------
-template <class P, int N>
+template <class P, int N, int ORD>
 __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile tl)
 {
     using R = typename P::R;
+    static_assert(ORD == 2 || (ORD == 4 && N == 1), "4th order runs one cell per lane");
+    constexpr int HALO = ORD / 2;  // the outermost HALO layers are never updated
     const int xb = tl.bx * 64 * N;
     const int iy = tl.by * a.ty + threadIdx.y;
-    if (iy < 1 || iy > a.ny - 2 || xb > a.nx - 2) return;  // the whole wave (one row)
+    if (iy < HALO || iy > a.ny - 1 - HALO || xb > a.nx - 1 - HALO) return;  // the whole wave (one row)
     const int z0 = a.zb + tl.bz * a.zc;
     const int z1 = min(z0 + a.zc, a.ze);
     if (z0 >= z1) return;
@@ -434,7 +437,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
     bool upd[N], all = true;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        upd[j] = L.lv && ix0 + j >= 1 && ix0 + j <= a.nx - 2;
+        upd[j] = L.lv && ix0 + j >= HALO && ix0 + j <= a.nx - 1 - HALO;
         all = all && upd[j];
     }
     const unsigned PL = a.plane, W = (unsigned)a.pitch;
@@ -463,11 +466,24 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
 #pragma unroll
     for (int j = 0; j < N; ++j) xs[j] = upd[j] ? slab_index(ix0 + j, a.nx, a.lo[0], a.hi[0]) : -1;
 
+    // z carries.  Order 2: txz, tyz at z-1; tzz, p at z.  Order 4 also txz, tyz at
+    // z-2 and z; tzz, p at z-1 and z+1.
     R txz_m[N], tyz_m[N], tzz_c[N], p_c[N];
+    R txz_mm[N], tyz_mm[N], txz_k[N], tyz_k[N], tzz_m[N], p_m[N], tzz_k[N], p_k[N];
     LD(PW_TXZ, o - PL, txz_m);
     LD(PW_TYZ, o - PL, tyz_m);
     LD(PW_TZZ, o, tzz_c);
     LD(PW_P, o, p_c);
+    if constexpr (ORD == 4) {
+        LD(PW_TXZ, o - 2 * PL, txz_mm);
+        LD(PW_TYZ, o - 2 * PL, tyz_mm);
+        LD(PW_TXZ, o, txz_k);
+        LD(PW_TYZ, o, tyz_k);
+        LD(PW_TZZ, o - PL, tzz_m);
+        LD(PW_P, o - PL, p_m);
+        LD(PW_TZZ, o + PL, tzz_k);
+        LD(PW_P, o + PL, p_k);
+    }
     // physical form: cell-centred rho, C1, C2 of this plane, carried up the column
     R rho_c[N], c1_c[N], c2_c[N];
     if constexpr (!P::scaled) {
@@ -475,25 +491,68 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         LDC(PC_C1, o, c1_c);
         LDC(PC_C2, o, c2_c);
     }
+    const R C4A = (R)1.125, C4B = (R)(-1.0 / 24.0);
 
     for (int iz = z0; iz < z1; ++iz, o += PL) {
-        R txz_c[N], tyz_c[N], tzz_n[N], p_n[N], txx_c[N], txx_r[N], tyy_c[N], tyy_q[N];
-        R txy_c[N], txy_l[N], txy_h[N], txz_l[N], tyz_h[N], p_r[N], p_q[N];
-        LD(PW_TXZ, o, txz_c);
-        LD(PW_TYZ, o, tyz_c);
-        LD(PW_TZZ, o + PL, tzz_n);
-        LD(PW_P, o + PL, p_n);
-        LD(PW_TXX, o, txx_c);
-        seg_right(L, txx_c, txx_r, [&] { return P::ld(a.f[PW_TXX], o + N); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
-        LD(PW_TYY, o, tyy_c);
-        LD(PW_TYY, o + W, tyy_q);
-        LD(PW_TXY, o, txy_c);
-        seg_left(L, txy_c, txy_l, [&] { return P::ld(a.f[PW_TXY], o - 1); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
-        LD(PW_TXY, o - W, txy_h);
-        seg_left(L, txz_c, txz_l, [&] { return P::ld(a.f[PW_TXZ], o - 1); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
-        LD(PW_TYZ, o - W, tyz_h);
-        seg_right(L, p_c, p_r, [&] { return P::ld(a.f[PW_P], o + N); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
-        LD(PW_P, o + W, p_q);
+        // undivided staggered differences, Float/main.cu:115-128 (order 2) or the
+        // 4th-order pair 9/8 (f1 - f0) - 1/24 (f2 - f_-1)
+        R dx1[N], dx2[N], dx3[N], ds1[N], dy1[N], dy2[N], dy3[N], ds2[N], dz1[N], dz2[N], dz3[N], ds3[N];
+        R txz_c[N], tyz_c[N], tzz_n[N], p_n[N];
+        if constexpr (ORD == 2) {
+            R txx_c[N], txx_r[N], tyy_c[N], tyy_q[N], txy_c[N], txy_l[N], txy_h[N], txz_l[N], tyz_h[N], p_r[N],
+                p_q[N];
+            LD(PW_TXZ, o, txz_c);
+            LD(PW_TYZ, o, tyz_c);
+            LD(PW_TZZ, o + PL, tzz_n);
+            LD(PW_P, o + PL, p_n);
+            LD(PW_TXX, o, txx_c);
+            seg_right(L, txx_c, txx_r, [&] { return P::ld(a.f[PW_TXX], o + N); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
+            LD(PW_TYY, o, tyy_c);
+            LD(PW_TYY, o + W, tyy_q);
+            LD(PW_TXY, o, txy_c);
+            seg_left(L, txy_c, txy_l, [&] { return P::ld(a.f[PW_TXY], o - 1); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
+            LD(PW_TXY, o - W, txy_h);
+            seg_left(L, txz_c, txz_l, [&] { return P::ld(a.f[PW_TXZ], o - 1); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
+            LD(PW_TYZ, o - W, tyz_h);
+            seg_right(L, p_c, p_r, [&] { return P::ld(a.f[PW_P], o + N); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
+            LD(PW_P, o + W, p_q);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                dx1[j] = txx_r[j] - txx_c[j]; dx2[j] = txy_c[j] - txy_h[j]; dx3[j] = txz_c[j] - txz_m[j];
+                ds1[j] = p_r[j] - p_c[j];
+                dy1[j] = tyy_q[j] - tyy_c[j]; dy2[j] = txy_c[j] - txy_l[j]; dy3[j] = tyz_c[j] - tyz_m[j];
+                ds2[j] = p_q[j] - p_c[j];
+                dz1[j] = tzz_n[j] - tzz_c[j]; dz2[j] = txz_c[j] - txz_l[j]; dz3[j] = tyz_c[j] - tyz_h[j];
+                ds3[j] = p_n[j] - p_c[j];
+            }
+        } else {
+            // N == 1: neighbours two cells out along x and y from L1/L2, along z from the carries
+            auto F = [&](int fid, long d) { return P::ld(a.f[fid], (unsigned)((long)o + d)); };
+            const long X = 1, Y = (long)W;
+            R txz_p, tyz_p, tzz_nn, p_nn;
+            txz_p = F(PW_TXZ, (long)PL);
+            tyz_p = F(PW_TYZ, (long)PL);
+            tzz_nn = F(PW_TZZ, 2 * (long)PL);
+            p_nn = F(PW_P, 2 * (long)PL);
+            const R txx0 = F(PW_TXX, 0), txy0 = F(PW_TXY, 0), tyy0 = F(PW_TYY, 0);
+            dx1[0] = C4A * (F(PW_TXX, X) - txx0) + C4B * (F(PW_TXX, 2 * X) - F(PW_TXX, -X));
+            dx2[0] = C4A * (txy0 - F(PW_TXY, -Y)) + C4B * (F(PW_TXY, Y) - F(PW_TXY, -2 * Y));
+            dx3[0] = C4A * (txz_k[0] - txz_m[0]) + C4B * (txz_p - txz_mm[0]);
+            ds1[0] = C4A * (F(PW_P, X) - p_c[0]) + C4B * (F(PW_P, 2 * X) - F(PW_P, -X));
+            dy1[0] = C4A * (F(PW_TYY, Y) - tyy0) + C4B * (F(PW_TYY, 2 * Y) - F(PW_TYY, -Y));
+            dy2[0] = C4A * (txy0 - F(PW_TXY, -X)) + C4B * (F(PW_TXY, X) - F(PW_TXY, -2 * X));
+            dy3[0] = C4A * (tyz_k[0] - tyz_m[0]) + C4B * (tyz_p - tyz_mm[0]);
+            ds2[0] = C4A * (F(PW_P, Y) - p_c[0]) + C4B * (F(PW_P, 2 * Y) - F(PW_P, -Y));
+            dz1[0] = C4A * (tzz_k[0] - tzz_c[0]) + C4B * (tzz_nn - tzz_m[0]);
+            dz2[0] = C4A * (txz_k[0] - F(PW_TXZ, -X)) + C4B * (F(PW_TXZ, X) - F(PW_TXZ, -2 * X));
+            dz3[0] = C4A * (tyz_k[0] - F(PW_TYZ, -Y)) + C4B * (F(PW_TYZ, Y) - F(PW_TYZ, -2 * Y));
+            ds3[0] = C4A * (p_k[0] - p_c[0]) + C4B * (p_nn - p_m[0]);
+            // shift the z carries to the next plane
+            txz_mm[0] = txz_m[0]; txz_m[0] = txz_k[0]; txz_k[0] = txz_p;
+            tyz_mm[0] = tyz_m[0]; tyz_m[0] = tyz_k[0]; tyz_k[0] = tyz_p;
+            tzz_m[0] = tzz_c[0]; tzz_c[0] = tzz_k[0]; tzz_k[0] = tzz_nn;
+            p_m[0] = p_c[0]; p_c[0] = p_k[0]; p_k[0] = p_nn;
+        }
         const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
         Tab4<R> tz = {0, 1, 0, 1};
         if (zs >= 0) tz = a.tab[2][iz];
@@ -502,10 +561,10 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
 #pragma unroll
         for (int j = 0; j < N; ++j) {
             // Float/main.cu:115-128
-            R x1 = D(txx_r[j] - txx_c[j]), x2 = D(txy_c[j] - txy_h[j]), x3 = D(txz_c[j] - txz_m[j]);
-            R y1 = D(tyy_q[j] - tyy_c[j]), y2 = D(txy_c[j] - txy_l[j]), y3 = D(tyz_c[j] - tyz_m[j]);
-            R z1_ = D(tzz_n[j] - tzz_c[j]), z2 = D(txz_c[j] - txz_l[j]), z3 = D(tyz_c[j] - tyz_h[j]);
-            R q1 = D(p_r[j] - p_c[j]), q2 = D(p_q[j] - p_c[j]), q3 = D(p_n[j] - p_c[j]);
+            R x1 = D(dx1[j]), x2 = D(dx2[j]), x3 = D(dx3[j]);
+            R y1 = D(dy1[j]), y2 = D(dy2[j]), y3 = D(dy3[j]);
+            R z1_ = D(dz1[j]), z2 = D(dz2[j]), z3 = D(dz3[j]);
+            R q1 = D(ds1[j]), q2 = D(ds2[j]), q3 = D(ds3[j]);
             const unsigned ix = (unsigned)(ix0 + j);
             if (xs[j] >= 0) {
                 const Tab4<R> tx = a.tab[0][ix];
@@ -606,12 +665,14 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         ST(PW_VWX, o, wx1); ST(PW_VWY, o, wy1); ST(PW_VWZ, o, wz1);
         ST(PW_VUX, o, ux1); ST(PW_VUY, o, uy1); ST(PW_VUZ, o, uz1);
 
+        if constexpr (ORD == 2) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            txz_m[j] = txz_c[j];
-            tyz_m[j] = tyz_c[j];
-            tzz_c[j] = tzz_n[j];
-            p_c[j] = p_n[j];
+            for (int j = 0; j < N; ++j) {
+                txz_m[j] = txz_c[j];
+                tyz_m[j] = tyz_c[j];
+                tzz_c[j] = tzz_n[j];
+                p_c[j] = p_n[j];
+            }
         }
     }
 }
@@ -619,13 +680,15 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
 // --------------------------------------------------------------------------
 // Stress kernel: Float/main.cu:194-295 (physical), Half/main.cu:270-371 (scaled)
 // --------------------------------------------------------------------------
-template <class P, int N>
+template <class P, int N, int ORD>
 __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
 {
     using R = typename P::R;
+    static_assert(ORD == 2 || (ORD == 4 && N == 1), "4th order runs one cell per lane");
+    constexpr int HALO = ORD / 2;
     const int xb = tl.bx * 64 * N;
     const int iy = tl.by * a.ty + threadIdx.y;
-    if (iy < 1 || iy > a.ny - 2 || xb > a.nx - 2) return;  // the whole wave (one row)
+    if (iy < HALO || iy > a.ny - 1 - HALO || xb > a.nx - 1 - HALO) return;  // the whole wave (one row)
     const int z0 = a.zb + tl.bz * a.zc;
     const int z1 = min(z0 + a.zc, a.ze);
     if (z0 >= z1) return;
@@ -634,7 +697,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     bool upd[N], all = true;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        upd[j] = L.lv && ix0 + j >= 1 && ix0 + j <= a.nx - 2;
+        upd[j] = L.lv && ix0 + j >= HALO && ix0 + j <= a.nx - 1 - HALO;
         all = all && upd[j];
     }
     const unsigned PL = a.plane, W = (unsigned)a.pitch;
@@ -663,11 +726,24 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
 #pragma unroll
     for (int j = 0; j < N; ++j) xs[j] = upd[j] ? slab_index(ix0 + j, a.nx, a.lo[0], a.hi[0]) : -1;
 
+    // z carries.  Order 2: vuz, vwz at z-1; vux, vuy at z.  Order 4 also vuz, vwz
+    // at z-2 and z; vux, vuy at z-1 and z+1.
     R vuz_m[N], vwz_m[N], vux_c[N], vuy_c[N];
+    R vuz_mm[N], vwz_mm[N], vuz_k[N], vwz_k[N], vux_m[N], vuy_m[N], vux_k[N], vuy_k[N];
     LD(PW_VUZ, o - PL, vuz_m);
     LD(PW_VWZ, o - PL, vwz_m);
     LD(PW_VUX, o, vux_c);
     LD(PW_VUY, o, vuy_c);
+    if constexpr (ORD == 4) {
+        LD(PW_VUZ, o - 2 * PL, vuz_mm);
+        LD(PW_VWZ, o - 2 * PL, vwz_mm);
+        LD(PW_VUZ, o, vuz_k);
+        LD(PW_VWZ, o, vwz_k);
+        LD(PW_VUX, o - PL, vux_m);
+        LD(PW_VUY, o - PL, vuy_m);
+        LD(PW_VUX, o + PL, vux_k);
+        LD(PW_VUY, o + PL, vuy_k);
+    }
     // physical form: cell-centred mu at (x, y), (x+1, y), (x, y+1) of this plane
     R mu_c[N], mu_x[N], mu_y[N];
     if constexpr (!P::scaled) {
@@ -675,24 +751,60 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         seg_right(L, mu_c, mu_x, [&] { return P::ldc(a.c[PC_MU], o + N); }, [&](R (&r)[N]) { LDC(PC_MU, o + 1, r); });
         LDC(PC_MU, o + W, mu_y);
     }
+    const R C4A = (R)1.125, C4B = (R)(-1.0 / 24.0);
 
     for (int iz = z0; iz < z1; ++iz, o += PL) {
-        R vux_n[N], vuy_n[N], vuz_c[N], vwz_c[N], vux_l[N], vux_q[N], vuy_h[N], vuy_r[N];
-        R vuz_r[N], vuz_q[N], vwx_c[N], vwx_l[N], vwy_c[N], vwy_h[N];
-        LD(PW_VUX, o + PL, vux_n);
-        LD(PW_VUY, o + PL, vuy_n);
-        LD(PW_VUZ, o, vuz_c);
-        LD(PW_VWZ, o, vwz_c);
-        seg_left(L, vux_c, vux_l, [&] { return P::ld(a.f[PW_VUX], o - 1); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
-        LD(PW_VUX, o + W, vux_q);
-        LD(PW_VUY, o - W, vuy_h);
-        seg_right(L, vuy_c, vuy_r, [&] { return P::ld(a.f[PW_VUY], o + N); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
-        seg_right(L, vuz_c, vuz_r, [&] { return P::ld(a.f[PW_VUZ], o + N); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
-        LD(PW_VUZ, o + W, vuz_q);
-        LD(PW_VWX, o, vwx_c);
-        seg_left(L, vwx_c, vwx_l, [&] { return P::ld(a.f[PW_VWX], o - 1); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
-        LD(PW_VWY, o, vwy_c);
-        LD(PW_VWY, o - W, vwy_h);
+        // undivided staggered differences, Float/main.cu:234-249 (order 2) or the
+        // 4th-order pair
+        R dxx[N], dyy[N], dzz[N], ewx[N], ewy[N], ewz[N], dxy[N], dyx[N], dxz[N], dzx[N], dyz[N], dzy[N];
+        R vux_n[N], vuy_n[N], vuz_c[N], vwz_c[N];
+        if constexpr (ORD == 2) {
+            R vux_l[N], vux_q[N], vuy_h[N], vuy_r[N], vuz_r[N], vuz_q[N], vwx_c[N], vwx_l[N], vwy_c[N], vwy_h[N];
+            LD(PW_VUX, o + PL, vux_n);
+            LD(PW_VUY, o + PL, vuy_n);
+            LD(PW_VUZ, o, vuz_c);
+            LD(PW_VWZ, o, vwz_c);
+            seg_left(L, vux_c, vux_l, [&] { return P::ld(a.f[PW_VUX], o - 1); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
+            LD(PW_VUX, o + W, vux_q);
+            LD(PW_VUY, o - W, vuy_h);
+            seg_right(L, vuy_c, vuy_r, [&] { return P::ld(a.f[PW_VUY], o + N); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
+            seg_right(L, vuz_c, vuz_r, [&] { return P::ld(a.f[PW_VUZ], o + N); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
+            LD(PW_VUZ, o + W, vuz_q);
+            LD(PW_VWX, o, vwx_c);
+            seg_left(L, vwx_c, vwx_l, [&] { return P::ld(a.f[PW_VWX], o - 1); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
+            LD(PW_VWY, o, vwy_c);
+            LD(PW_VWY, o - W, vwy_h);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                dxx[j] = vux_c[j] - vux_l[j]; dyy[j] = vuy_c[j] - vuy_h[j]; dzz[j] = vuz_c[j] - vuz_m[j];
+                ewx[j] = vwx_c[j] - vwx_l[j]; ewy[j] = vwy_c[j] - vwy_h[j]; ewz[j] = vwz_c[j] - vwz_m[j];
+                dxy[j] = vux_q[j] - vux_c[j]; dyx[j] = vuy_r[j] - vuy_c[j];
+                dxz[j] = vux_n[j] - vux_c[j]; dzx[j] = vuz_r[j] - vuz_c[j];
+                dyz[j] = vuy_n[j] - vuy_c[j]; dzy[j] = vuz_q[j] - vuz_c[j];
+            }
+        } else {
+            auto F = [&](int fid, long d) { return P::ld(a.f[fid], (unsigned)((long)o + d)); };
+            const long X = 1, Y = (long)W;
+            const R vuz_p = F(PW_VUZ, (long)PL), vwz_p = F(PW_VWZ, (long)PL);
+            const R vux_nn = F(PW_VUX, 2 * (long)PL), vuy_nn = F(PW_VUY, 2 * (long)PL);
+            const R vwx0 = F(PW_VWX, 0), vwy0 = F(PW_VWY, 0);
+            dxx[0] = C4A * (vux_c[0] - F(PW_VUX, -X)) + C4B * (F(PW_VUX, X) - F(PW_VUX, -2 * X));
+            dyy[0] = C4A * (vuy_c[0] - F(PW_VUY, -Y)) + C4B * (F(PW_VUY, Y) - F(PW_VUY, -2 * Y));
+            dzz[0] = C4A * (vuz_k[0] - vuz_m[0]) + C4B * (vuz_p - vuz_mm[0]);
+            ewx[0] = C4A * (vwx0 - F(PW_VWX, -X)) + C4B * (F(PW_VWX, X) - F(PW_VWX, -2 * X));
+            ewy[0] = C4A * (vwy0 - F(PW_VWY, -Y)) + C4B * (F(PW_VWY, Y) - F(PW_VWY, -2 * Y));
+            ewz[0] = C4A * (vwz_k[0] - vwz_m[0]) + C4B * (vwz_p - vwz_mm[0]);
+            dxy[0] = C4A * (F(PW_VUX, Y) - vux_c[0]) + C4B * (F(PW_VUX, 2 * Y) - F(PW_VUX, -Y));
+            dyx[0] = C4A * (F(PW_VUY, X) - vuy_c[0]) + C4B * (F(PW_VUY, 2 * X) - F(PW_VUY, -X));
+            dxz[0] = C4A * (vux_k[0] - vux_c[0]) + C4B * (vux_nn - vux_m[0]);
+            dzx[0] = C4A * (F(PW_VUZ, X) - vuz_k[0]) + C4B * (F(PW_VUZ, 2 * X) - F(PW_VUZ, -X));
+            dyz[0] = C4A * (vuy_k[0] - vuy_c[0]) + C4B * (vuy_nn - vuy_m[0]);
+            dzy[0] = C4A * (F(PW_VUZ, Y) - vuz_k[0]) + C4B * (F(PW_VUZ, 2 * Y) - F(PW_VUZ, -Y));
+            vuz_mm[0] = vuz_m[0]; vuz_m[0] = vuz_k[0]; vuz_k[0] = vuz_p;
+            vwz_mm[0] = vwz_m[0]; vwz_m[0] = vwz_k[0]; vwz_k[0] = vwz_p;
+            vux_m[0] = vux_c[0]; vux_c[0] = vux_k[0]; vux_k[0] = vux_nn;
+            vuy_m[0] = vuy_c[0]; vuy_c[0] = vuy_k[0]; vuy_k[0] = vuy_nn;
+        }
         const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
         Tab4<R> tz = {0, 1, 0, 1};
         if (zs >= 0) tz = a.tab[2][iz];
@@ -721,11 +833,11 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
 #pragma unroll
         for (int j = 0; j < N; ++j) {
             // Float/main.cu:234-249
-            R uxx = D(vux_c[j] - vux_l[j]), uyy = D(vuy_c[j] - vuy_h[j]), uzz = D(vuz_c[j] - vuz_m[j]);
-            R wx = D(vwx_c[j] - vwx_l[j]), wy = D(vwy_c[j] - vwy_h[j]), wz = D(vwz_c[j] - vwz_m[j]);
-            R uxy = D(vux_q[j] - vux_c[j]), uyx = D(vuy_r[j] - vuy_c[j]);
-            R uxz = D(vux_n[j] - vux_c[j]), uzx = D(vuz_r[j] - vuz_c[j]);
-            R uyz = D(vuy_n[j] - vuy_c[j]), uzy = D(vuz_q[j] - vuz_c[j]);
+            R uxx = D(dxx[j]), uyy = D(dyy[j]), uzz = D(dzz[j]);
+            R wx = D(ewx[j]), wy = D(ewy[j]), wz = D(ewz[j]);
+            R uxy = D(dxy[j]), uyx = D(dyx[j]);
+            R uxz = D(dxz[j]), uzx = D(dzx[j]);
+            R uyz = D(dyz[j]), uzy = D(dzy[j]);
             const unsigned ix = (unsigned)(ix0 + j);
             if (xs[j] >= 0) {
                 const Tab4<R> tx = a.tab[0][ix];
@@ -783,23 +895,25 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         ST(PW_TXX, o, xx1); ST(PW_TYY, o, yy1); ST(PW_TZZ, o, zz1);
         ST(PW_TXY, o, xy1); ST(PW_TYZ, o, yz1); ST(PW_TXZ, o, xz1);
 
+        if constexpr (ORD == 2) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            vuz_m[j] = vuz_c[j];
-            vwz_m[j] = vwz_c[j];
-            vux_c[j] = vux_n[j];
-            vuy_c[j] = vuy_n[j];
+            for (int j = 0; j < N; ++j) {
+                vuz_m[j] = vuz_c[j];
+                vwz_m[j] = vwz_c[j];
+                vux_c[j] = vux_n[j];
+                vuy_c[j] = vuy_n[j];
+            }
         }
     }
 }
 
-template <class P, int N> __global__ __launch_bounds__(512) void k_velocity(const StepArgs<P> a)
+template <class P, int N, int ORD> __global__ __launch_bounds__(512) void k_velocity(const StepArgs<P> a)
 {
-    velocity_tile<P, N>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
+    velocity_tile<P, N, ORD>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
 }
-template <class P, int N> __global__ __launch_bounds__(512) void k_stress(const StepArgs<P> a)
+template <class P, int N, int ORD> __global__ __launch_bounds__(512) void k_stress(const StepArgs<P> a)
 {
-    stress_tile<P, N>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
+    stress_tile<P, N, ORD>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
 }
 
 // --------------------------------------------------------------------------
@@ -1260,7 +1374,7 @@ template <class P, int N> __global__ __launch_bounds__(512) void k_velocity_batc
     const ShotDesc<P>* d = (const ShotDesc<P>*)c.descs + s;
     if (c.it[s] + k >= d->nt) return;
     const StepArgs<P> a = d->a;
-    velocity_tile<P, N>(a, tile_of_index(t, a.gx, a.gy));
+    velocity_tile<P, N, 2>(a, tile_of_index(t, a.gx, a.gy));
 }
 
 template <class P, int N> __global__ __launch_bounds__(512) void k_stress_batch(const BatchArgs c, int k)
@@ -1271,7 +1385,7 @@ template <class P, int N> __global__ __launch_bounds__(512) void k_stress_batch(
     const ShotDesc<P>* d = (const ShotDesc<P>*)c.descs + s;
     if (c.it[s] + k >= d->nt) return;
     const StepArgs<P> a = d->a;
-    stress_tile<P, N>(a, tile_of_index(t, a.gx, a.gy));
+    stress_tile<P, N, 2>(a, tile_of_index(t, a.gx, a.gy));
 }
 
 // one thread per (shot, source channel); the arithmetic of k_source
@@ -1387,8 +1501,12 @@ hipError_t launch_velocity(const DevBuffers& b, hipStream_t st, int zb, int ze)
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
         const StepArgs<P> a = range_args<P>(b, zb, ze);
+        if (b.order == 4) {
+            k_velocity<P, 1, 4><<<a.nblocks, dim3(64, a.ty), 0, st>>>(a);
+            return hipGetLastError();
+        }
         return dispatch_vx(a.vx, [&](auto n) {
-            k_velocity<P, decltype(n)::value><<<a.nblocks, dim3(64, a.ty), 0, st>>>(a);
+            k_velocity<P, decltype(n)::value, 2><<<a.nblocks, dim3(64, a.ty), 0, st>>>(a);
             return hipGetLastError();
         });
     });
@@ -1400,8 +1518,12 @@ hipError_t launch_stress(const DevBuffers& b, hipStream_t st, int zb, int ze)
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
         const StepArgs<P> a = range_args<P>(b, zb, ze);
+        if (b.order == 4) {
+            k_stress<P, 1, 4><<<a.nblocks, dim3(64, a.ty), 0, st>>>(a);
+            return hipGetLastError();
+        }
         return dispatch_vx(a.vx, [&](auto n) {
-            k_stress<P, decltype(n)::value><<<a.nblocks, dim3(64, a.ty), 0, st>>>(a);
+            k_stress<P, decltype(n)::value, 2><<<a.nblocks, dim3(64, a.ty), 0, st>>>(a);
             return hipGetLastError();
         });
     });

@@ -32,6 +32,7 @@ struct pw_solver {
     double f0 = 0.0, w[4] = {1, 1, 1, 0};
     // receivers
     int nrec = 0;
+    bool ref_receivers = false;  // set by pw_set_receivers_reference: the five reference channels
     int* d_rfield = nullptr;
     unsigned* d_roff = nullptr;
     float* d_seis = nullptr;

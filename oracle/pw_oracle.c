@@ -190,6 +190,7 @@ static int check_args(const or_args* a)
     if (!a || a->nx < 4 || a->ny < 4 || a->nz < 4 || a->np < 1) return 1;
     if (a->nx - 2 * a->np < 1 || a->ny - 2 * a->np < 1 || a->nz - 2 * a->np < 1) return 1;
     if (a->nt < 0 || a->h <= 0 || a->dt <= 0) return 1;
+    if (a->order != 0 && a->order != 2 && a->order != 4) return 1;
     if (a->sx < 0 || a->sx >= a->nx || a->sy < 0 || a->sy >= a->ny || a->sz < 0 || a->sz >= a->nz) return 1;
     if (a->nrec < 0 || (a->nrec > 0 && (!a->rec_field || !a->rec_index || !a->seis))) return 1;
     if (a->nsnap < 0 || (a->nsnap > 0 && (!a->snap_step || !a->snap_field || !a->snaps))) return 1;
@@ -350,8 +351,8 @@ int or_time_f32(or_args* a, const float* base, int warm, int steps, double* seco
         s.tyy[sn] += (float)a->src_w[1] * I;
         s.tzz[sn] += (float)a->src_w[2] * I;
         s.ss[sn] += (float)a->src_w[3] * I;
-        f32_fd_v(&s, a->nx, a->ny, a->nz, (float)a->h, (float)a->dt, a->pml_everywhere);
-        f32_fd_t(&s, a->nx, a->ny, a->nz, (float)a->h, (float)a->dt, a->pml_everywhere);
+        f32_fd_v(&s, a->nx, a->ny, a->nz, (float)a->h, (float)a->dt, a->pml_everywhere, a->order);
+        f32_fd_t(&s, a->nx, a->ny, a->nz, (float)a->h, (float)a->dt, a->pml_everywhere, a->order);
     }
 #ifdef _OPENMP
     t1 = omp_get_wtime();
@@ -930,7 +931,7 @@ int or_run_half(or_args* a, const float* base)
     half_state h;
     int em, epml, k, it, r;
     float source_scale, stress_base, velocity_base, dynamic_scale, stress_out, velocity_out;
-    if (check_args(a) || !base) return 1;
+    if (check_args(a) || !base || a->order == 4) return 1;  /* the Half program is 2nd order */
     set_threads(a);
     if (half_setup(a, base, &h, &em, &epml)) { half_free(&h); return 2; }
     a->half_em_out = em;
@@ -1008,4 +1009,10 @@ int or_run_half(or_args* a, const float* base)
     }
     half_free(&h);
     return 0;
+}
+
+/* The staggered difference of the update (order 2 or 4) on a 1-D array, FP64. */
+double or_stagger_diff_f64(int order, int forward, const double* f, long o, long st)
+{
+    return f64_stagger_diff(order, forward, f, o, st);
 }

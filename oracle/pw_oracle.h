@@ -68,6 +68,8 @@ typedef struct {
     double half_dyn_end;     /* DYNAMIC_SCALE_END */
     int half_dyn_interval;   /* DYNAMIC_SCALE_INTERVAL */
     int half_em_out, half_epml_out;  /* out: chosen exponents */
+    int order;               /* spatial order of the staggered differences: 2 (the
+                                reference; 0 means 2) or 4 (FP32/FP64 runs only) */
 } or_args;
 
 /* create_model (Float/main.cu:1590-1655): the borehole-in-formation model.
@@ -123,4 +125,8 @@ int or_half_precompute(const or_args* a, const float* base, int* em, int* epml,
 #ifdef __cplusplus
 }
 #endif
+/* The staggered difference the update applies (order 2 or 4), FP64, on a 1-D
+ * array: forward (point o + st/2) or backward (point o - st/2), undivided. */
+double or_stagger_diff_f64(int order, int forward, const double* f, long o, long st);
+
 #endif

@@ -31,6 +31,7 @@ class OrArgs(C.Structure):
         ("pml_everywhere", C.c_int), ("nthreads", C.c_int),
         ("half_es", C.c_int), ("half_dyn_end", C.c_double), ("half_dyn_interval", C.c_int),
         ("half_em_out", C.c_int), ("half_epml_out", C.c_int),
+        ("order", C.c_int),
     ]
 
 
@@ -184,6 +185,7 @@ def _args(grid, nt, receivers, snapshots, snap_y, final, pml_everywhere=False, n
     a.half_es = int(half.get("es", 0))
     a.half_dyn_end = float(half.get("dyn_end", 16.0))
     a.half_dyn_interval = int(half.get("dyn_interval", 100))
+    a.order = int(grid.get("order", 2))
     return a, keep
 
 
@@ -238,3 +240,15 @@ def reference_grid():
     n = 128 + 2 * 32
     return {"nx": n, "ny": n, "nz": n, "np": 32, "h": 0.01, "dt": 0.9e-6, "rc": 1e-6, "v0": 4000.0,
             "f0": 3000.0, "src": (n // 2, n // 2, 20 + 32), "src_w": (1.0, 1.0, 1.0, 0.0)}
+
+
+def stagger_diff(order, forward, f, o, stride=1):
+    """The oracle's staggered difference (the update's own macro) of the FP64
+    array f at index o: forward (to o + stride/2) or backward, undivided."""
+    L = lib()
+    L.or_stagger_diff_f64.restype = C.c_double
+    L.or_stagger_diff_f64.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.c_long, C.c_long]
+    a = np.ascontiguousarray(f, np.float64)
+    return L.or_stagger_diff_f64(int(order), 1 if forward else 0, a.ctypes.data_as(C.POINTER(C.c_double)), int(o),
+                                 int(stride))
+

@@ -86,7 +86,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     import subprocess
     structs = {"pw_config": pw.Config, "pw_info": pw.Info, "pw_rock": pw.Rock, "pw_layered": pw.Layered,
                "pw_server_config": pw.ServerConfig, "pw_shot": pw.ShotSpec, "pw_shot_status": pw.ShotStatus,
-               "pw_server_stats": pw.ServerStats}
+               "pw_server_stats": pw.ServerStats, "pw_bin_header": pw.BinHeader}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "porowave.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
@@ -113,3 +113,40 @@ def test_server_needs_a_device():
     """Without a GPU the server refuses to start (no CPU fallback)."""
     with pytest.raises(pw.PorowaveError):
         pw.Server()
+
+
+def test_config_validation_errors(monkeypatch):
+    """Bad configurations are refused with their reason before any device call."""
+    with pytest.raises(pw.PorowaveError) as ei:
+        pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, space_order=3)
+    assert ei.value.status == 1 and "space_order" in str(ei.value)
+    with pytest.raises(pw.PorowaveError) as ei:
+        pw.Solver(16, 40, 40, 6, 5.0, 5e-4, 10, space_order=4)  # 4th order needs 2*np+5 cells
+    assert ei.value.status == 1
+    monkeypatch.setenv("PW_FUSED", "1")
+    with pytest.raises(pw.PorowaveError) as ei:
+        pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, space_order=4)
+    assert ei.value.status == 6 and "fused" in str(ei.value)  # PW_ERR_UNSUPPORTED
+
+
+def test_binary_files_round_trip(tmp_path):
+    """pw_write_bin / pw_read_bin: an 80-byte header then the values, bit-exact."""
+    import ctypes as C
+    assert C.sizeof(pw.BinHeader) == 80
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((7, 33)).astype(np.float32)
+    a[0, 0] = np.float32(1e-40)  # a subnormal survives
+    p = tmp_path / "sisx.bin"
+    pw.write_bin(str(p), a, kind=0, field=6, dt=0.5e-3, h=5.0, label="sisx")
+    raw = p.read_bytes()
+    assert len(raw) == 80 + a.size * 4 and raw[:4] == b"PWB1"
+    assert np.array_equal(np.frombuffer(raw[80:], "<f4").reshape(a.shape), a)
+    head, b = pw.read_bin(str(p))
+    assert np.array_equal(a, b)
+    assert (head["kind"], head["nr"], head["nc"], head["field"], head["step"]) == (0, 7, 33, 6, -1)
+    assert head["dt"] == 0.5e-3 and head["h"] == 5.0 and head["label"] == "sisx" and head["version"] == 1
+    q = tmp_path / "junk.bin"
+    q.write_bytes(b"not a porowave file" * 10)
+    with pytest.raises(pw.PorowaveError) as ei:
+        pw.read_bin(str(q))
+    assert ei.value.status == 7  # PW_ERR_IO

@@ -126,3 +126,41 @@ def test_precompute_staggered_means():
     assert out[14, 7, 7, 7] == 0.0 and out[15, 6, 7, 7] == 0.0  # cells touching the zero
     # outer layer of staggered arrays is never computed (stays zero)
     assert np.all(out[0, :, :, 0] == 0) and np.all(out[14, -1] == 0)
+
+
+def test_staggered_differences_pin_the_fourth_order_stencil():
+    """The oracle's staggered difference (the macro its update expands, for both
+    orders): exact for polynomials up to degree 4 at order 4 (only up to degree 2
+    at order 2), second/fourth-order convergence on a smooth function, and the
+    forward/backward points o +- 1/2.  The call sites (which field, which stride,
+    which direction per term) are pinned at order 2 by the reference snapshots."""
+    import numpy as np
+    x = np.arange(40, dtype=np.float64)
+    for deg in (1, 2, 3, 4):
+        f = (x - 17.3) ** deg
+        for o in (10, 20):
+            for fwd, xm in ((True, o + 0.5), (False, o - 0.5)):
+                exact = deg * (xm - 17.3) ** (deg - 1)
+                d4 = O.stagger_diff(4, fwd, f, o)
+                d2 = O.stagger_diff(2, fwd, f, o)
+                assert abs(d4 - exact) <= 1e-9 * max(1.0, abs(exact)), (deg, fwd, d4, exact)
+                if deg <= 2:
+                    assert abs(d2 - exact) <= 1e-9 * max(1.0, abs(exact))
+                else:
+                    assert abs(d2 - exact) > 1e-3 * abs(exact)  # order 2 is not exact for cubics
+    # convergence: error ratio on halving h is 2^order
+    errs = {2: [], 4: []}
+    for h in (0.2, 0.1, 0.05):
+        xs = np.arange(0, 4, h)
+        f = np.sin(1.3 * xs)
+        o = len(xs) // 2
+        exact = 1.3 * np.cos(1.3 * (xs[o] + 0.5 * h))
+        for order in (2, 4):
+            errs[order].append(abs(O.stagger_diff(order, True, f, o) / h - exact))
+    for order in (2, 4):
+        r = errs[order][0] / errs[order][1], errs[order][1] / errs[order][2]
+        assert all(abs(v - 2 ** order) < 0.1 * 2 ** order for v in r), (order, r)
+    # strides: the same stencil along a strided axis
+    g = np.zeros(200)
+    g[::10] = (np.arange(20) - 7.1) ** 3
+    assert abs(O.stagger_diff(4, True, g, 80, 10) - 3 * (8.5 - 7.1) ** 2) < 1e-9
