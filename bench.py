@@ -203,7 +203,7 @@ def precision_report(args):
     fields, idx = receivers(n, np_)
     snap_step = min(1000, nt - 1)
     res = {}
-    for prec in ("fp64", "fp32", "comp16", "bf16"):
+    for prec in ("fp64", "fp32", "comp16", "comp16f", "bf16"):
         stream = torch.cuda.Stream()
         s = pw.Solver(n, n, n, np_, h, dt, nt, precision=prec, cpml_r=1e-4, cpml_v0=4000.0, stream=stream.cuda_stream)
         s.set_medium_layered(spec)
@@ -236,7 +236,7 @@ def precision_report(args):
            "config": {"workload": f"biot_two_layer_{n}^3", "steps": nt, "receivers": len(idx), "snapshot_step": snap_step},
            "data": "synthetic (two-layer perturbed Biot medium generated on the GPU, seed 0)", "precisions": {}}
     nr = len(idx) // len(chans)
-    for prec in ("fp32", "comp16", "bf16", "fp64"):
+    for prec in ("fp32", "comp16", "comp16f", "bf16", "fp64"):
         r = res[prec]
         e = {c: rel(r["seis"][k * nr:(k + 1) * nr], ref["seis"][k * nr:(k + 1) * nr]) for k, c in enumerate(chans)}
         out["precisions"][prec] = {"gcell_steps_per_s": round(r["gcell"], 3),
@@ -254,7 +254,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--np", type=int, default=20)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "comp16", "bf16", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "comp16", "comp16f", "bf16", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--prof-steps", type=int, default=40)
@@ -336,7 +336,7 @@ def main():
         kname, kbytes, kms = "k_stress", info["bytes_stress"] * cells, avg_t
     achieved = kbytes / (kms / 1e3) / 1e9
     dtype = {"fp32": "f32", "comp16": "f16+i8 storage, f32 math", "bf16": "bf16 storage, f32 math",
-             "fp64": "f64"}[args.precision]
+             "comp16f": "f16+i8 state, f32 coefficients and math", "fp64": "f64"}[args.precision]
     out = {
         "metric": METRIC,
         "value": round(gcell, 4),

@@ -52,7 +52,10 @@ typedef enum {
     PW_FP32 = 0,   /* Float/main.cu storage */
     PW_COMP16 = 1, /* Half/main.cu: FP16 value + int8 residual, FP16 coefficients */
     PW_BF16 = 2,   /* BF16 wavefields + memory variables, FP16 coefficients */
-    PW_FP64 = 3    /* FP64 oracle run */
+    PW_FP64 = 3,   /* FP64 oracle run */
+    PW_COMP16F = 4 /* the comp16 state (FP16 value + int8 residual; stresses scaled by 2^es,
+                      velocities by 2^(es-em) as Half/main.cu:793-807 chooses them) with the
+                      Float update: FP32 cell-centred coefficients, FP32 memory variables */
 } pw_precision;
 
 /* Wavefield ids for receivers, snapshots and pw_get_field. */

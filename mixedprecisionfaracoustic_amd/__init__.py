@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libporowave.so")
 FIELDS = ["vux", "vuy", "vuz", "vwx", "vwy", "vwz", "txx", "tyy", "tzz", "txy", "txz", "tyz", "p"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 BASE = ["rho", "rhof", "C1", "C2", "HH", "H2u", "C", "M", "mu"]
-PRECISIONS = {"fp32": 0, "comp16": 1, "bf16": 2, "fp64": 3}
+PRECISIONS = {"fp32": 0, "comp16": 1, "bf16": 2, "fp64": 3, "comp16f": 4}
 # The reference's seismogram channels (Float/main.cu:1085-1093, 1213-1222).
 REFERENCE_CHANNELS = [("sisx", "txx"), ("sisz", "tzz"), ("sisvu", "vux"), ("sisvw", "vwx"), ("sisp", "p")]
 STATUS = {0: "PW_OK", 1: "PW_ERR_INVALID", 2: "PW_ERR_HIP", 3: "PW_ERR_OUT_OF_MEMORY", 4: "PW_ERR_STATE",

@@ -62,7 +62,7 @@ void build_tables(const pw_config& c, int prec, int n, std::vector<double>& out4
     const float DT = (float)c.dt;
     cpml_profile_reference_f32(n, c.np, (float)c.h, DT, (float)c.cpml_r, (float)c.cpml_v0, d.data(), d2.data(),
                                e.data(), e2.data());
-    if (prec == PW_FP32) {
+    if (prec == PW_FP32 || prec == PW_COMP16F) {
         for (int i = 0; i < n; ++i) {
             out4[4 * i + 0] = (float)(DT * d[i]) * 0.5f;
             out4[4 * i + 1] = e[i];
@@ -137,7 +137,7 @@ namespace {
 
 pw_status check_config(const pw_config& c)
 {
-    if (c.precision < PW_FP32 || c.precision > PW_FP64) return fail(PW_ERR_INVALID, "pw_create: unknown precision");
+    if (c.precision < PW_FP32 || c.precision > PW_COMP16F) return fail(PW_ERR_INVALID, "pw_create: unknown precision");
     if (c.np < 1 || c.nx < 2 * c.np + 3 || c.ny < 2 * c.np + 3 || c.nz < 2 * c.np + 3)
         return fail(PW_ERR_INVALID, "pw_create: grid must be at least 2*np+3 cells per axis");
     if (!(c.h > 0) || !(c.dt > 0) || c.nt < 1) return fail(PW_ERR_INVALID, "pw_create: h, dt and nt must be positive");
@@ -207,7 +207,7 @@ void plan_layout(const pw_config& c, int fused, Layout& L)
         L.off.push_back(L.total);
         L.total += align256(bytes);
     };
-    const bool c16 = prec == PW_COMP16;
+    const bool c16 = has_lo(prec);
     const int nbuf = fused ? 2 : 1;  // the fused step reads time n and writes time n+1
     for (int b2 = 0; b2 < nbuf; ++b2) {
         for (int k = 0; k < PW_NFIELDS; ++k) take(cells * (c16 ? 2 : sb));
@@ -269,7 +269,7 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     s->inv_scale_pml = L.isp;
     const std::vector<size_t>& off = L.off;
     const size_t total = L.total;
-    const bool c16 = s->prec == PW_COMP16;
+    const bool c16 = has_lo(s->prec);
     const int nc = ncoef(s->prec);
     (void)g;
 
@@ -313,6 +313,7 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     s->b.xcd_map = xcd ? 1 : 0;
     s->b.slab = order == 4 ? 0 : std::max(0, slab);  // the z-slab interleave assumes a one-plane halo
     s->b.order = order;
+    s->b.sc_stress = s->b.sc_vel = s->b.sc_vw = 1.0f;
 
     if (c.stream) {
         s->stream = (hipStream_t)c.stream;
@@ -382,9 +383,9 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
     m.dt = c.dt;
     double* d_res = nullptr;
     PW_HIP_TRY(hipMalloc(&d_res, 4 * sizeof(double)));
-    double res[3] = {0, 0, 0};
+    double res[4] = {0, 0, 0, 0};
     hipError_t e = medium_reduce(m, d_res, s->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(res, d_res, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(res, d_res, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(d_res);
     if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_set_medium: reduction: ") + hipGetErrorString(e));
@@ -399,7 +400,7 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
                                          " > 1 (vp_max " + std::to_string(s->vp_max) + " m/s)");
     // Half exponent, Half/main.cu:793-807 (float arithmetic as there)
     double sm = 1.0, ism = 1.0;
-    if (is_scaled(s->prec)) {
+    if (is_scaled(s->prec) || s->prec == PW_COMP16F) {
         int em = -23;
         const float Kmax = (float)res[0], Rmax = (float)res[1], DT = (float)c.dt;
         if (Kmax > 0.0f && Rmax > 0.0f) {
@@ -412,6 +413,21 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
         s->source_scale = std::ldexp(1.0f, c.half_es);
         s->stress_base = std::ldexp(1.0f, -c.half_es);
         s->vel_base = std::ldexp(1.0f, em - c.half_es);
+        if (s->prec == PW_COMP16F) {
+            // the Half state's scales on a physical-unit update: coefficients stay unscaled
+            s->b.sc_stress = std::ldexp(1.0f, c.half_es);
+            s->b.sc_vel = std::ldexp(1.0f, c.half_es - em);
+            // The relative fluid velocity is up to rhof/CC1 times the solid velocity
+            // (res[3], largest in the most permeable rock) and far smaller in tight
+            // rock.  Its own scale lifts it so that the largest vw is stored about 16x
+            // the solid velocity: the tight-rock vw stays in FP16's normal range, and
+            // near-source values (stresses and velocities of order 1-10 stored) stay
+            // far below the FP16 maximum.
+            int kw = 0;
+            if (res[3] > 0.0 && std::isfinite(res[3])) kw = std::max(0, std::min(40, (int)std::lrint(-std::log2(res[3])) + 4));
+            s->b.sc_vw = std::ldexp(s->b.sc_vel, kw);
+            sm = ism = 1.0;
+        }
     }
     e = medium_coefficients(m, s->b, sm, ism, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);

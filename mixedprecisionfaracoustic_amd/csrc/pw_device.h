@@ -35,11 +35,13 @@ enum ScaledCoef {
 constexpr int kMaxCoef = 22;
 
 inline bool is_scaled(int prec) { return prec == PW_COMP16 || prec == PW_BF16; }
+// the compensated FP16 state: FP16 "hi" arrays plus int8 "lo" residual arrays
+inline bool has_lo(int prec) { return prec == PW_COMP16 || prec == PW_COMP16F; }
 inline int ncoef(int prec) { return is_scaled(prec) ? (int)SC_N : (int)PC_N; }
 // Bytes of one stored wavefield value / coefficient / memory variable.
-inline int state_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : prec == PW_COMP16 ? 3 : 2; }
-inline int coef_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : 2; }
-inline int mem_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : 2; }
+inline int state_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : has_lo(prec) ? 3 : 2; }
+inline int coef_bytes(int prec) { return prec == PW_FP64 ? 8 : (prec == PW_FP32 || prec == PW_COMP16F) ? 4 : 2; }
+inline int mem_bytes(int prec) { return prec == PW_FP64 ? 8 : (prec == PW_FP32 || prec == PW_COMP16F) ? 4 : 2; }
 
 // Device pointers of one solver.  For comp16, f[] holds the FP16 "hi" arrays
 // and lo[] the int8 residuals; other precisions leave lo[] null.
@@ -56,6 +58,7 @@ struct DevBuffers {
     int ty;          // rows per tile in the step kernels (block 64 x ty)
     int vx;          // cells per lane along x in the step kernels (1, 2, 4)
     int order;       // spatial order of the staggered differences: 2 or 4
+    float sc_stress, sc_vel, sc_vw;  // comp16f: stored value = physical * scale (powers of two)
     int xcd_map;     // 1: XCD-aware block -> tile mapping in the step kernels
     int slab;        // >0: interleave velocity/stress launches over z-slabs of this many planes
     int fused;       // 1: one fused launch per step over double-buffered state
@@ -129,7 +132,8 @@ struct MediumArgs {
     int nx, ny, nz;                // total grid
     double dt;
 };
-// res[0] = Kmax, res[1] = Rmax, res[2] = max fast-P velocity squared (Half/main.cu:745-791)
+// res[0] = Kmax, res[1] = Rmax, res[2] = max fast-P velocity squared (Half/main.cu:745-791),
+// res[3] = max rhof / CC1 (the size of the relative fluid velocity against the solid's)
 hipError_t medium_reduce(const MediumArgs& m, double* d_res3, hipStream_t st);
 // Writes the coefficient arrays of b (physical or scaled form).
 hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double scale_m, double inv_scale_m,

@@ -143,7 +143,7 @@ struct PolF32 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned) { m[o] = v; }
-    static __device__ __forceinline__ R rt(R v, unsigned) { return v; }  // value as stored
+    static __device__ __forceinline__ R rt(const Field&, R v) { return v; }  // value as stored
 };
 struct PolF64 {
     using R = double; using Coef = double; using Mem = double;
@@ -155,7 +155,7 @@ struct PolF64 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned) { m[o] = v; }
-    static __device__ __forceinline__ R rt(R v, unsigned) { return v; }
+    static __device__ __forceinline__ R rt(const Field&, R v) { return v; }
 };
 struct PolC16 {
     using R = float; using Coef = f16; using Mem = f16;
@@ -195,14 +195,64 @@ struct PolC16 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return (float)c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return (float)m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned z) { m[o] = (f16)round32(v, z); }
-    static __device__ __forceinline__ R rt(R v, unsigned z)
+    static __device__ __forceinline__ R rt(const Field& f, R v)
     {
         f16 h;
         int8_t l;
-        c16_encode(v, h, l, z);
+        c16_encode(v, h, l, f.z);
         return c16_decode(h, l);
     }
 };
+// comp16 storage under the physical (Float) update: values are stored times a
+// power-of-two scale per field class (exact), coefficients and memory variables
+// are FP32.
+struct PolC16F {
+    using R = float; using Coef = float; using Mem = float;
+    static constexpr bool scaled = false;
+    struct Field { f16* hi; int8_t* lo; unsigned z; float sc, isc; };
+    static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return c16_decode(f.hi[o], f.lo[o]) * f.isc; }
+    static __device__ __forceinline__ void st(const Field& f, unsigned o, R v)
+    {
+        f16 h;
+        int8_t l;
+        c16_encode(v * f.sc, h, l, f.z);
+        f.hi[o] = h;
+        f.lo[o] = l;
+    }
+    template <int N> static __device__ __forceinline__ void ldv(const Field& f, unsigned o, R (&r)[N])
+    {
+        const Pack<f16, N> h = ld_pack<N>(f.hi, o);
+        const Pack<int8_t, N> l = ld_pack<N>(f.lo, o);
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = c16_decode(h.v[j], l.v[j]) * f.isc;
+    }
+    template <int N> static __device__ __forceinline__ void stv(const Field& f, unsigned o, const R (&r)[N])
+    {
+        Pack<f16, N> h;
+        Pack<int8_t, N> l;
+#pragma unroll
+        for (int j = 0; j < N; ++j) c16_encode(r[j] * f.sc, h.v[j], l.v[j], f.z);
+        st_pack<N>(f.hi, o, h);
+        st_pack<N>(f.lo, o, l);
+    }
+    static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
+    template <int N> static __device__ __forceinline__ void ldcv(const Coef* c, unsigned o, R (&r)[N])
+    {
+        const Pack<Coef, N> t = ld_pack<N>(c, o);
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = t.v[j];
+    }
+    static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
+    static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned) { m[o] = v; }
+    static __device__ __forceinline__ R rt(const Field& f, R v)
+    {
+        f16 h;
+        int8_t l;
+        c16_encode(v * f.sc, h, l, f.z);
+        return c16_decode(h, l) * f.isc;
+    }
+};
+
 struct PolB16 {
     using R = float; using Coef = f16; using Mem = bf16;
     static constexpr bool scaled = true;
@@ -213,7 +263,7 @@ struct PolB16 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return (float)c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return (float)m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned z) { m[o] = (bf16)round32(v, z); }
-    static __device__ __forceinline__ R rt(R v, unsigned z) { return (float)(bf16)round32(v, z); }
+    static __device__ __forceinline__ R rt(const Field& f, R v) { return (float)(bf16)round32(v, f.z); }
 };
 
 template <class P> struct StepArgs {
@@ -239,6 +289,11 @@ template <> __host__ PolF32::Field make_field<PolF32>(const DevBuffers& b, int k
 template <> __host__ PolF64::Field make_field<PolF64>(const DevBuffers& b, int k) { return {(double*)b.f[k]}; }
 template <> __host__ PolC16::Field make_field<PolC16>(const DevBuffers& b, int k) { return {(f16*)b.f[k], b.lo[k], 0u}; }
 template <> __host__ PolB16::Field make_field<PolB16>(const DevBuffers& b, int k) { return {(bf16*)b.f[k], 0u}; }
+template <> __host__ PolC16F::Field make_field<PolC16F>(const DevBuffers& b, int k)
+{
+    const float sc = k <= PW_VUZ ? b.sc_vel : (k <= PW_VWZ ? b.sc_vw : b.sc_stress);
+    return {(f16*)b.f[k], b.lo[k], 0u, sc, 1.0f / sc};
+}
 
 template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
 {
@@ -1059,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_fused(const FusedArgs<P> A)
             if (mask & 2) v_axis<P, 1>(A, ix, iy, z, own, u[1], u[4]);
             if (mask & 4) v_axis<P, 2>(A, ix, iy, z, own, u[2], u[5]);
             // the stress update reads the velocities as stored (Half: encoded)
-            for (int c = 0; c < 6; ++c) u[c] = P::rt(u[c], a.zero);
+            for (int c = 0; c < 6; ++c) u[c] = P::rt(a.f[PW_VUX + c], u[c]);
         } else if (in_xy && z >= 0 && z <= nz - 1) {
             const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z * PL;
             for (int c = 0; c < 3; ++c)
@@ -1083,7 +1138,7 @@ __global__ __launch_bounds__(256) void k_fused(const FusedArgs<P> A)
             if (need & 1) v_axis<P, 0>(A, hx, hy, z, false, vu[0], vw[0]);
             if (need & 2) v_axis<P, 1>(A, hx, hy, z, false, vu[1], vw[1]);
             if (need & 4) v_axis<P, 2>(A, hx, hy, z, false, vu[2], vw[2]);
-            for (int c = 0; c < 3; ++c) { vu[c] = P::rt(vu[c], a.zero); vw[c] = P::rt(vw[c], a.zero); }
+            for (int c = 0; c < 3; ++c) { vu[c] = P::rt(a.f[PW_VUX + c], vu[c]); vw[c] = P::rt(a.f[PW_VWX + c], vw[c]); }
         } else if (h_in && z >= 0 && z <= nz - 1) {
             const unsigned o = (unsigned)hx + (unsigned)hy * W + (unsigned)z * PL;
             for (int c = 0; c < 3; ++c)
@@ -1491,6 +1546,7 @@ template <class F> static hipError_t dispatch(int prec, F&& f)
     case PW_FP64: return f(PolF64{});
     case PW_COMP16: return f(PolC16{});
     case PW_BF16: return f(PolB16{});
+    case PW_COMP16F: return f(PolC16F{});
     default: return hipErrorInvalidValue;
     }
 }

@@ -89,11 +89,11 @@ __device__ __forceinline__ void atomic_max_pos(double* addr, double v)
     atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
 }
 
-__global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double* res)
+__global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double dt, double* res)
 {
-    __shared__ double sk[256], sr[256], sv[256];
+    __shared__ double sk[256], sr[256], sv[256], sw[256];
     const long long n = (long long)nx * ny * nz;
-    double kmax = 0.0, rmax = 0.0, vmax = 0.0;
+    double kmax = 0.0, rmax = 0.0, vmax = 0.0, wmax = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const int ix = (int)(i % nx), iy = (int)((i / nx) % ny), iz = (int)(i / ((long long)nx * ny));
@@ -113,17 +113,22 @@ __global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double* res)
                 rmax = fmax(rmax, 1.0 / fabs(Q));
                 rmax = fmax(rmax, fabs(rhof * ri[d] / Q));
             }
+            // relative fluid velocity per unit solid velocity in one step: the
+            // vw forcing rhof*rho^-1*S/CC1 against the vu forcing rho^-1*S
+            const double cc1 = 0.5 * c1[d] * dt + Q;
+            if (c1[d] != 0.0 && cc1 != 0.0) wmax = fmax(wmax, fabs(rhof / cc1));
         }
         vmax = fmax(vmax, vp2_cell(v.at(PW_RHO, ix, iy, iz), rhof, v.at(PW_C1, ix, iy, iz), v.at(PW_C2, ix, iy, iz),
                                    HH, C, M));
     }
-    sk[threadIdx.x] = kmax; sr[threadIdx.x] = rmax; sv[threadIdx.x] = vmax;
+    sk[threadIdx.x] = kmax; sr[threadIdx.x] = rmax; sv[threadIdx.x] = vmax; sw[threadIdx.x] = wmax;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             sk[threadIdx.x] = fmax(sk[threadIdx.x], sk[threadIdx.x + w]);
             sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + w]);
             sv[threadIdx.x] = fmax(sv[threadIdx.x], sv[threadIdx.x + w]);
+            sw[threadIdx.x] = fmax(sw[threadIdx.x], sw[threadIdx.x + w]);
         }
         __syncthreads();
     }
@@ -131,6 +136,7 @@ __global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double* res)
         atomic_max_pos(res + 0, sk[0]);
         atomic_max_pos(res + 1, sr[0]);
         atomic_max_pos(res + 2, sv[0]);
+        atomic_max_pos(res + 3, sw[0]);
     }
 }
 
@@ -240,9 +246,9 @@ static BaseView make_view(const MediumArgs& m)
 
 hipError_t medium_reduce(const MediumArgs& m, double* d_res3, hipStream_t st)
 {
-    hipError_t e = hipMemsetAsync(d_res3, 0, 3 * sizeof(double), st);
+    hipError_t e = hipMemsetAsync(d_res3, 0, 4 * sizeof(double), st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_medium_reduce, dim3(2048), dim3(256), 0, st, make_view(m), m.nx, m.ny, m.nz, d_res3);
+    hipLaunchKernelGGL(k_medium_reduce, dim3(2048), dim3(256), 0, st, make_view(m), m.nx, m.ny, m.nz, m.dt, d_res3);
     return hipGetLastError();
 }
 
@@ -258,6 +264,7 @@ hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double 
     switch (b.prec) {
     case PW_FP32: hipLaunchKernelGGL(k_coef_physical<float>, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, out); break;
     case PW_FP64: hipLaunchKernelGGL(k_coef_physical<double>, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, out); break;
+    case PW_COMP16F: hipLaunchKernelGGL(k_coef_physical<float>, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, out); break;
     case PW_COMP16:
     case PW_BF16:
         hipLaunchKernelGGL(k_coef_scaled, grid, block, 0, st, v, m.nx, m.ny, m.nz, (float)m.dt, (float)scale_m,

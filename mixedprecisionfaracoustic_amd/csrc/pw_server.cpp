@@ -166,7 +166,7 @@ void teardown_shot(Shot* q)
 // ---- the rounds ----
 struct Engine {
     pw_server* srv;
-    BatchGroup groups[4];         // one per precision
+    BatchGroup groups[5];         // one per precision
     std::vector<Shot*> members;   // active shots in group order when the tables were built
     bool stale = true;            // membership changed since the tables / graph were built
     hipGraphExec_t exec = nullptr;
@@ -187,7 +187,7 @@ struct Engine {
         exec = nullptr;
         last_K = 0;
         members.clear();
-        for (int p = 0; p < 4; ++p) {
+        for (int p = 0; p < 5; ++p) {
             std::vector<BatchShot> bs;
             std::vector<int> its;
             for (Shot* q : srv->active) {
@@ -409,7 +409,7 @@ pw_status check_shot(const pw_server* srv, const pw_shot* r)
 {
     if (!r) return fail(PW_ERR_INVALID, "pw_server_submit: null shot");
     if (!srv->d_medium) return fail(PW_ERR_STATE, "pw_server_submit: no medium set");
-    if (r->precision < PW_FP32 || r->precision > PW_FP64) return fail(PW_ERR_INVALID, "pw_server_submit: precision");
+    if (r->precision < PW_FP32 || r->precision > PW_COMP16F) return fail(PW_ERR_INVALID, "pw_server_submit: precision");
     if (r->np < 1 || r->nx < 2 * r->np + 3 || r->ny < 2 * r->np + 3 || r->nz < 2 * r->np + 3)
         return fail(PW_ERR_INVALID, "pw_server_submit: grid must be at least 2*np+3 cells per axis");
     if (!(r->h > 0) || !(r->dt > 0) || r->nt < 1 || !(r->f0 > 0))

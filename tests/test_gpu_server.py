@@ -29,11 +29,11 @@ def volume():
 
 
 def shot_list():
-    """Shots of different sizes, precisions, lengths, crops and sources."""
+    """Shots of different sizes, all five precisions, lengths, crops and sources."""
     out = []
     specs = [((28, 26, 30), "fp32", 120, (3, 5, 7)), ((32, 32, 32), "comp16", 150, (10, 2, 4)),
              ((36, 30, 34), "fp32", 64, (0, 0, 0)), ((30, 34, 28), "bf16", 90, (20, 6, 12)),
-             ((26, 28, 36), "fp64", 75, (8, 10, 20)), ((34, 28, 30), "comp16", 101, (5, 9, 11))]
+             ((26, 28, 36), "fp64", 75, (8, 10, 20)), ((34, 28, 30), "comp16f", 101, (5, 9, 11))]
     for k, ((nx, ny, nz), prec, nt, crop) in enumerate(specs):
         src = (nx // 2 - 1 + k % 3, ny // 2 + 1 - k % 2, nz // 2 - 2 + k % 4)
         iy, iz = ny // 2 + 2, NP + 3
