@@ -247,6 +247,24 @@ def precision_report(args):
     return 0
 
 
+def hbm_ceiling():
+    """Run the standalone bandwidth probe built next to the library (plain copy
+    and 2/6/12-read streaming kernels over 256 MiB arrays) as a child process,
+    and print its JSON line: the rates a kernel of the simplest shape reaches,
+    against which the step kernels' rates are read."""
+    import subprocess
+    probe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mixedprecisionfaracoustic_amd", "pw_hbm_probe")
+    if not os.path.exists(probe):
+        raise SystemExit(f"{probe} is not built (python -m mixedprecisionfaracoustic_amd.build)")
+    r = subprocess.run([probe], capture_output=True, text=True)
+    sys.stderr.write(r.stdout + r.stderr)
+    if r.returncode != 0:
+        return r.returncode
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    print(json.dumps({"metric": "hbm_ceiling", "peak_datasheet_gb_s": 8000.0, **line}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,10 +286,14 @@ def main():
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--round-steps", type=int, default=32)
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--hbm-ceiling", action="store_true",
+                    help="run the streaming-kernel bandwidth probe (the practical HBM roof) instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         return 2
+    if args.hbm_ceiling:
+        return hbm_ceiling()
     if args.serving:
         return serving_main(args)
     if args.precision_report:

@@ -12,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libporowave.so")
 OBJDIR = os.path.join(HERE, "_build")
+# standalone HBM bandwidth probe (bench.py --hbm-ceiling), not part of the library
+PROBE = os.path.join(HERE, "pw_hbm_probe")
 SOURCES = ["pw_api.cpp", "pw_medium.cpp", "pw_server.cpp", "pw_output.cpp", "pw_kernels.hip", "pw_medium_gpu.hip"]
 ARCH = os.environ.get("PW_OFFLOAD_ARCH", "gfx950")
 
@@ -37,7 +39,9 @@ def flags():
 def _compile(src, verbose):
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
     lang = ["-x", "hip"]
-    cmd = [hipcc()] + flags() + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+    # extra flags for tuning experiments (e.g. -DPW_WAVES_PER_EU=8)
+    extra = os.environ.get("PW_HIPCC_FLAGS", "").split()
+    cmd = [hipcc()] + flags() + extra + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -48,10 +52,23 @@ def _compile(src, verbose):
     return obj
 
 
+def _build_probe(verbose):
+    cmd = [hipcc()] + flags() + ["-x", "hip", os.path.join(CSRC, "pw_hbm_probe.cpp"), "-o", PROBE + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on pw_hbm_probe.cpp:\n{r.stdout}\n{r.stderr}")
+    os.replace(PROBE + ".tmp", PROBE)
+    return PROBE
+
+
 def build(verbose=False):
     os.makedirs(OBJDIR, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=6) as ex:
+    with ThreadPoolExecutor(max_workers=7) as ex:
+        probe = ex.submit(_build_probe, verbose)
         objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
+        probe.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     if verbose:

@@ -1,0 +1,145 @@
+// pw_hbm_probe -- measured HBM bandwidth of plain streaming kernels on this GPU.
+//
+// The step kernels are HBM-bound, so the rate that a kernel of the simplest
+// possible shape reaches is the practical roof they are judged against, next
+// to the datasheet figure.  Each pattern below streams separate 256 MiB arrays
+// with 16-byte loads and stores in a grid-stride loop:
+//   copy         1 read : 1 write
+//   stream 2:1   2 reads : 1 write
+//   stream 6:1   6 reads : 1 write
+//   stream 12:1  12 reads : 1 write  (the step kernels' read/write mix and
+//                                     number of concurrent streams)
+//   read 12      12 reads, one value per thread written at the end
+// Rates count the bytes each pattern must move (every array once), timed
+// with HIP events over repeated launches, best over a few grid sizes.
+//
+// Output: one line per pattern, "pattern blocks GB/s", then a JSON summary.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#define CHECK(x)                                                                                  \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) {                                                                   \
+            std::fprintf(stderr, "HIP error %s at line %d\n", hipGetErrorString(e_), __LINE__); \
+            std::exit(1);                                                                         \
+        }                                                                                         \
+    } while (0)
+
+namespace {
+
+constexpr int kMaxIn = 12;
+constexpr int kThreads = 256;
+struct Inputs {
+    const float4* a[kMaxIn];
+};
+
+__global__ __launch_bounds__(kThreads) void k_fill(float4* p, size_t n4, float v)
+{
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads)
+        p[i] = make_float4(v, v, v, v);
+}
+
+// R reads : 1 write (R = 1 is a copy)
+template <int R>
+__global__ __launch_bounds__(kThreads) void k_stream(Inputs in, float4* __restrict__ out, size_t n4)
+{
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads) {
+        float4 s = in.a[0][i];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            const float4 v = in.a[r][i];
+            s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+        }
+        out[i] = s;
+    }
+}
+
+// R reads, one float per thread written (sums keep the loads live)
+template <int R>
+__global__ __launch_bounds__(kThreads) void k_read(Inputs in, float* __restrict__ out, size_t n4)
+{
+    float acc = 0.0f;
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kThreads) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float4 v = in.a[r][i];
+            acc += v.x + v.y + v.z + v.w;
+        }
+    }
+    out[blockIdx.x * (size_t)kThreads + threadIdx.x] = acc;
+}
+
+template <class F>
+double best_rate(F launch, double bytes, int iters, const int* grids, int ngrids, int* best_grid)
+{
+    hipEvent_t e0, e1;
+    CHECK(hipEventCreate(&e0));
+    CHECK(hipEventCreate(&e1));
+    double best = 0.0;
+    for (int g = 0; g < ngrids; ++g) {
+        launch(grids[g]);  // warm-up
+        CHECK(hipGetLastError());
+        CHECK(hipEventRecord(e0));
+        for (int it = 0; it < iters; ++it) launch(grids[g]);
+        CHECK(hipEventRecord(e1));
+        CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double rate = bytes * iters / (ms * 1e-3) / 1e9;
+        if (rate > best) { best = rate; *best_grid = grids[g]; }
+    }
+    CHECK(hipEventDestroy(e0));
+    CHECK(hipEventDestroy(e1));
+    return best;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    const int iters = argc > 1 ? std::atoi(argv[1]) : 20;
+    const size_t n = (size_t)1 << 26;  // floats per array: 256 MiB
+    const size_t n4 = n / 4;
+    const double arr = (double)n * sizeof(float);
+    const int grids[4] = {1024, 2048, 4096, 16384};
+    const int ngrids = 4;
+
+    float4* buf[kMaxIn + 1];
+    for (int r = 0; r <= kMaxIn; ++r) {
+        CHECK(hipMalloc(&buf[r], n * sizeof(float)));
+        k_fill<<<4096, kThreads>>>(buf[r], n4, 1.0f + r);
+    }
+    float* red = nullptr;
+    CHECK(hipMalloc(&red, (size_t)grids[ngrids - 1] * kThreads * sizeof(float)));
+    CHECK(hipDeviceSynchronize());
+    Inputs in;
+    for (int r = 0; r < kMaxIn; ++r) in.a[r] = buf[r];
+    float4* out = buf[kMaxIn];
+
+    struct Row { const char* name; double rate; int grid; } rows[5];
+    int g = 0;
+    rows[0] = {"copy", best_rate([&](int b) { k_stream<1><<<b, kThreads>>>(in, out, n4); }, 2 * arr, iters, grids, ngrids, &g), g};
+    rows[1] = {"stream_2r1w", best_rate([&](int b) { k_stream<2><<<b, kThreads>>>(in, out, n4); }, 3 * arr, iters, grids, ngrids, &g), g};
+    rows[2] = {"stream_6r1w", best_rate([&](int b) { k_stream<6><<<b, kThreads>>>(in, out, n4); }, 7 * arr, iters, grids, ngrids, &g), g};
+    rows[3] = {"stream_12r1w", best_rate([&](int b) { k_stream<12><<<b, kThreads>>>(in, out, n4); }, 13 * arr, iters, grids, ngrids, &g), g};
+    rows[4] = {"read_12", best_rate([&](int b) { k_read<12><<<b, kThreads>>>(in, red, n4); }, 12 * arr, iters, grids, ngrids, &g), g};
+
+    // the 2:1 stream of arrays filled with 1 and 2 must give 3
+    k_stream<2><<<1024, kThreads>>>(in, out, n4);
+    float4 h;
+    CHECK(hipMemcpy(&h, out + n4 - 1, sizeof(h), hipMemcpyDeviceToHost));
+    if (h.x != 3.0f || h.w != 3.0f) {
+        std::fprintf(stderr, "probe check failed: %g\n", h.x);
+        return 1;
+    }
+    for (const Row& r : rows) std::printf("%-14s %6d blocks %8.1f GB/s\n", r.name, r.grid, r.rate);
+    std::printf("{");
+    for (int i = 0; i < 5; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
+    std::printf(", \"unit\": \"GB/s\", \"array_bytes\": %.0f, \"iters\": %d}\n", arr, iters);
+    for (int r = 0; r <= kMaxIn; ++r) CHECK(hipFree(buf[r]));
+    CHECK(hipFree(red));
+    return 0;
+}

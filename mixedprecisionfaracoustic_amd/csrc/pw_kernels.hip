@@ -962,11 +962,22 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     }
 }
 
-template <class P, int N, int ORD> __global__ __launch_bounds__(512) void k_velocity(const StepArgs<P> a)
+// Occupancy hint for the single-shot step kernels (waves per SIMD), set at build
+// time for tuning; 0 leaves the register allocation to the compiler.
+#ifndef PW_WAVES_PER_EU
+#define PW_WAVES_PER_EU 0
+#endif
+#if PW_WAVES_PER_EU > 0
+#define PW_OCCUPANCY __attribute__((amdgpu_waves_per_eu(PW_WAVES_PER_EU)))
+#else
+#define PW_OCCUPANCY
+#endif
+
+template <class P, int N, int ORD> __global__ __launch_bounds__(512) PW_OCCUPANCY void k_velocity(const StepArgs<P> a)
 {
     velocity_tile<P, N, ORD>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
 }
-template <class P, int N, int ORD> __global__ __launch_bounds__(512) void k_stress(const StepArgs<P> a)
+template <class P, int N, int ORD> __global__ __launch_bounds__(512) PW_OCCUPANCY void k_stress(const StepArgs<P> a)
 {
     stress_tile<P, N, ORD>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
 }
@@ -976,10 +987,10 @@ template <class P, int N, int ORD> __global__ __launch_bounds__(512) void k_stre
 //
 // The state is double-buffered (read time n from s.f, write time n+1 to g), so
 // each wavefield is read once and written once per step instead of read twice
-// (39 -> 26 touches).  A block owns a 64 x 4 column tile and marches up its z
+// (39 -> 26 touches).  A block owns a 62 x 6 column tile and marches up its z
 // chunk: at plane k it computes the new velocities of plane k+1 (tile cells: all
-// three axes, stored; the one-cell ring around the tile: only the axis
-// components the stress stencil reads, not stored), then the stress update of
+// three axes, stored; the one-cell ring around the tile: the axis components
+// the stress stencil reads there, not stored), then the stress update of
 // plane k from registers (own column, planes k-1..k+1) and LDS (x/y neighbours).
 // Ring cells belong to neighbouring tiles and are recomputed here from the same
 // time-n inputs, so they come out bit-identical to the owner's values.  The CPML
@@ -1076,43 +1087,45 @@ __device__ __forceinline__ void v_axis(const FusedArgs<P>& A, int ix, int iy, in
     }
 }
 
-template <class P>
-__global__ __launch_bounds__(256) void k_fused(const FusedArgs<P> A)
+// Block of 64 x ROWS threads, one cell per thread: lanes 1..62 of rows
+// 1..ROWS-2 own a 62 x (ROWS-2) tile; lane 0 / 63 and row 0 / ROWS-1 are the
+// one-cell ring around it.  Every thread computes new velocities at its cell,
+// so the ring costs whole waves of uniform work instead of a few divergent
+// lanes: ring row 0 (y - 1) computes the y components the stress stencil reads
+// there (vuy, vwy), ring row ROWS-1 (y + 1) the x and z ones (vux, vuz), the
+// rows in between all three axes (lanes 0 and 63 supply vux/vwx and vuy/vuz).
+// Only owned cells store velocities and memory variables and update stresses.
+template <class P, int ROWS>
+__global__ __launch_bounds__(64 * ROWS) void k_fused(const FusedArgs<P> A)
 {
     using R = typename P::R;
+    constexpr int OWN_X = 62, OWN_Y = ROWS - 2;
     const StepArgs<P>& a = A.s;
     // velocity of the current plane for the tile and its ring: vux, vuy, vuz, vwx, vwy
-    __shared__ R lds[2][5][6 * 66];
+    __shared__ R lds[2][5][ROWS * 64];
     const Tile tl = tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map);
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
-    const int ix = tl.bx * 64 + tx, iy = tl.by * 4 + ty;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int ix = tl.bx * OWN_X - 1 + tx, iy = tl.by * OWN_Y - 1 + ty;
     const int z0 = a.zb + tl.bz * a.zc;
     const int z1 = min(z0 + a.zc, a.ze);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const unsigned PL = a.plane, W = (unsigned)a.pitch;
     // no early return below: every thread takes part in the barriers
-    const bool in_xy = ix <= nx - 1 && iy <= ny - 1;
+    const bool in_xy = ix >= 0 && ix <= nx - 1 && iy >= 0 && iy <= ny - 1;
     const bool up_xy = ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2;
-    const int lt = (ty + 1) * 66 + (tx + 1);
-    // ring cell of this thread: 0 = row y-1 (vuy, vwy), 1 = row y+4 (vux, vuz),
-    // 2 = column x-1 (vux, vwx), 3 = column x+64 (vuy, vuz)
-    int hk = -1, hx = 0, hy = 0;
-    if (tid < 64) { hk = 0; hx = tl.bx * 64 + tid; hy = tl.by * 4 - 1; }
-    else if (tid < 128) { hk = 1; hx = tl.bx * 64 + tid - 64; hy = tl.by * 4 + 4; }
-    else if (tid < 132) { hk = 2; hx = tl.bx * 64 - 1; hy = tl.by * 4 + tid - 128; }
-    else if (tid < 136) { hk = 3; hx = tl.bx * 64 + 64; hy = tl.by * 4 + tid - 132; }
-    const bool h_in = hk >= 0 && hx >= 0 && hx <= nx - 1 && hy >= 0 && hy <= ny - 1;
-    const bool h_up = hk >= 0 && hx >= 1 && hx <= nx - 2 && hy >= 1 && hy <= ny - 2;
-    const int lh = hk >= 0 ? (hy - tl.by * 4 + 1) * 66 + (hx - tl.bx * 64 + 1) : 0;
+    const bool own = tx >= 1 && tx <= OWN_X && ty >= 1 && ty <= OWN_Y;
+    // axes this row computes (1 x, 2 y, 4 z): wave-uniform
+    const int rmask = ty == 0 ? 2 : (ty == ROWS - 1 ? 5 : 7);
+    const int lt = ty * 64 + tx;
 
-    // new velocities of the tile cell at plane z for the axes in mask (1 x, 2 y, 4 z);
-    // cells that are never updated keep their stored value
-    auto v_tile = [&](int z, bool own, int mask, R* u) {
+    // new velocities at plane z for the axes in mask; cells that are never
+    // updated keep their stored value
+    auto v_cell = [&](int z, bool wr, int mask, R* u) {
         if (z0 >= z1) return;
         if (up_xy && z >= 1 && z <= nz - 2) {
-            if (mask & 1) v_axis<P, 0>(A, ix, iy, z, own, u[0], u[3]);
-            if (mask & 2) v_axis<P, 1>(A, ix, iy, z, own, u[1], u[4]);
-            if (mask & 4) v_axis<P, 2>(A, ix, iy, z, own, u[2], u[5]);
+            if (mask & 1) v_axis<P, 0>(A, ix, iy, z, wr, u[0], u[3]);
+            if (mask & 2) v_axis<P, 1>(A, ix, iy, z, wr, u[1], u[4]);
+            if (mask & 4) v_axis<P, 2>(A, ix, iy, z, wr, u[2], u[5]);
             // the stress update reads the velocities as stored (Half: encoded)
             for (int c = 0; c < 6; ++c) u[c] = P::rt(a.f[PW_VUX + c], u[c]);
         } else if (in_xy && z >= 0 && z <= nz - 1) {
@@ -1121,62 +1134,42 @@ __global__ __launch_bounds__(256) void k_fused(const FusedArgs<P> A)
                 if (mask & (1 << c)) { u[c] = P::ld(a.f[PW_VUX + c], o); u[3 + c] = P::ld(a.f[PW_VWX + c], o); }
         }
     };
-    auto store_tile = [&](int z, const R* u) {
-        if (z0 >= z1 || !up_xy || z < 1 || z > nz - 2) return;
+    auto store_cell = [&](int z, const R* u) {
+        if (z0 >= z1 || !own || !up_xy || z < 1 || z > nz - 2) return;
         const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z * PL;
         for (int c = 0; c < 6; ++c) P::st(A.g[PW_VUX + c], o, u[c]);
     };
-    auto put_tile = [&](int buf, const R* u) {
+    auto put_cell = [&](int buf, const R* u) {
         lds[buf][0][lt] = u[0]; lds[buf][1][lt] = u[1]; lds[buf][2][lt] = u[2];
         lds[buf][3][lt] = u[3]; lds[buf][4][lt] = u[4];
     };
-    auto ring = [&](int z, int buf) {
-        if (hk < 0 || z0 >= z1) return;
-        R vu[3] = {0, 0, 0}, vw[3] = {0, 0, 0};
-        const int need = hk == 0 ? 2 : (hk == 1 ? 5 : (hk == 2 ? 1 : 6));
-        if (h_up && z >= 1 && z <= nz - 2) {
-            if (need & 1) v_axis<P, 0>(A, hx, hy, z, false, vu[0], vw[0]);
-            if (need & 2) v_axis<P, 1>(A, hx, hy, z, false, vu[1], vw[1]);
-            if (need & 4) v_axis<P, 2>(A, hx, hy, z, false, vu[2], vw[2]);
-            for (int c = 0; c < 3; ++c) { vu[c] = P::rt(a.f[PW_VUX + c], vu[c]); vw[c] = P::rt(a.f[PW_VWX + c], vw[c]); }
-        } else if (h_in && z >= 0 && z <= nz - 1) {
-            const unsigned o = (unsigned)hx + (unsigned)hy * W + (unsigned)z * PL;
-            for (int c = 0; c < 3; ++c)
-                if (need & (1 << c)) { vu[c] = P::ld(a.f[PW_VUX + c], o); vw[c] = P::ld(a.f[PW_VWX + c], o); }
-        }
-        if (hk == 0) { lds[buf][1][lh] = vu[1]; lds[buf][4][lh] = vw[1]; }
-        else if (hk == 1) { lds[buf][0][lh] = vu[0]; lds[buf][2][lh] = vu[2]; }
-        else if (hk == 2) { lds[buf][0][lh] = vu[0]; lds[buf][3][lh] = vw[0]; }
-        else { lds[buf][1][lh] = vu[1]; lds[buf][2][lh] = vu[2]; }
-    };
 
     R prev[6] = {0, 0, 0, 0, 0, 0}, cur[6] = {0, 0, 0, 0, 0, 0}, nxt[6] = {0, 0, 0, 0, 0, 0};
-    v_tile(z0 - 1, false, 4, prev);  // vuz, vwz below the chunk (owned by the block below)
-    v_tile(z0, true, 7, cur);
-    store_tile(z0, cur);
-    put_tile(0, cur);
-    ring(z0, 0);
+    // vuz, vwz below the chunk (owned by the block below): only the tile reads them
+    if (own) v_cell(z0 - 1, false, 4, prev);
+    v_cell(z0, own, rmask, cur);
+    store_cell(z0, cur);
+    put_cell(0, cur);
     __syncthreads();
 
     for (int k = z0; k < z1; ++k) {
         const int cb = (k - z0) & 1, nb = cb ^ 1;
         if (k + 1 < z1) {
-            v_tile(k + 1, true, 7, nxt);
-            store_tile(k + 1, nxt);
-            put_tile(nb, nxt);
-            ring(k + 1, nb);
-        } else {
-            v_tile(k + 1, false, 3, nxt);  // vux, vuy above the chunk (owned by the block above)
+            v_cell(k + 1, own, rmask, nxt);
+            store_cell(k + 1, nxt);
+            put_cell(nb, nxt);
+        } else if (own) {
+            v_cell(k + 1, false, 3, nxt);  // vux, vuy above the chunk (owned by the block above)
         }
         // stress update of plane k (Float/main.cu:234-293 / Half/main.cu:297-370)
-        if (up_xy) {
+        if (own && up_xy) {
             const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)k * PL;
             const Deriv<P> D(a.h);
     const R dt = a.dt;
-            const R vux_l = lds[cb][0][lt - 1], vux_q = lds[cb][0][lt + 66];
-            const R vuy_h = lds[cb][1][lt - 66], vuy_r = lds[cb][1][lt + 1];
-            const R vuz_r = lds[cb][2][lt + 1], vuz_q = lds[cb][2][lt + 66];
-            const R vwx_l = lds[cb][3][lt - 1], vwy_h = lds[cb][4][lt - 66];
+            const R vux_l = lds[cb][0][lt - 1], vux_q = lds[cb][0][lt + 64];
+            const R vuy_h = lds[cb][1][lt - 64], vuy_r = lds[cb][1][lt + 1];
+            const R vuz_r = lds[cb][2][lt + 1], vuz_q = lds[cb][2][lt + 64];
+            const R vwx_l = lds[cb][3][lt - 1], vwy_h = lds[cb][4][lt - 64];
             R uxx = D(cur[0] - vux_l), uyy = D(cur[1] - vuy_h), uzz = D(cur[2] - prev[2]);
             R wx = D(cur[3] - vwx_l), wy = D(cur[4] - vwy_h), wz = D(cur[5] - prev[5]);
             R uxy = D(vux_q - cur[0]), uyx = D(vuy_r - cur[1]);
@@ -1589,11 +1582,15 @@ hipError_t launch_fused(const DevBuffers& b, hipStream_t st)
 {
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
+        constexpr int ROWS = 8;
         FusedArgs<P> A;
-        DevBuffers b4 = b;
-        b4.ty = 4;  // the fused kernel's LDS tile is 64 x 4, one cell per lane
-        b4.vx = 1;
-        A.s = make_args<P>(b4);
+        DevBuffers b1 = b;
+        b1.vx = 1;
+        A.s = make_args<P>(b1);
+        // tiles of 62 x (ROWS - 2) owned cells (the kernel's ring takes the rest of the block)
+        A.s.gx = (b.g.nx + 61) / 62;
+        A.s.gy = (b.g.ny + ROWS - 3) / (ROWS - 2);
+        A.s.nblocks = A.s.gx * A.s.gy * ((A.s.ze - A.s.zb + b.zc - 1) / b.zc);
         for (int k = 0; k < PW_NFIELDS; ++k) {
             DevBuffers w = b;
             w.f[k] = b.fo[k];
@@ -1602,7 +1599,7 @@ hipError_t launch_fused(const DevBuffers& b, hipStream_t st)
         }
         for (int ax = 0; ax < 3; ++ax)
             for (int k = 0; k < kMemPerAxis; ++k) A.mo[ax][k] = (typename P::Mem*)b.memo[ax][k];
-        k_fused<P><<<A.s.nblocks, dim3(64, 4), 0, st>>>(A);
+        k_fused<P, ROWS><<<A.s.nblocks, dim3(64, ROWS), 0, st>>>(A);
         return hipGetLastError();
     });
 }
