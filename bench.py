@@ -264,6 +264,77 @@ def reference_scheme_bytes(prec, n, np_):
     return 19 * s + mv * c + 16 * f * m, 20 * s + mt * c + 16 * f * m
 
 
+def rank_share_main(args):
+    """One interior rank's share of the 512^3 workload split into z-slabs: the
+    step kernels at that shape, n x n x planes owned planes plus one ghost
+    plane on each z face.  The z faces are open (no absorbing layer, no z
+    memory variables); the ghost planes are held like edge planes, where a
+    decomposed run would write its neighbours' planes between the two kernels.
+    Throughput counts owned cells only."""
+    import numpy as np
+    import torch
+    import mixedprecisionfaracoustic_amd as pw
+
+    n, planes, np_, h, dt = args.n, args.rank_planes, args.np, 5.0, 0.5e-3
+    nz = planes + 2
+    nt = args.warmup + args.steps + args.prof_steps
+    stream = torch.cuda.Stream()
+    solver = pw.Solver(n, n, nz, np_, h, dt, nt, precision=args.precision, cpml_r=1e-4, cpml_v0=4000.0,
+                       stream=stream.cuda_stream, cpml_open=("z_lo", "z_hi"))
+    # the two-layer model with its interface inside the slab
+    spec = workload(n, np_)
+    spec.z_top[1] = (nz - 2 * np_) // 2
+    solver.set_medium_layered(spec)
+    solver.set_source(n // 2, n // 2, nz // 2, 25.0, (1.0, 1.0, 1.0, -1.0))
+    xs = np.linspace(np_, n - 1 - np_, 256).round().astype(np.int64)
+    idx = [int(x) + (n // 2) * n + (nz // 2) * n * n for x in xs] * 2
+    solver.set_receivers(["txx"] * 256 + ["vux"] * 256, idx)
+    info = solver.info()
+    solver.run(args.warmup)
+    solver.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    solver.run(args.steps)
+    ev1.record(stream)
+    solver.synchronize()
+    torch.cuda.synchronize()
+    ms_total = ev0.elapsed_time(ev1)
+    finite = bool(np.isfinite(solver.seismogram()).all())
+    solver.set_profiling(True)
+    solver.run(min(args.prof_steps, nt - solver.step))
+    ms_v, ms_t, launches = solver.kernel_times()
+    solver.close()
+    cells_total = float(n) * n * nz
+    owned = float(n) * n * planes
+    sec = ms_total / 1e3
+    avg_v, avg_t = ms_v / launches, ms_t / launches
+    if ms_v >= ms_t:
+        kname, kbytes, kms = "k_velocity", info["bytes_velocity"] * cells_total, avg_v
+    else:
+        kname, kbytes, kms = "k_stress", info["bytes_stress"] * cells_total, avg_t
+    achieved = kbytes / (kms / 1e3) / 1e9
+    # halo traffic a decomposed run would add per rank and step: after each of
+    # the two kernels, 2 fields per face leave and 2 arrive on each z face
+    state_b = {"fp32": 4, "fp64": 8, "comp16": 3, "comp16f": 3, "bf16": 2}[args.precision]
+    halo_bytes = 2 * 2 * 2 * n * n * state_b
+    print(json.dumps({
+        "metric": "rank share: Gcell-steps/s over owned cells", "value": round(owned * args.steps / sec / 1e9, 4),
+        "unit": "Gcell-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_total / args.steps, 4), "higher_is_better": True,
+        "config": {"workload": f"biot_two_layer_rank_share_{n}x{n}x{planes}+2_{args.precision}",
+                   "grid": [n, n, nz], "owned_planes": planes, "cpml_xy": np_, "cpml_open": ["z_lo", "z_hi"],
+                   "precision": args.precision},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": kbytes,
+                     "avg_launch_ms": round(kms, 4)},
+        "bytes_per_cell_step": round(info["bytes_per_cell_step"], 2),
+        "kernel_ms": {"k_velocity": round(avg_v, 4), "k_stress": round(avg_t, 4)},
+        "halo_bytes_per_step": halo_bytes, "finite": finite}), flush=True)
+    return 0
+
+
 def hbm_ceiling():
     """Run the standalone bandwidth probe built next to the library (plain copy
     and 2/6/12-read streaming kernels over 256 MiB arrays) as a child process,
@@ -303,6 +374,9 @@ def main():
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--round-steps", type=int, default=32)
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--rank-share", action="store_true",
+                    help="one interior rank's z-slab share of the workload (open z faces) instead")
+    ap.add_argument("--rank-planes", type=int, default=64)
     ap.add_argument("--hbm-ceiling", action="store_true",
                     help="run the streaming-kernel bandwidth probe (the practical HBM roof) instead")
     args = ap.parse_args()
@@ -311,6 +385,8 @@ def main():
         return 2
     if args.hbm_ceiling:
         return hbm_ceiling()
+    if args.rank_share:
+        return rank_share_main(args)
     if args.serving:
         return serving_main(args)
     if args.precision_report:

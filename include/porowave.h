@@ -103,7 +103,22 @@ typedef struct pw_config {
                                234-249) or 4 (9/8, -1/24; two-cell halo, the outermost
                                two layers are not updated; Courant limit 6/7).  4 runs
                                the two-kernel step with one cell per lane. */
+    int cpml_open;          /* faces without an absorbing layer, a bit mask
+                               (PW_FACE_*; 0, the default, damps all six as the
+                               reference does).  An open face keeps the interior
+                               profile (no damping, no memory variables) and its
+                               outermost plane is held like every edge plane: the
+                               ghost plane of one rank's z-slab in a decomposed
+                               grid.  No counterpart in the reference. */
 } pw_config;
+
+/* pw_config.cpml_open bits */
+#define PW_FACE_X_LO 1
+#define PW_FACE_X_HI 2
+#define PW_FACE_Y_LO 4
+#define PW_FACE_Y_HI 8
+#define PW_FACE_Z_LO 16
+#define PW_FACE_Z_HI 32
 
 /* Fills a config with the reference's values (Float/main.cu:20-23, 305-322):
  * 192^3 total, NP 32, H 0.01 m, DT 0.9 us, NT 3500, RC 1e-6, v0 4000, FP32. */

@@ -39,7 +39,8 @@ class Config(C.Structure):
                 ("h", C.c_double), ("dt", C.c_double), ("nt", C.c_int), ("precision", C.c_int),
                 ("cpml_r", C.c_double), ("cpml_v0", C.c_double),
                 ("half_es", C.c_int), ("half_dyn_end", C.c_double), ("half_dyn_interval", C.c_int),
-                ("device", C.c_int), ("stream", C.c_void_p), ("check_cfl", C.c_int), ("space_order", C.c_int)]
+                ("device", C.c_int), ("stream", C.c_void_p), ("check_cfl", C.c_int), ("space_order", C.c_int),
+                ("cpml_open", C.c_int)]
 
 
 class Info(C.Structure):
@@ -162,6 +163,21 @@ def lib():
     return _lib
 
 
+FACES = {"x_lo": 1, "x_hi": 2, "y_lo": 4, "y_hi": 8, "z_lo": 16, "z_hi": 32}
+
+
+def face_mask(faces):
+    """PW_FACE_* bit mask from face names (or an int mask, passed through)."""
+    if isinstance(faces, (int, np.integer)):
+        return int(faces)
+    m = 0
+    for f in faces:
+        if f not in FACES:
+            raise ValueError(f"unknown face {f!r}; faces are {sorted(FACES)}")
+        m |= FACES[f]
+    return m
+
+
 def _check(status):
     if status != 0:
         raise PorowaveError(status, lib().pw_last_error().decode(errors="replace"))
@@ -245,11 +261,14 @@ class Solver:
     nx, ny, nz are the total grid including the CPML (the reference's NX_ext).
     precision: "fp32" (Float path), "comp16" (Half path: FP16 + int8 residual),
     "bf16", or "fp64" (the FP64 oracle run).
+    cpml_open: faces without an absorbing layer, any of "x_lo", "x_hi", "y_lo",
+    "y_hi", "z_lo", "z_hi" (or the PW_FACE_* bit mask); e.g. ("z_lo", "z_hi")
+    is one interior rank's z-slab, whose outermost planes are its ghost planes.
     """
 
     def __init__(self, nx, ny, nz, np_, h, dt, nt, precision="fp32", cpml_r=1e-6, cpml_v0=4000.0,
                  device=0, check_cfl=True, half_es=0, half_dyn_end=16.0, half_dyn_interval=100, stream=None,
-                 space_order=2):
+                 space_order=2, cpml_open=0):
         L = lib()
         cfg = Config()
         L.pw_config_default(C.byref(cfg))
@@ -262,6 +281,7 @@ class Solver:
         cfg.stream = stream
         cfg.check_cfl = 1 if check_cfl else 0
         cfg.space_order = int(space_order)
+        cfg.cpml_open = face_mask(cpml_open)
         h_ = C.c_void_p()
         _check(L.pw_create(C.byref(cfg), C.byref(h_)))
         self._h = h_

@@ -41,15 +41,28 @@ bool env_int(const char* name, int* v)
     return true;
 }
 
+// Faces in pw_config.cpml_open get the interior profile (d = 0, e = 1) on
+// their half of the axis: no damping and, from slab_widths, no memory variables.
+template <class T> void open_faces(const pw_config& c, int ax, int n, T* d, T* d2, T* e, T* e2)
+{
+    const bool lo = (c.cpml_open >> (2 * ax)) & 1, hi = (c.cpml_open >> (2 * ax + 1)) & 1;
+    for (int i = 0; i < n; ++i)
+        if (i < n / 2 ? lo : hi) {
+            d[i] = d2[i] = (T)0;
+            e[i] = e2[i] = (T)1;
+        }
+}
+
 // CPML tables in the form the kernels consume: per index {c_full, e_full,
 // c_half, e_half} with c = dt*d/2.
-void build_tables(const pw_config& c, int prec, int n, std::vector<double>& out4, int* epml_out, float* isp_out,
-                  float* pml_max_out)
+void build_tables(const pw_config& c, int prec, int ax, int n, std::vector<double>& out4, int* epml_out,
+                  float* isp_out, float* pml_max_out)
 {
     out4.assign((size_t)n * 4, 0.0);
     if (prec == PW_FP64) {
         std::vector<double> d(n), d2(n), e(n), e2(n);
         cpml_profile_f64(n, c.np, c.h, c.dt, c.cpml_r, c.cpml_v0, d.data(), d2.data(), e.data(), e2.data());
+        open_faces(c, ax, n, d.data(), d2.data(), e.data(), e2.data());
         for (int i = 0; i < n; ++i) {
             out4[4 * i + 0] = c.dt * d[i] * 0.5;
             out4[4 * i + 1] = e[i];
@@ -62,6 +75,7 @@ void build_tables(const pw_config& c, int prec, int n, std::vector<double>& out4
     const float DT = (float)c.dt;
     cpml_profile_reference_f32(n, c.np, (float)c.h, DT, (float)c.cpml_r, (float)c.cpml_v0, d.data(), d2.data(),
                                e.data(), e2.data());
+    open_faces(c, ax, n, d.data(), d2.data(), e.data(), e2.data());
     if (prec == PW_FP32 || prec == PW_COMP16F) {
         for (int i = 0; i < n; ++i) {
             out4[4 * i + 0] = (float)(DT * d[i]) * 0.5f;
@@ -145,6 +159,8 @@ pw_status check_config(const pw_config& c)
         return fail(PW_ERR_INVALID, "pw_create: cpml_r must be in (0,1) and cpml_v0 positive");
     if (c.space_order != 0 && c.space_order != 2 && c.space_order != 4)
         return fail(PW_ERR_INVALID, "pw_create: space_order must be 2 or 4");
+    if (c.cpml_open < 0 || c.cpml_open > 63)
+        return fail(PW_ERR_INVALID, "pw_create: cpml_open must be a mask of the six PW_FACE_* bits");
     if (c.space_order == 4 && (c.nx < 2 * c.np + 5 || c.ny < 2 * c.np + 5 || c.nz < 2 * c.np + 5))
         return fail(PW_ERR_INVALID, "pw_create: 4th order needs at least 2*np+5 cells per axis");
     const unsigned long long pitch = (unsigned long long)(c.nx + 63) / 64 * 64;
@@ -180,7 +196,7 @@ void plan_layout(const pw_config& c, int fused, Layout& L)
     if (is_scaled(prec)) {
         float pmax = 0.0f, pm;
         for (int ax = 0; ax < 3; ++ax) {
-            build_tables(c, prec, n3[ax], L.tab[ax], &L.epml, &L.isp, &pm);
+            build_tables(c, prec, ax, n3[ax], L.tab[ax], &L.epml, &L.isp, &pm);
             pmax = std::max(pmax, pm);
         }
         // Half/main.cu:833-839
@@ -191,7 +207,7 @@ void plan_layout(const pw_config& c, int fused, Layout& L)
     }
     for (int ax = 0; ax < 3; ++ax) {
         float pm;
-        build_tables(c, prec, n3[ax], L.tab[ax], &L.epml, &L.isp, &pm);
+        build_tables(c, prec, ax, n3[ax], L.tab[ax], &L.epml, &L.isp, &pm);
         slab_widths(L.tab[ax], n3[ax], &g.lo[ax], &g.hi[ax]);
     }
     const long long W = pitch;

@@ -150,6 +150,23 @@ void or_pml_profile_f64(int n, int np, double H, double DT, double RC, double v0
     }
 }
 
+/* Faces listed in or_args.pml_open: the interior profile on that half of the
+ * axis, i.e. what the reference profile gives every cell outside its layers. */
+static void open_faces_f32(int mask, int axis, int n, float* d, float* d2, float* e, float* e2)
+{
+    const int lo = (mask >> (2 * axis)) & 1, hi = (mask >> (2 * axis + 1)) & 1;
+    int i;
+    for (i = 0; i < n; i++)
+        if (i < n / 2 ? lo : hi) { d[i] = 0.0f; d2[i] = 0.0f; e[i] = 1.0f; e2[i] = 1.0f; }
+}
+static void open_faces_f64(int mask, int axis, int n, double* d, double* d2, double* e, double* e2)
+{
+    const int lo = (mask >> (2 * axis)) & 1, hi = (mask >> (2 * axis + 1)) & 1;
+    int i;
+    for (i = 0; i < n; i++)
+        if (i < n / 2 ? lo : hi) { d[i] = 0.0; d2[i] = 0.0; e[i] = 1.0; e2[i] = 1.0; }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Ricker, Float/main.cu:1050, 1059                                           */
 /* ------------------------------------------------------------------------- */
@@ -191,6 +208,7 @@ static int check_args(const or_args* a)
     if (a->nx - 2 * a->np < 1 || a->ny - 2 * a->np < 1 || a->nz - 2 * a->np < 1) return 1;
     if (a->nt < 0 || a->h <= 0 || a->dt <= 0) return 1;
     if (a->order != 0 && a->order != 2 && a->order != 4) return 1;
+    if (a->pml_open < 0 || a->pml_open > 63) return 1;
     if (a->sx < 0 || a->sx >= a->nx || a->sy < 0 || a->sy >= a->ny || a->sz < 0 || a->sz >= a->nz) return 1;
     if (a->nrec < 0 || (a->nrec > 0 && (!a->rec_field || !a->rec_index || !a->seis))) return 1;
     if (a->nsnap < 0 || (a->nsnap > 0 && (!a->snap_step || !a->snap_field || !a->snaps))) return 1;
@@ -294,6 +312,9 @@ static void precompute_f32_into(const or_args* a, const float* base, f32_state* 
     or_pml_profile_f32(nx, np, (float)a->h, (float)a->dt, (float)a->rc, (float)a->v0, s->dxi, s->dxi2, s->exi, s->exi2);
     or_pml_profile_f32(ny, np, (float)a->h, (float)a->dt, (float)a->rc, (float)a->v0, s->dyj, s->dyj2, s->eyj, s->eyj2);
     or_pml_profile_f32(nz, np, (float)a->h, (float)a->dt, (float)a->rc, (float)a->v0, s->dzk, s->dzk2, s->ezk, s->ezk2);
+    open_faces_f32(a->pml_open, 0, nx, s->dxi, s->dxi2, s->exi, s->exi2);
+    open_faces_f32(a->pml_open, 1, ny, s->dyj, s->dyj2, s->eyj, s->eyj2);
+    open_faces_f32(a->pml_open, 2, nz, s->dzk, s->dzk2, s->ezk, s->ezk2);
 }
 
 int or_precompute_f32(const or_args* a, const float* base, float* out17)
@@ -421,6 +442,9 @@ int or_run_f64(or_args* a, const double* base)
     or_pml_profile_f64(nx, np, a->h, a->dt, a->rc, a->v0, s.dxi, s.dxi2, s.exi, s.exi2);
     or_pml_profile_f64(ny, np, a->h, a->dt, a->rc, a->v0, s.dyj, s.dyj2, s.eyj, s.eyj2);
     or_pml_profile_f64(nz, np, a->h, a->dt, a->rc, a->v0, s.dzk, s.dzk2, s.ezk, s.ezk2);
+    open_faces_f64(a->pml_open, 0, nx, s.dxi, s.dxi2, s.exi, s.exi2);
+    open_faces_f64(a->pml_open, 1, ny, s.dyj, s.dyj2, s.eyj, s.eyj2);
+    open_faces_f64(a->pml_open, 2, nz, s.dzk, s.dzk2, s.ezk, s.ezk2);
     stf = (double*)malloc(sizeof(double) * (a->nt > 0 ? a->nt : 1));
     for (it = 0; it < a->nt; it++) stf[it] = or_ricker_f64(it, a->dt, a->f0);
     f64_time_loop(&s, a, a->h, a->dt, stf);

@@ -70,6 +70,10 @@ typedef struct {
     int half_em_out, half_epml_out;  /* out: chosen exponents */
     int order;               /* spatial order of the staggered differences: 2 (the
                                 reference; 0 means 2) or 4 (FP32/FP64 runs only) */
+    int pml_open;            /* faces without an absorbing layer, bit 2*axis (low
+                                face) and 2*axis+1 (high face), axis 0 = x: the
+                                interior profile (d = 0, e = 1) on that half of the
+                                axis.  0: the reference (all six faces damped) */
 } or_args;
 
 /* create_model (Float/main.cu:1590-1655): the borehole-in-formation model.

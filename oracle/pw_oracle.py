@@ -31,7 +31,7 @@ class OrArgs(C.Structure):
         ("pml_everywhere", C.c_int), ("nthreads", C.c_int),
         ("half_es", C.c_int), ("half_dyn_end", C.c_double), ("half_dyn_interval", C.c_int),
         ("half_em_out", C.c_int), ("half_epml_out", C.c_int),
-        ("order", C.c_int),
+        ("order", C.c_int), ("pml_open", C.c_int),
     ]
 
 
@@ -186,6 +186,7 @@ def _args(grid, nt, receivers, snapshots, snap_y, final, pml_everywhere=False, n
     a.half_dyn_end = float(half.get("dyn_end", 16.0))
     a.half_dyn_interval = int(half.get("dyn_interval", 100))
     a.order = int(grid.get("order", 2))
+    a.pml_open = int(grid.get("open", 0))
     return a, keep
 
 
