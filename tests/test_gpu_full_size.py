@@ -1,0 +1,109 @@
+"""The benchmark workloads at their full sizes, checked through properties that
+need no CPU reference (the oracle cannot run these sizes in test time; it is
+compared with the GPU at smaller sizes in the other GPU tests):
+
+* 512^3, the two-layer perturbed workload of the benchmark (FP32, comp16);
+* 1024^3 with 8 random layers, NP 32, dx 4 m, dt 0.4 ms, 20 Hz: the largest
+  workload (about 100 GB of FP32 state, coefficients and memory variables).
+
+Properties:
+* linearity under a power-of-two source gain: scaling every input of a linear
+  update by 2 scales every rounded intermediate by 2 exactly, except where a
+  value falls below the normal range; so the seismograms double, to 1e-6
+  relative L2 (FP32) and bit-exactly on nearly every sample;
+* a run split into chunks is bit-identical to one run (the step counter, the
+  source time function and the receiver slots continue across calls);
+* seismograms are finite and carry signal.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mixedprecisionfaracoustic_amd as pw
+from cases import rel_l2
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import bench  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def linearity(a, b, tol):
+    """b is the run with twice the source: b == 2a to tol, mostly bit-exactly."""
+    assert np.isfinite(a).all() and np.isfinite(b).all()
+    assert np.abs(a).max() > 0
+    e = rel_l2(b, 2.0 * a)
+    exact = float(np.mean(b == 2.0 * a))
+    print(f"linearity: rel-L2 {e:.2e}, bit-exact samples {exact:.4f}")
+    assert e <= tol
+    return exact
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("comp16", 1e-5)])
+def test_512_workload_linear_and_chunk_exact(prec, tol):
+    n, np_, nt = 512, 20, 100
+    s = pw.Solver(n, n, n, np_, 5.0, 0.5e-3, nt, precision=prec, cpml_r=1e-4, cpml_v0=4000.0)
+    s.set_medium_layered(bench.workload(n, np_))
+    fields, idx = bench.receivers(n, np_)
+    # plus a line through the source, which the wave reaches within 100 steps
+    near = [int(x) + (n // 2) * n + (n // 2) * n * n for x in range(n // 2 - 40, n // 2 + 41, 4)]
+    s.set_receivers(fields + ["txx"] * len(near) + ["vux"] * len(near), idx + near + near)
+    w = np.array([1.0, 1.0, 1.0, -1.0])
+    s.set_source(n // 2, n // 2, n // 2, 25.0, w)
+    s.run(40)
+    s.run(60)
+    chunked = s.seismogram()
+    s.reset()
+    s.run(nt)
+    whole = s.seismogram()
+    assert np.array_equal(chunked, whole)
+    s.reset()
+    s.set_source(n // 2, n // 2, n // 2, 25.0, 2.0 * w)
+    s.run(nt)
+    doubled = s.seismogram()
+    s.close()
+    exact = linearity(whole, doubled, tol)
+    if prec == "fp32":
+        assert exact > 0.99
+
+
+def config3_layers(nz, np_, seed=3):
+    """8 horizontal layers: porosity U[0.05, 0.3], permeability log-uniform
+    [1e-15, 1e-12] m^2, dry moduli from a porosity trend (stiffer when less
+    porous), layer tops evenly spread with a seeded jitter."""
+    rng = np.random.default_rng(seed)
+    tops = [0] + sorted(int(v) for v in rng.choice(np.arange(np_ + 40, nz - np_ - 40), 7, replace=False))
+    rocks = []
+    for _ in range(8):
+        phi = float(rng.uniform(0.05, 0.3))
+        kb = float(35e9 * (1 - phi / 0.4) ** 2 + 2e9)
+        mu = 0.8 * kb
+        rocks.append(pw.rock(Ks=35e9, Kb=kb, mu=mu, Kf=2.25e9, rhos=2650.0, rhof=1000.0, phi=phi,
+                             kappa=float(10 ** rng.uniform(-15, -12)), eta=1e-3, tortuosity=2.0, pore_m=8.0))
+    return pw.layered(tops, rocks, perturb=0.05, corr_cells=8.0, clamp=3.0, seed=seed)
+
+
+def test_1024_workload_runs_linear():
+    n, np_, nt = 1024, 32, 60
+    s = pw.Solver(n, n, n, np_, 4.0, 0.4e-3, nt, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
+    info = s.info()
+    print(f"1024^3 FP32: {info['device_bytes'] / 1e9:.1f} GB on the device, "
+          f"{info['bytes_per_cell_step']:.1f} B per cell-step")
+    assert info["device_bytes"] < 150e9
+    s.set_medium_layered(config3_layers(n, np_))
+    c = n // 2
+    near = [x + c * n + c * n * n for x in range(c - 24, c + 25, 2)]
+    s.set_receivers(["txx"] * len(near) + ["vux"] * len(near) + ["p"] * len(near), near * 3)
+    w = np.array([1.0, 1.0, 1.0, -1.0])
+    s.set_source(c, c, c, 20.0, w)
+    s.run(nt)
+    once = s.seismogram()
+    s.reset()
+    s.set_source(c, c, c, 20.0, 2.0 * w)
+    s.run(25)
+    s.run(nt - 25)
+    doubled = s.seismogram()
+    s.close()
+    linearity(once, doubled, 1e-6)
