@@ -1,0 +1,101 @@
+"""Edge cases on the GPU, against the oracle where there is a result to compare.
+
+* Ragged and minimal grids: the smallest grid the CPML allows (2 np + 3 per
+  axis), and odd sizes just past a tile edge (67 or 130 cells in x: 3 or 2 live
+  lanes in the last 64-lane tile; 9 or 66 rows in y against 4-row tiles; z
+  chunks that do not divide the updated planes), on a heterogeneous medium.
+  FP32 against the Float oracle (same rule as the other parity tests), FP64
+  against the FP64 oracle (1e-10), comp16 against the Half path (1e-3, floor
+  rule).
+* A source in the CPML corner cell (1, 1, 1).
+* Receivers on the edge planes, which are never updated: their traces stay 0.
+* No receivers; a zero-step run; running past nt is an error that leaves the
+  handle's state unchanged.
+"""
+import numpy as np
+import pytest
+
+import cases
+import mixedprecisionfaracoustic_amd as pw
+from cases import rel_l2
+from test_gpu_parity import FIELDS, check, fp32_floor
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    ((7, 7, 7), 2),      # minimum: 2 np + 3 on every axis, a 3^3 interior
+    ((67, 9, 8), 2),     # 3 live lanes in the second x tile, ragged y rows
+    ((130, 7, 23), 2),   # 2 live lanes in the third x tile, ragged z chunks
+    ((9, 66, 11), 3),    # one partial x tile, 17 y tiles with the last half full
+]
+
+
+def hetero_case(shape, np_, nt=60, src=None, seed=0):
+    nx, ny, nz = shape
+    grid = {"nx": nx, "ny": ny, "nz": nz, "np": np_, "h": 5.0, "dt": 0.5e-3, "rc": 1e-4, "v0": 4000.0,
+            "f0": 30.0, "src": src or (nx // 2, ny // 2, nz // 2), "src_w": (1.0, 1.0, 1.0, -1.0)}
+    b = pw.biot_from_rock(cases.sandstone())
+    rng = np.random.default_rng(seed)
+    shape_i = (nz - 2 * np_, ny - 2 * np_, nx - 2 * np_)
+    base = b[:, None, None, None] * (1.0 + 0.05 * rng.uniform(-1.0, 1.0, (9,) + shape_i))
+    return grid, base, nt
+
+
+def line_receivers(grid):
+    """Every cell of the x line through the source, edge cells included, for
+    the five reference channels."""
+    nx, ny = grid["nx"], grid["ny"]
+    sx, sy, sz = grid["src"]
+    return [(f, x + sy * nx + sz * nx * ny) for f in ("txx", "tzz", "vux", "vwx", "p") for x in range(nx)]
+
+
+@pytest.mark.parametrize("shape,np_", SHAPES)
+def test_ragged_grids_match_oracle(shape, np_):
+    grid, base, nt = hetero_case(shape, np_)
+    recs = line_receivers(grid)
+    g32, _ = cases.run_gpu(grid, nt, "fp32", recs, base=base, fields=FIELDS)
+    ref32 = cases.run_oracle("f32", grid, nt, recs, base=base, final=True)
+    floor, ref64 = fp32_floor(grid, nt, recs, base, ref32)
+    check(g32, ref32, recs, 1e-5, f"fp32 {shape}", floor=floor)
+    g64, _ = cases.run_gpu(grid, nt, "fp64", recs, base=base, fields=FIELDS)
+    check(g64, ref64, recs, 1e-10, f"fp64 {shape}")
+    refh = cases.run_oracle("half", grid, nt, recs, base=base, final=True)
+    floor_h, _ = fp32_floor(grid, nt, recs, base, refh, "Half")
+    g16, _ = cases.run_gpu(grid, nt, "comp16", recs, base=base, fields=FIELDS)
+    check(g16, refh, recs, 1e-3, f"comp16 {shape} vs Half", floor=floor_h)
+    # edge cells are never updated: their traces are exactly zero
+    nx = grid["nx"]
+    for k, (f, i) in enumerate(recs):
+        if i % nx in (0, nx - 1):
+            assert not g32["seis"][k].any(), (f, i)
+
+
+def test_source_in_cpml_corner_matches_oracle():
+    grid, base, nt = hetero_case((20, 18, 22), 4, nt=80, src=(1, 1, 1))
+    recs = [(f, x + 1 * 20 + 1 * 20 * 18) for f in ("txx", "vux", "p") for x in range(20)]
+    recs += [(f, 10 + 9 * 20 + 11 * 20 * 18) for f in ("txx", "vux", "p")]
+    g32, _ = cases.run_gpu(grid, nt, "fp32", recs, base=base, fields=FIELDS)
+    ref32 = cases.run_oracle("f32", grid, nt, recs, base=base, final=True)
+    floor, _ = fp32_floor(grid, nt, recs, base, ref32)
+    check(g32, ref32, recs, 1e-5, "fp32 corner source", floor=floor)
+
+
+def test_no_receivers_zero_steps_and_overrun():
+    grid, base, nt = hetero_case((40, 36, 44), 6, nt=50)
+    s = cases.make_solver(grid, nt, "fp32", base=base)
+    s.run(0)
+    assert s.step == 0
+    assert not any(s.field(f).any() for f in FIELDS)
+    s.run(nt)
+    ref = cases.run_oracle("f32", grid, nt, [], base=base, final=True)
+    for f in ("txx", "vux", "p"):
+        e = rel_l2(s.field(f), ref["final"][cases.O.FIELD_ID[f]])
+        print(f"no receivers, field {f}: rel-L2 {e:.2e}")
+        assert e <= 1e-5
+    before = s.field("txx").copy()
+    with pytest.raises(pw.PorowaveError):
+        s.run(1)
+    assert s.step == nt
+    assert np.array_equal(s.field("txx"), before)
+    assert s.seismogram().size == 0
+    s.close()
