@@ -14,6 +14,8 @@ LIB = os.path.join(HERE, "libporowave.so")
 OBJDIR = os.path.join(HERE, "_build")
 # standalone HBM bandwidth probe (bench.py --hbm-ceiling), not part of the library
 PROBE = os.path.join(HERE, "pw_hbm_probe")
+# command-line replacement for the reference program's main(), linked to the library
+CLI = os.path.join(HERE, "porowave")
 SOURCES = ["pw_api.cpp", "pw_medium.cpp", "pw_server.cpp", "pw_output.cpp", "pw_kernels.hip", "pw_medium_gpu.hip"]
 ARCH = os.environ.get("PW_OFFLOAD_ARCH", "gfx950")
 
@@ -77,6 +79,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    cmd = [hipcc(), "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"), os.path.join(CSRC, "pw_cli.cpp"),
+           "-o", CLI + ".tmp", "-L", HERE, "-lporowave", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on pw_cli.cpp:\n{r.stdout}\n{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

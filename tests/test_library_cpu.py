@@ -150,3 +150,22 @@ def test_binary_files_round_trip(tmp_path):
     with pytest.raises(pw.PorowaveError) as ei:
         pw.read_bin(str(q))
     assert ei.value.status == 7  # PW_ERR_IO
+
+
+def test_cli_usage_and_argument_errors():
+    """The porowave command line (the reference main()'s replacement) parses its
+    options before touching a device: --help succeeds, bad values exit 2, and
+    without a GPU pw_create's error comes back as exit 1 with its text."""
+    import subprocess
+    cli = os.path.join(ROOT, "mixedprecisionfaracoustic_amd", "porowave")
+    if not os.path.exists(cli):
+        pytest.skip("porowave not built")
+    r = subprocess.run([cli, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "--precision" in r.stdout
+    for bad in (["--precision", "fp8"], ["--format", "csv"], ["--nt", "0"], ["--n", "x"], ["--bogus", "1"], ["--nt"]):
+        r = subprocess.run([cli] + bad, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, (bad, r.stderr)
+    import torch
+    if not torch.cuda.is_available():
+        r = subprocess.run([cli, "--nt", "2", "--quiet"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 1 and "pw_create" in r.stderr
