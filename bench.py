@@ -374,6 +374,8 @@ def main():
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--round-steps", type=int, default=32)
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--volume-every", type=int, default=0,
+                    help="also take a full 3-D txx snapshot every K steps (asynchronous copy to the host)")
     ap.add_argument("--rank-share", action="store_true",
                     help="one interior rank's z-slab share of the workload (open z faces) instead")
     ap.add_argument("--rank-planes", type=int, default=64)
@@ -408,6 +410,10 @@ def main():
     solver.set_receivers(fields, idx)
     snap_steps = [s for s in range(500, nt, 500)] or [nt - 1]
     solver.set_snapshots(snap_steps, ["txx"] * len(snap_steps), n // 2)
+    vol_steps = list(range(args.volume_every - 1, nt, args.volume_every)) if args.volume_every > 0 else []
+    if vol_steps:
+        # full 3-D txx volumes, copied to pinned host memory while the run goes on
+        solver.set_volume_snapshots(vol_steps, ["txx"] * len(vol_steps))
     info = solver.info()
 
     # warm-up (untimed)
@@ -483,6 +489,10 @@ def main():
         "wall_s": round(wall, 3),
         "finite": finite,
     }
+    if vol_steps:
+        timed = [v for v in vol_steps if args.warmup <= v < args.warmup + args.steps]
+        out["volume_snapshots"] = {"every": args.volume_every, "in_timed_steps": len(timed),
+                                   "bytes_each": int(cells * 4)}
     solver.close()
     if not args.no_cpu_baseline:
         base = None

@@ -187,6 +187,19 @@ pw_status pw_get_snapshots(pw_solver* s, float* host);
 /* Copies out one full wavefield decoded to physical FP32: nz*ny*nx floats. */
 pw_status pw_get_field(pw_solver* s, int field, float* host);
 
+/* Full 3-D snapshots without stalling the run (no reference counterpart: the
+ * reference copies whole arrays to the host synchronously inside its loop,
+ * Float/main.cu:1075-1081, and keeps y-slices only).  After step step[k] the
+ * field is decoded to physical FP32 on the handle's stream into one of two
+ * device staging volumes, and copied to library-owned pinned host memory on a
+ * second stream while the following steps run.  Replaces any earlier set; n = 0
+ * clears it.  Holds n * nz*ny*nx floats of pinned host memory and two staging
+ * volumes on the device. */
+pw_status pw_set_volume_snapshots(pw_solver* s, int n, const int* step, const int* field);
+/* Waits for volume snapshot k's copy and copies it out: nz*ny*nx floats.
+ * PW_ERR_STATE if its step has not run yet. */
+pw_status pw_get_volume_snapshot(pw_solver* s, int k, float* host);
+
 /* Replaces wfile / wfile3d (Float/main.cu:1501-1516, 1563-1588): writes nr rows
  * of nc values as "%e " text, each row followed by "\n" and a blank line. */
 pw_status pw_write_dat(const char* path, const float* data, int nr, int nc);

@@ -117,6 +117,8 @@ _SIGS = {
     "pw_get_seismogram": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "pw_get_snapshots": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "pw_get_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float)]),
+    "pw_set_volume_snapshots": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "pw_get_volume_snapshot": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float)]),
     "pw_write_dat": (C.c_int, [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int]),
     "pw_get_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
     "pw_medium_borehole": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
@@ -359,6 +361,20 @@ class Solver:
         _check(lib().pw_set_snapshots(self._h, len(s), s.ctypes.data_as(C.POINTER(C.c_int)),
                                       f.ctypes.data_as(C.POINTER(C.c_int)), y))
         self.nsnap = len(s)
+
+    def set_volume_snapshots(self, steps, fields):
+        """Full 3-D snapshots after the given steps, copied to pinned host memory
+        on a second stream while the run goes on (volume_snapshot(k) reads one)."""
+        s = np.ascontiguousarray(steps, np.int32)
+        f = np.ascontiguousarray([FIELD_ID[x] if isinstance(x, str) else int(x) for x in fields], np.int32)
+        _check(lib().pw_set_volume_snapshots(self._h, len(s), s.ctypes.data_as(C.POINTER(C.c_int)),
+                                             f.ctypes.data_as(C.POINTER(C.c_int))))
+        self.nvol = len(s)
+
+    def volume_snapshot(self, k):
+        out = np.zeros(self.shape, np.float32)
+        _check(lib().pw_get_volume_snapshot(self._h, int(k), _fp(out)))
+        return out
 
     # -- stepping -----------------------------------------------------------
     def run(self, nsteps):

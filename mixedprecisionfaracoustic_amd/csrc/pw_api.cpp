@@ -372,10 +372,40 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     return PW_OK;
 }
 
+namespace {
+
+// Releases the volume-snapshot buffers, events and copy stream's work (after
+// both streams have drained).
+void free_volumes(pw_solver* s)
+{
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
+    for (float* p : s->vol_host) (void)hipHostFree(p);
+    for (hipEvent_t e : s->vol_done) (void)hipEventDestroy(e);
+    for (int j = 0; j < 2; ++j) {
+        (void)hipFree(s->d_vstage[j]);
+        if (s->ev_decoded[j]) (void)hipEventDestroy(s->ev_decoded[j]);
+        if (s->ev_copied[j]) (void)hipEventDestroy(s->ev_copied[j]);
+        s->d_vstage[j] = nullptr;
+        s->ev_decoded[j] = s->ev_copied[j] = nullptr;
+    }
+    s->vol_host.clear();
+    s->vol_done.clear();
+    s->vol_taken.clear();
+    s->vol_step.clear();
+    s->vol_field.clear();
+    s->nvol = 0;
+    s->vol_next = 0;
+}
+
+}  // namespace
+
 extern "C" void pw_destroy(pw_solver* s)
 {
     if (!s) return;
     (void)hipSetDevice(s->cfg.device);
+    free_volumes(s);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     (void)hipFree(s->d_rfield);
     (void)hipFree(s->d_roff);
@@ -677,6 +707,8 @@ extern "C" pw_status pw_reset(pw_solver* s)
     if (s->d_snaps)
         PW_HIP_TRY(launch_zero(s->d_snaps, align256((size_t)s->nsnap * s->cfg.nz * s->cfg.nx * 4), s->stream));
     PW_HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->copy_stream) PW_HIP_TRY(hipStreamSynchronize(s->copy_stream));
+    std::fill(s->vol_taken.begin(), s->vol_taken.end(), 0);
     s->step = 0;
     s->dyn_scale = 1.0f;
     return PW_OK;
@@ -802,6 +834,23 @@ extern "C" pw_status pw_run(pw_solver* s, int nsteps)
                 PW_HIP_TRY(launch_snapshot(s->b, s->stream, s->snap_field[q], s->snap_y,
                                            s->d_snaps + (size_t)q * c.nz * c.nx, sc));
             }
+        for (int q = 0; q < s->nvol; ++q)
+            if (s->vol_step[q] == it) {
+                const float sc = s->vol_field[q] <= PW_VWZ ? sv : ss;
+                const size_t vbytes = (size_t)c.nx * c.ny * c.nz * sizeof(float);
+                const int j = s->vol_next;
+                s->vol_next ^= 1;
+                // the staging volume's previous copy must be done before it is overwritten
+                PW_HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_copied[j], 0));
+                PW_HIP_TRY(launch_decode(s->b, s->stream, s->vol_field[q], s->d_vstage[j], sc));
+                PW_HIP_TRY(hipEventRecord(s->ev_decoded[j], s->stream));
+                PW_HIP_TRY(hipStreamWaitEvent(s->copy_stream, s->ev_decoded[j], 0));
+                PW_HIP_TRY(hipMemcpyAsync(s->vol_host[q], s->d_vstage[j], vbytes, hipMemcpyDeviceToHost,
+                                          s->copy_stream));
+                PW_HIP_TRY(hipEventRecord(s->ev_copied[j], s->copy_stream));
+                PW_HIP_TRY(hipEventRecord(s->vol_done[q], s->copy_stream));
+                s->vol_taken[q] = 1;
+            }
         s->step++;
     }
     return PW_OK;
@@ -840,6 +889,62 @@ extern "C" pw_status pw_get_snapshots(pw_solver* s, float* host)
     PW_HIP_TRY(hipStreamSynchronize(s->stream));
     PW_HIP_TRY(hipMemcpy(host, s->d_snaps, (size_t)s->nsnap * s->cfg.nz * s->cfg.nx * sizeof(float),
                          hipMemcpyDeviceToHost));
+    return PW_OK;
+}
+
+extern "C" pw_status pw_set_volume_snapshots(pw_solver* s, int n, const int* step, const int* field)
+{
+    pw_status st = check_handle(s, "pw_set_volume_snapshots");
+    if (st != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    if (n < 0 || (n > 0 && (!step || !field))) return fail(PW_ERR_INVALID, "pw_set_volume_snapshots: bad arguments");
+    for (int k = 0; k < n; ++k)
+        if (step[k] < 0 || step[k] >= c.nt || field[k] < 0 || field[k] >= PW_NFIELDS)
+            return fail(PW_ERR_INVALID, "pw_set_volume_snapshots: step or field out of range");
+    PW_HIP_TRY(hipSetDevice(c.device));
+    free_volumes(s);
+    if (n == 0) return PW_OK;
+    const size_t vbytes = (size_t)c.nx * c.ny * c.nz * sizeof(float);
+    hipError_t e = hipSuccess;
+    if (!s->copy_stream) e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking);
+    for (int k = 0; k < n && e == hipSuccess; ++k) {
+        float* p = nullptr;
+        e = hipHostMalloc((void**)&p, vbytes, hipHostMallocDefault);
+        if (e != hipSuccess) break;
+        s->vol_host.push_back(p);
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) s->vol_done.push_back(ev);
+    }
+    for (int j = 0; j < 2 && e == hipSuccess; ++j) {
+        e = hipMalloc((void**)&s->d_vstage[j], vbytes);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_decoded[j], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_copied[j], hipEventDisableTiming);
+        // recorded once so that the first wait on it passes
+        if (e == hipSuccess) e = hipEventRecord(s->ev_copied[j], s->copy_stream);
+    }
+    if (e != hipSuccess) {
+        free_volumes(s);
+        (void)hipGetLastError();
+        return fail(PW_ERR_OUT_OF_MEMORY, std::string("pw_set_volume_snapshots: ") + hipGetErrorString(e));
+    }
+    s->vol_step.assign(step, step + n);
+    s->vol_field.assign(field, field + n);
+    s->vol_taken.assign(n, 0);
+    s->nvol = n;
+    return PW_OK;
+}
+
+extern "C" pw_status pw_get_volume_snapshot(pw_solver* s, int k, float* host)
+{
+    pw_status st = check_handle(s, "pw_get_volume_snapshot");
+    if (st != PW_OK) return st;
+    if (k < 0 || k >= s->nvol || !host) return fail(PW_ERR_INVALID, "pw_get_volume_snapshot: bad arguments");
+    if (!s->vol_taken[k]) return fail(PW_ERR_STATE, "pw_get_volume_snapshot: its step has not run yet");
+    const pw_config& c = s->cfg;
+    PW_HIP_TRY(hipSetDevice(c.device));
+    PW_HIP_TRY(hipEventSynchronize(s->vol_done[k]));
+    std::memcpy(host, s->vol_host[k], (size_t)c.nx * c.ny * c.nz * sizeof(float));
     return PW_OK;
 }
 

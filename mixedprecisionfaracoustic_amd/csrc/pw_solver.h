@@ -40,6 +40,18 @@ struct pw_solver {
     int nsnap = 0, snap_y = 0;
     std::vector<int> snap_step, snap_field;
     float* d_snaps = nullptr;
+    // full-volume snapshots (pw_set_volume_snapshots): decoded on the compute
+    // stream into one of two device staging volumes, then copied to pinned host
+    // memory on copy_stream while the steps go on
+    int nvol = 0;
+    std::vector<int> vol_step, vol_field;
+    std::vector<float*> vol_host;        // hipHostMalloc'ed, one volume each
+    std::vector<hipEvent_t> vol_done;    // recorded on copy_stream after each copy
+    std::vector<char> vol_taken;         // the snapshot's step has run
+    float* d_vstage[2] = {nullptr, nullptr};
+    hipEvent_t ev_decoded[2] = {nullptr, nullptr}, ev_copied[2] = {nullptr, nullptr};
+    int vol_next = 0;
+    hipStream_t copy_stream = nullptr;
     int step = 0;
     // per-kernel timing (pw_set_profiling)
     bool profiling = false;
