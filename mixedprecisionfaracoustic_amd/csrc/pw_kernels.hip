@@ -361,15 +361,39 @@ __device__ __forceinline__ Tile tile_of_block(int gx, int gy, int nblocks, int x
     return tile_of_index(xcd_remap(blockIdx.x, (unsigned)nblocks, xcd_map), gx, gy);
 }
 
+// One CPML term in single-variable form (see header comment), its memory
+// variable m already loaded from phi[k].
+template <class P>
+__device__ __forceinline__ typename P::R cpml_with(typename P::Mem* phi, unsigned k, typename P::R m, typename P::R c,
+                                                   typename P::R e, typename P::R der, unsigned z)
+{
+    using R = typename P::R;
+    const R psi = m - c * der;
+    P::stm(phi, k, e * (psi - c * der), z);
+    return der + psi;
+}
+
+// The four CPML terms of one damped axis.  All four memory variables are loaded
+// before any is stored: a store could alias the next load as far as the
+// compiler knows, so term-by-term code would make four memory round trips.
+template <class P>
+__device__ __forceinline__ void cpml4(typename P::Mem* const* phi, unsigned k, const typename P::R (&c)[4],
+                                      const typename P::R (&e)[4], typename P::R* const (&der)[4], unsigned z)
+{
+    using R = typename P::R;
+    R m[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m[q] = P::ldm(phi[q], k);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) *der[q] = cpml_with<P>(phi[q], k, m[q], c[q], e[q], *der[q], z);
+}
+
 // One CPML term in single-variable form (see header comment).
 template <class P>
 __device__ __forceinline__ typename P::R cpml(typename P::Mem* phi, unsigned k, typename P::R c, typename P::R e,
                                               typename P::R der, unsigned z)
 {
-    using R = typename P::R;
-    const R psi = P::ldm(phi, k) - c * der;
-    P::stm(phi, k, e * (psi - c * der), z);
-    return der + psi;
+    return cpml_with<P>(phi, k, P::ldm(phi, k), c, e, der, z);
 }
 
 // --------------------------------------------------------------------------
@@ -608,6 +632,18 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             tzz_m[0] = tzz_c[0]; tzz_c[0] = tzz_k[0]; tzz_k[0] = tzz_nn;
             p_m[0] = p_c[0]; p_c[0] = p_k[0]; p_k[0] = p_nn;
         }
+        // The old velocities are loaded before the CPML memory-variable updates
+        // below: those stores could alias them as far as the compiler knows, so
+        // loads placed after the stores would wait for a second memory round trip
+        // per plane.  (Hoisting the coefficients as well costs a wave per SIMD in
+        // registers and measured slower.)
+        R wx0[N], wy0[N], wz0[N], ux0[N], uy0[N], uz0[N];
+        LD(PW_VWX, o, wx0);
+        LD(PW_VWY, o, wy0);
+        LD(PW_VWZ, o, wz0);
+        LD(PW_VUX, o, ux0);
+        LD(PW_VUY, o, uy0);
+        LD(PW_VUZ, o, uz0);
         const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
         Tab4<R> tz = {0, 1, 0, 1};
         if (zs >= 0) tz = a.tab[2][iz];
@@ -621,27 +657,22 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             R z1_ = D(dz1[j]), z2 = D(dz2[j]), z3 = D(dz3[j]);
             R q1 = D(ds1[j]), q2 = D(ds2[j]), q3 = D(ds3[j]);
             const unsigned ix = (unsigned)(ix0 + j);
+            // terms 0, 1 on the half-point profile, 2, 3 on the full one
             if (xs[j] >= 0) {
                 const Tab4<R> tx = a.tab[0][ix];
                 const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs[j];
-                x1 = cpml<P>(a.m[0][0], k, tx.ch, tx.eh, x1, a.zero);
-                q1 = cpml<P>(a.m[0][1], k, tx.ch, tx.eh, q1, a.zero);
-                y2 = cpml<P>(a.m[0][2], k, tx.cf, tx.ef, y2, a.zero);
-                z2 = cpml<P>(a.m[0][3], k, tx.cf, tx.ef, z2, a.zero);
+                cpml4<P>(a.m[0], k, {tx.ch, tx.ch, tx.cf, tx.cf}, {tx.eh, tx.eh, tx.ef, tx.ef}, {&x1, &q1, &y2, &z2},
+                         a.zero);
             }
             if (ys >= 0 && upd[j]) {
                 const unsigned k = ((unsigned)iz * YW + ys) * W + ix;
-                y1 = cpml<P>(a.m[1][0], k, ty.ch, ty.eh, y1, a.zero);
-                q2 = cpml<P>(a.m[1][1], k, ty.ch, ty.eh, q2, a.zero);
-                x2 = cpml<P>(a.m[1][2], k, ty.cf, ty.ef, x2, a.zero);
-                z3 = cpml<P>(a.m[1][3], k, ty.cf, ty.ef, z3, a.zero);
+                cpml4<P>(a.m[1], k, {ty.ch, ty.ch, ty.cf, ty.cf}, {ty.eh, ty.eh, ty.ef, ty.ef}, {&y1, &q2, &x2, &z3},
+                         a.zero);
             }
             if (zs >= 0 && upd[j]) {
                 const unsigned k = ((unsigned)zs * a.ny + iy) * W + ix;
-                z1_ = cpml<P>(a.m[2][0], k, tz.ch, tz.eh, z1_, a.zero);
-                q3 = cpml<P>(a.m[2][1], k, tz.ch, tz.eh, q3, a.zero);
-                x3 = cpml<P>(a.m[2][2], k, tz.cf, tz.ef, x3, a.zero);
-                y3 = cpml<P>(a.m[2][3], k, tz.cf, tz.ef, y3, a.zero);
+                cpml4<P>(a.m[2], k, {tz.ch, tz.ch, tz.cf, tz.cf}, {tz.eh, tz.eh, tz.ef, tz.ef}, {&z1_, &q3, &x3, &y3},
+                         a.zero);
             }
             sx[j] = x1 + x2 + x3;
             sy[j] = y1 + y2 + y3;
@@ -651,13 +682,6 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             s3[j] = q3;
         }
 
-        R wx0[N], wy0[N], wz0[N], ux0[N], uy0[N], uz0[N];
-        LD(PW_VWX, o, wx0);
-        LD(PW_VWY, o, wy0);
-        LD(PW_VWZ, o, wz0);
-        LD(PW_VUX, o, ux0);
-        LD(PW_VUY, o, uy0);
-        LD(PW_VUZ, o, uz0);
         R wx1[N], wy1[N], wz1[N], ux1[N], uy1[N], uz1[N];
         if constexpr (!P::scaled) {
             R rho_n[N], rho_x[N], rho_y[N], c1_n[N], c1_x[N], c1_y[N], c2_n[N], c2_x[N], c2_y[N], rf[N];
@@ -894,27 +918,22 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
             R uxz = D(dxz[j]), uzx = D(dzx[j]);
             R uyz = D(dyz[j]), uzy = D(dzy[j]);
             const unsigned ix = (unsigned)(ix0 + j);
+            // terms 4, 5 on the full profile, 6, 7 on the half-point one
             if (xs[j] >= 0) {
                 const Tab4<R> tx = a.tab[0][ix];
                 const unsigned k = ((unsigned)iz * a.ny + iy) * XW + xs[j];
-                uxx = cpml<P>(a.m[0][4], k, tx.cf, tx.ef, uxx, a.zero);
-                wx = cpml<P>(a.m[0][5], k, tx.cf, tx.ef, wx, a.zero);
-                uyx = cpml<P>(a.m[0][6], k, tx.ch, tx.eh, uyx, a.zero);
-                uzx = cpml<P>(a.m[0][7], k, tx.ch, tx.eh, uzx, a.zero);
+                cpml4<P>(a.m[0] + 4, k, {tx.cf, tx.cf, tx.ch, tx.ch}, {tx.ef, tx.ef, tx.eh, tx.eh},
+                         {&uxx, &wx, &uyx, &uzx}, a.zero);
             }
             if (ys >= 0 && upd[j]) {
                 const unsigned k = ((unsigned)iz * YW + ys) * W + ix;
-                uyy = cpml<P>(a.m[1][4], k, ty.cf, ty.ef, uyy, a.zero);
-                wy = cpml<P>(a.m[1][5], k, ty.cf, ty.ef, wy, a.zero);
-                uxy = cpml<P>(a.m[1][6], k, ty.ch, ty.eh, uxy, a.zero);
-                uzy = cpml<P>(a.m[1][7], k, ty.ch, ty.eh, uzy, a.zero);
+                cpml4<P>(a.m[1] + 4, k, {ty.cf, ty.cf, ty.ch, ty.ch}, {ty.ef, ty.ef, ty.eh, ty.eh},
+                         {&uyy, &wy, &uxy, &uzy}, a.zero);
             }
             if (zs >= 0 && upd[j]) {
                 const unsigned k = ((unsigned)zs * a.ny + iy) * W + ix;
-                uzz = cpml<P>(a.m[2][4], k, tz.cf, tz.ef, uzz, a.zero);
-                wz = cpml<P>(a.m[2][5], k, tz.cf, tz.ef, wz, a.zero);
-                uxz = cpml<P>(a.m[2][6], k, tz.ch, tz.eh, uxz, a.zero);
-                uyz = cpml<P>(a.m[2][7], k, tz.ch, tz.eh, uyz, a.zero);
+                cpml4<P>(a.m[2] + 4, k, {tz.cf, tz.cf, tz.ch, tz.ch}, {tz.ef, tz.ef, tz.eh, tz.eh},
+                         {&uzz, &wz, &uxz, &uyz}, a.zero);
             }
             const R du = uxx + uyy + uzz, dw = wx + wy + wz;
             if constexpr (!P::scaled) {
