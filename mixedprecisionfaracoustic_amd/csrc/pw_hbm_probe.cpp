@@ -10,6 +10,8 @@
 //   stream 12:1  12 reads : 1 write  (the step kernels' read/write mix and
 //                                     number of concurrent streams)
 //   read 12      12 reads, one value per thread written at the end
+//   step-like    the velocity kernel's mix with 4-byte accesses: 17 arrays read
+//                (13 wavefields + 4 coefficients), 6 of them rewritten in place
 // Rates count the bytes each pattern must move (every array once), timed
 // with HIP events over repeated launches, best over a few grid sizes.
 //
@@ -72,6 +74,22 @@ __global__ __launch_bounds__(kThreads) void k_read(Inputs in, float* __restrict_
     out[blockIdx.x * (size_t)kThreads + threadIdx.x] = acc;
 }
 
+// 17 reads of separate arrays, the first 6 rewritten in place, one float per thread
+struct StepLike {
+    float* rw[6];
+    const float* ro[11];
+};
+__global__ __launch_bounds__(kThreads) void k_step_like(StepLike a, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) acc += a.ro[k][i];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a.rw[k][i] = a.rw[k][i] * 0.5f + acc * 1e-3f;
+    }
+}
+
 template <class F>
 double best_rate(F launch, double bytes, int iters, const int* grids, int ngrids, int* best_grid)
 {
@@ -119,15 +137,28 @@ int main(int argc, char** argv)
     for (int r = 0; r < kMaxIn; ++r) in.a[r] = buf[r];
     float4* out = buf[kMaxIn];
 
-    struct Row { const char* name; double rate; int grid; } rows[5];
+    struct Row { const char* name; double rate; int grid; } rows[6];
     int g = 0;
     rows[0] = {"copy", best_rate([&](int b) { k_stream<1><<<b, kThreads>>>(in, out, n4); }, 2 * arr, iters, grids, ngrids, &g), g};
     rows[1] = {"stream_2r1w", best_rate([&](int b) { k_stream<2><<<b, kThreads>>>(in, out, n4); }, 3 * arr, iters, grids, ngrids, &g), g};
     rows[2] = {"stream_6r1w", best_rate([&](int b) { k_stream<6><<<b, kThreads>>>(in, out, n4); }, 7 * arr, iters, grids, ngrids, &g), g};
     rows[3] = {"stream_12r1w", best_rate([&](int b) { k_stream<12><<<b, kThreads>>>(in, out, n4); }, 13 * arr, iters, grids, ngrids, &g), g};
     rows[4] = {"read_12", best_rate([&](int b) { k_read<12><<<b, kThreads>>>(in, red, n4); }, 12 * arr, iters, grids, ngrids, &g), g};
+    // the step-like mix over the first 12 buffers plus 5 more arrays of the same size
+    float* extra[5];
+    for (int r = 0; r < 5; ++r) {
+        CHECK(hipMalloc(&extra[r], n * sizeof(float)));
+        k_fill<<<4096, kThreads>>>((float4*)extra[r], n4, 1e-3f);
+    }
+    StepLike sl;
+    for (int r = 0; r < 6; ++r) sl.rw[r] = (float*)buf[r];
+    for (int r = 0; r < 6; ++r) sl.ro[r] = (const float*)buf[6 + r];
+    for (int r = 0; r < 5; ++r) sl.ro[6 + r] = extra[r];
+    rows[5] = {"step_like", best_rate([&](int b) { k_step_like<<<b, kThreads>>>(sl, n); }, 23 * arr, iters, grids, ngrids, &g), g};
 
+    for (int r = 0; r < 5; ++r) CHECK(hipFree(extra[r]));
     // the 2:1 stream of arrays filled with 1 and 2 must give 3
+    for (int r = 0; r < 3; ++r) k_fill<<<4096, kThreads>>>(buf[r], n4, 1.0f + r);
     k_stream<2><<<1024, kThreads>>>(in, out, n4);
     float4 h;
     CHECK(hipMemcpy(&h, out + n4 - 1, sizeof(h), hipMemcpyDeviceToHost));
@@ -137,7 +168,7 @@ int main(int argc, char** argv)
     }
     for (const Row& r : rows) std::printf("%-14s %6d blocks %8.1f GB/s\n", r.name, r.grid, r.rate);
     std::printf("{");
-    for (int i = 0; i < 5; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
+    for (int i = 0; i < 6; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
     std::printf(", \"unit\": \"GB/s\", \"array_bytes\": %.0f, \"iters\": %d}\n", arr, iters);
     for (int r = 0; r <= kMaxIn; ++r) CHECK(hipFree(buf[r]));
     CHECK(hipFree(red));
