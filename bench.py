@@ -281,9 +281,9 @@ def rank_share_main(args):
     stream = torch.cuda.Stream()
     solver = pw.Solver(n, n, nz, np_, h, dt, nt, precision=args.precision, cpml_r=1e-4, cpml_v0=4000.0,
                        stream=stream.cuda_stream, cpml_open=("z_lo", "z_hi"))
-    # the two-layer model with its interface inside the slab
+    # the two-layer model with its interface mid-slab (z_top is a total-grid index)
     spec = workload(n, np_)
-    spec.z_top[1] = (nz - 2 * np_) // 2
+    spec.z_top[1] = nz // 2
     solver.set_medium_layered(spec)
     solver.set_source(n // 2, n // 2, nz // 2, 25.0, (1.0, 1.0, 1.0, -1.0))
     xs = np.linspace(np_, n - 1 - np_, 256).round().astype(np.int64)
