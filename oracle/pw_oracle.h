@@ -18,9 +18,11 @@
  *                                residual bound |dec(enc(v)) - v| <= unit/2
  *   or_half_precompute        -- method.md:1073-1169, 1187-1307 (em=-23, Aw, Bp, Bt,
  *                                scaled moduli)
- *   or_run_half               -- parity unpinned by reference data (no Half output is
- *                                committed in the reference); checked against
- *                                or_run_f32 within the compensated-FP16 error budget.
+ *   or_run_half               -- no reference data (no Half output is committed in
+ *                                the reference); pinned by invariants: within the
+ *                                compensated format's error of or_run_f64, outputs
+ *                                independent of the internal scaling (dynamic rescale
+ *                                off / every step, es = 4), linear in the source.
  */
 #ifndef PW_ORACLE_H
 #define PW_ORACLE_H
