@@ -451,31 +451,68 @@ struct Lanes {
     unsigned sh;             // lanes past the row end read the row's last N cells instead
 };
 
-// r[j] = value at x+1 of element j of c.  One cell per lane: a plain (coalesced,
-// cache-resident) load at x+1.  N cells: registers, the adjacent lane, and for
-// lane 63 one load from the next tile.
+// Cross-lane moves within the wave (DPP): lane i receives lane i+1's value
+// (wave_shl1) or lane i-1's (wave_shr1); the lane that has no source lane
+// receives 0 and is overwritten by the caller.
+#ifndef PW_X_DPP
+#define PW_X_DPP 1
+#endif
+template <int CTRL> __device__ __forceinline__ float lane_move(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL> __device__ __forceinline__ double lane_move(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+
+// r[j] = value at x+1 of element j of c; on = the lane's offset of that value
+// for N = 1, of its last element's right neighbour otherwise; edge(q) loads
+// offset q.  One cell per lane: the next lane's centre value, and for lane 63
+// one load at its own neighbour offset, which is wave-uniform, so the load
+// touches one cache line instead of a misaligned 64-lane row.  N cells:
+// registers, the adjacent lane, and for lane 63 one load from the next tile.
 template <class R, int N, class EdgeLoad, class RowLoad>
-__device__ __forceinline__ void seg_right(const Lanes& L, const R (&c)[N], R (&r)[N], EdgeLoad edge, RowLoad row)
+__device__ __forceinline__ void seg_right(const Lanes& L, const R (&c)[N], R (&r)[N], unsigned on, EdgeLoad edge,
+                                          RowLoad row)
 {
     if constexpr (N == 1) {
-        row(r);
+        if constexpr (PW_X_DPP) {
+            const R s = lane_move<kDppWaveShl1>(c[0]);
+            const R e = edge(__builtin_amdgcn_readlane(on, 63));
+            r[0] = L.lane == 63 ? e : s;
+        } else {
+            row(r);
+        }
     } else {
         R s = __shfl_down(c[0], 1, 64);
-        if (L.lane == 63) s = L.edge_r ? edge() : (R)0;
+        if (L.lane == 63) s = L.edge_r ? edge(on) : (R)0;
 #pragma unroll
         for (int j = 0; j < N - 1; ++j) r[j] = c[j + 1];
         r[N - 1] = s;
     }
 }
-// r[j] = value at x-1 of element j
+// r[j] = value at x-1 of element j; on = the lane's offset of its first
+// element's left neighbour (lane 0 loads it)
 template <class R, int N, class EdgeLoad, class RowLoad>
-__device__ __forceinline__ void seg_left(const Lanes& L, const R (&c)[N], R (&r)[N], EdgeLoad edge, RowLoad row)
+__device__ __forceinline__ void seg_left(const Lanes& L, const R (&c)[N], R (&r)[N], unsigned on, EdgeLoad edge,
+                                         RowLoad row)
 {
     if constexpr (N == 1) {
-        row(r);
+        if constexpr (PW_X_DPP) {
+            const R s = lane_move<kDppWaveShr1>(c[0]);
+            const R e = edge(__builtin_amdgcn_readlane(on, 0));
+            r[0] = L.lane == 0 ? e : s;
+        } else {
+            row(r);
+        }
     } else {
         R s = __shfl_up(c[N - 1], 1, 64);
-        if (L.lane == 0) s = L.edge_l ? edge() : (R)0;
+        if (L.lane == 0) s = L.edge_l ? edge(on) : (R)0;
         r[0] = s;
 #pragma unroll
         for (int j = 1; j < N; ++j) r[j] = c[j - 1];
@@ -585,15 +622,15 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             LD(PW_TZZ, o + PL, tzz_n);
             LD(PW_P, o + PL, p_n);
             LD(PW_TXX, o, txx_c);
-            seg_right(L, txx_c, txx_r, [&] { return P::ld(a.f[PW_TXX], o + N); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
+            seg_right(L, txx_c, txx_r, o + N, [&](unsigned q) { return P::ld(a.f[PW_TXX], q); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
             LD(PW_TYY, o, tyy_c);
             LD(PW_TYY, o + W, tyy_q);
             LD(PW_TXY, o, txy_c);
-            seg_left(L, txy_c, txy_l, [&] { return P::ld(a.f[PW_TXY], o - 1); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
+            seg_left(L, txy_c, txy_l, o - 1, [&](unsigned q) { return P::ld(a.f[PW_TXY], q); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
             LD(PW_TXY, o - W, txy_h);
-            seg_left(L, txz_c, txz_l, [&] { return P::ld(a.f[PW_TXZ], o - 1); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
+            seg_left(L, txz_c, txz_l, o - 1, [&](unsigned q) { return P::ld(a.f[PW_TXZ], q); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
             LD(PW_TYZ, o - W, tyz_h);
-            seg_right(L, p_c, p_r, [&] { return P::ld(a.f[PW_P], o + N); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
+            seg_right(L, p_c, p_r, o + N, [&](unsigned q) { return P::ld(a.f[PW_P], q); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
             LD(PW_P, o + W, p_q);
 #pragma unroll
             for (int j = 0; j < N; ++j) {
@@ -686,13 +723,13 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         if constexpr (!P::scaled) {
             R rho_n[N], rho_x[N], rho_y[N], c1_n[N], c1_x[N], c1_y[N], c2_n[N], c2_x[N], c2_y[N], rf[N];
             LDC(PC_RHO, o + PL, rho_n);
-            seg_right(L, rho_c, rho_x, [&] { return P::ldc(a.c[PC_RHO], o + N); }, [&](R (&r)[N]) { LDC(PC_RHO, o + 1, r); });
+            seg_right(L, rho_c, rho_x, o + N, [&](unsigned q) { return P::ldc(a.c[PC_RHO], q); }, [&](R (&r)[N]) { LDC(PC_RHO, o + 1, r); });
             LDC(PC_RHO, o + W, rho_y);
             LDC(PC_C1, o + PL, c1_n);
-            seg_right(L, c1_c, c1_x, [&] { return P::ldc(a.c[PC_C1], o + N); }, [&](R (&r)[N]) { LDC(PC_C1, o + 1, r); });
+            seg_right(L, c1_c, c1_x, o + N, [&](unsigned q) { return P::ldc(a.c[PC_C1], q); }, [&](R (&r)[N]) { LDC(PC_C1, o + 1, r); });
             LDC(PC_C1, o + W, c1_y);
             LDC(PC_C2, o + PL, c2_n);
-            seg_right(L, c2_c, c2_x, [&] { return P::ldc(a.c[PC_C2], o + N); }, [&](R (&r)[N]) { LDC(PC_C2, o + 1, r); });
+            seg_right(L, c2_c, c2_x, o + N, [&](unsigned q) { return P::ldc(a.c[PC_C2], q); }, [&](R (&r)[N]) { LDC(PC_C2, o + 1, r); });
             LDC(PC_C2, o + W, c2_y);
             LDC(PC_RHOF, o, rf);
 #pragma unroll
@@ -827,7 +864,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     R mu_c[N], mu_x[N], mu_y[N];
     if constexpr (!P::scaled) {
         LDC(PC_MU, o, mu_c);
-        seg_right(L, mu_c, mu_x, [&] { return P::ldc(a.c[PC_MU], o + N); }, [&](R (&r)[N]) { LDC(PC_MU, o + 1, r); });
+        seg_right(L, mu_c, mu_x, o + N, [&](unsigned q) { return P::ldc(a.c[PC_MU], q); }, [&](R (&r)[N]) { LDC(PC_MU, o + 1, r); });
         LDC(PC_MU, o + W, mu_y);
     }
     const R C4A = (R)1.125, C4B = (R)(-1.0 / 24.0);
@@ -843,14 +880,14 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
             LD(PW_VUY, o + PL, vuy_n);
             LD(PW_VUZ, o, vuz_c);
             LD(PW_VWZ, o, vwz_c);
-            seg_left(L, vux_c, vux_l, [&] { return P::ld(a.f[PW_VUX], o - 1); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
+            seg_left(L, vux_c, vux_l, o - 1, [&](unsigned q) { return P::ld(a.f[PW_VUX], q); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
             LD(PW_VUX, o + W, vux_q);
             LD(PW_VUY, o - W, vuy_h);
-            seg_right(L, vuy_c, vuy_r, [&] { return P::ld(a.f[PW_VUY], o + N); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
-            seg_right(L, vuz_c, vuz_r, [&] { return P::ld(a.f[PW_VUZ], o + N); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
+            seg_right(L, vuy_c, vuy_r, o + N, [&](unsigned q) { return P::ld(a.f[PW_VUY], q); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
+            seg_right(L, vuz_c, vuz_r, o + N, [&](unsigned q) { return P::ld(a.f[PW_VUZ], q); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
             LD(PW_VUZ, o + W, vuz_q);
             LD(PW_VWX, o, vwx_c);
-            seg_left(L, vwx_c, vwx_l, [&] { return P::ld(a.f[PW_VWX], o - 1); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
+            seg_left(L, vwx_c, vwx_l, o - 1, [&](unsigned q) { return P::ld(a.f[PW_VWX], q); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
             LD(PW_VWY, o, vwy_c);
             LD(PW_VWY, o - W, vwy_h);
 #pragma unroll
@@ -901,9 +938,9 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         R C_[N], M_[N], HH_[N], H2u_[N], mxy[N], myz[N], mxz[N];
         if constexpr (!P::scaled) {
             LDC(PC_MU, o + PL, mu_n);
-            seg_right(L, mu_n, mu_xn, [&] { return P::ldc(a.c[PC_MU], o + PL + N); }, [&](R (&r)[N]) { LDC(PC_MU, o + PL + 1, r); });
+            seg_right(L, mu_n, mu_xn, o + PL + N, [&](unsigned q) { return P::ldc(a.c[PC_MU], q); }, [&](R (&r)[N]) { LDC(PC_MU, o + PL + 1, r); });
             LDC(PC_MU, o + PL + W, mu_yn);
-            seg_right(L, mu_y, mu_xy, [&] { return P::ldc(a.c[PC_MU], o + W + N); }, [&](R (&r)[N]) { LDC(PC_MU, o + W + 1, r); });
+            seg_right(L, mu_y, mu_xy, o + W + N, [&](unsigned q) { return P::ldc(a.c[PC_MU], q); }, [&](R (&r)[N]) { LDC(PC_MU, o + W + 1, r); });
             LDC(PC_C, o, C_); LDC(PC_M, o, M_); LDC(PC_HH, o, HH_); LDC(PC_H2U, o, H2u_);
         } else {
             LDC(SC_C, o, C_); LDC(SC_M, o, M_); LDC(SC_HH, o, HH_); LDC(SC_H2U, o, H2u_);
