@@ -371,7 +371,7 @@ def main():
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--slots", type=int, default=32)
+    ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--round-steps", type=int, default=32)
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--volume-every", type=int, default=0,
