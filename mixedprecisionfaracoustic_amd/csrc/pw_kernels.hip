@@ -6,12 +6,20 @@
 //                Crank-Nicolson viscous coupling (Float/main.cu:64-192)
 //   k_stress   : fused stress + pore-pressure update (Float/main.cu:194-295)
 //   k_gather   : receivers -> device seismogram buffer (Float/main.cu:1083-1093)
+// (Half: k_rescale every 100 steps before the source, Half/main.cu:1326-1364)
 //
 // Layout: one padded SoA array per wavefield (row pitch = nx rounded to 64
 // elements).  A block is a 64 x 4 (x, y) column tile that marches up a chunk of
 // z; each wave owns one 64-cell x row, so every load is a coalesced 256 B (FP32)
-// row segment, x/y neighbours hit L1/L2, and z neighbours are carried in
+// row segment.  x neighbours come from the adjacent lane (one DPP move; only
+// the edge lane loads), y neighbours hit L1/L2, and z neighbours are carried in
 // registers from the previous iteration (each value is loaded once per block).
+// A plane's loads are issued ahead of its CPML memory-variable stores, which
+// the compiler cannot prove disjoint from them.
+//
+// Also here: the fused one-launch step (opt-in), the batched kernels of the
+// shot server (one launch per kernel over many shots), the rescale, snapshot,
+// decode and fill side kernels.
 //
 // CPML: the memory variables exist only on the boundary slabs of each axis (in
 // the interior the reference's psi is identically zero).  Each term keeps ONE
