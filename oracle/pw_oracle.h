@@ -13,7 +13,9 @@
  *   or_run_f32                -- the 11 committed Float/<name>.dat snapshots (192^3,
  *                                501..3500 steps), mirror symmetry, fluid vw == 0
  *   or_run_f64                -- or_run_f32 within FP32 rounding; textbook acoustic
- *                                limit (fluid medium, c = sqrt(Kf/rhof)); linearity
+ *                                limit (fluid medium: c = sqrt(Kf/rhof) to 0.4 %, 1/r
+ *                                spreading, vw and shear stresses exactly 0);
+ *                                linearity in the source (bit-exact)
  *   or_half_*  codec          -- IEEE-754 binary16 (exhaustive vs numpy float16),
  *                                residual bound |dec(enc(v)) - v| <= unit/2
  *   or_half_precompute        -- method.md:1073-1169, 1187-1307 (em=-23, Aw, Bp, Bt,
