@@ -166,6 +166,10 @@ pw_status check_config(const pw_config& c)
     const unsigned long long pitch = (unsigned long long)(c.nx + 63) / 64 * 64;
     if (pitch * c.ny * c.nz >= (1ull << 32) - (1ull << 20))
         return fail(PW_ERR_UNSUPPORTED, "pw_create: grid exceeds 2^32 padded cells (32-bit offsets)");
+    // a compensated field's residual bytes are addressed by a 32-bit offset from
+    // its FP16 values: 2 bytes per cell of values, then 1 per cell
+    if (has_lo(c.precision) && 3ull * pitch * c.ny * c.nz >= (1ull << 32) - (1ull << 20))
+        return fail(PW_ERR_UNSUPPORTED, "pw_create: comp16 grid exceeds 2^32 / 3 padded cells (32-bit offsets)");
     return PW_OK;
 }
 
@@ -226,9 +230,9 @@ void plan_layout(const pw_config& c, int fused, Layout& L)
     const bool c16 = has_lo(prec);
     const int nbuf = fused ? 2 : 1;  // the fused step reads time n and writes time n+1
     for (int b2 = 0; b2 < nbuf; ++b2) {
-        for (int k = 0; k < PW_NFIELDS; ++k) take(cells * (c16 ? 2 : sb));
-        if (c16)
-            for (int k = 0; k < PW_NFIELDS; ++k) take(cells * 1);
+        // compensated fields: FP16 values, then the residual bytes at the same
+        // offset (align256(2 * cells)) for every field
+        for (int k = 0; k < PW_NFIELDS; ++k) take(c16 ? align256(cells * 2) + cells : cells * sb);
         for (int ax = 0; ax < 3; ++ax)
             for (int k = 0; k < kMemPerAxis; ++k) take((size_t)g.slab_cells[ax] * mb);
     }
@@ -287,7 +291,6 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     const size_t total = L.total;
     const bool c16 = has_lo(s->prec);
     const int nc = ncoef(s->prec);
-    (void)g;
 
     hipError_t e = hipMalloc(&s->pool, total);
     if (e != hipSuccess) {
@@ -300,13 +303,14 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     s->state_bytes_total = L.zero_bytes;
     char* base = (char*)s->pool;
     size_t i = 0;
+    const size_t loff = align256((size_t)g.cells * 2);  // residual bytes after a field's FP16 values
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.f[k] = base + off[i++];
-    for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = c16 ? (int8_t*)(base + off[i++]) : nullptr;
+    for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = c16 ? (int8_t*)((char*)s->b.f[k] + loff) : nullptr;
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k < kMemPerAxis; ++k) s->b.mem[ax][k] = base + off[i++];
     s->b.fused = fused ? 1 : 0;
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.fo[k] = fused ? base + off[i++] : nullptr;
-    for (int k = 0; k < PW_NFIELDS; ++k) s->b.loo[k] = (fused && c16) ? (int8_t*)(base + off[i++]) : nullptr;
+    for (int k = 0; k < PW_NFIELDS; ++k) s->b.loo[k] = (fused && c16) ? (int8_t*)((char*)s->b.fo[k] + loff) : nullptr;
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k < kMemPerAxis; ++k) s->b.memo[ax][k] = fused ? base + off[i++] : nullptr;
     for (int k = 0; k < kMaxCoef; ++k) s->b.coef[k] = k < nc ? base + off[i++] : nullptr;

@@ -290,6 +290,11 @@ template <class P> struct StepArgs {
     int zb, ze;            // planes [zb, ze) updated by this launch
     int xcd_map;           // 1: XCD-aware block -> tile mapping
     typename P::R dt, h;
+    // compensated fields: bytes from a field's FP16 values to its residual bytes
+    // (the same for every field); comp16f: storage scale of the velocity,
+    // relative-velocity and stress classes and their inverses
+    unsigned loff;
+    float fsc[3], fisc[3];
 };
 
 template <class P> __host__ typename P::Field make_field(const DevBuffers& b, int k);
@@ -331,8 +336,112 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     a.xcd_map = b.xcd_map;
     a.dt = (R)b.dt;
     a.h = (R)b.h;
+    // the pool stores each compensated field's residual bytes right after its
+    // FP16 values, at one offset for all fields
+    a.loff = b.lo[0] ? (unsigned)((const char*)b.lo[0] - (const char*)b.f[0]) : 0u;
+    const float sc[3] = {b.sc_vel, b.sc_vw, b.sc_stress};
+    for (int k = 0; k < 3; ++k) {
+        a.fsc[k] = sc[k];
+        a.fisc[k] = 1.0f / sc[k];
+    }
     return a;
 }
+
+// --------------------------------------------------------------------------
+// Field access in the step kernels.  Each field puts one pointer, its value
+// array, into the kernels' scalar registers.  The compensated policies find a
+// field's residual bytes a.loff bytes after its FP16 values (one offset for all
+// fields), and the zero of round32 and comp16f's scale for the field's class
+// come from the launch, not from each field: otherwise 13 fields hold up to two
+// pointers, a zero and two scales each, more than the scalar register file,
+// and the overflow is reloaded lane by lane.  The values are exactly those of
+// the policies' own ld / st.
+// --------------------------------------------------------------------------
+template <class P> struct FA {
+    using R = typename P::R;
+    static __device__ __forceinline__ R ld(const StepArgs<P>& a, int fid, unsigned o) { return P::ld(a.f[fid], o); }
+    template <int N> static __device__ __forceinline__ void ldv(const StepArgs<P>& a, int fid, unsigned o, R (&r)[N])
+    {
+        P::template ldv<N>(a.f[fid], o, r);
+    }
+    static __device__ __forceinline__ void st(const StepArgs<P>& a, int fid, unsigned o, R v) { P::st(a.f[fid], o, v); }
+    template <int N>
+    static __device__ __forceinline__ void stv(const StepArgs<P>& a, int fid, unsigned o, const R (&r)[N])
+    {
+        P::template stv<N>(a.f[fid], o, r);
+    }
+};
+template <> struct FA<PolB16> {
+    using R = float;
+    static __device__ __forceinline__ R ld(const StepArgs<PolB16>& a, int fid, unsigned o) { return (float)a.f[fid].p[o]; }
+    template <int N> static __device__ __forceinline__ void ldv(const StepArgs<PolB16>& a, int fid, unsigned o, R (&r)[N])
+    {
+        const Pack<bf16, N> t = ld_pack<N>(a.f[fid].p, o);
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = (float)t.v[j];
+    }
+    static __device__ __forceinline__ void st(const StepArgs<PolB16>& a, int fid, unsigned o, R v)
+    {
+        a.f[fid].p[o] = (bf16)round32(v, a.zero);
+    }
+    template <int N>
+    static __device__ __forceinline__ void stv(const StepArgs<PolB16>& a, int fid, unsigned o, const R (&r)[N])
+    {
+        Pack<bf16, N> t;
+#pragma unroll
+        for (int j = 0; j < N; ++j) t.v[j] = narrow<bf16>(r[j], a.zero);
+        st_pack<N>(a.f[fid].p, o, t);
+    }
+};
+// comp16 (SC = false) and comp16f (SC = true, values stored times the class scale)
+template <class P, bool SC> struct FAComp {
+    using R = float;
+    static __device__ __forceinline__ int cls(int fid) { return fid <= PW_VUZ ? 0 : (fid <= PW_VWZ ? 1 : 2); }
+    static __device__ __forceinline__ R ld(const StepArgs<P>& a, int fid, unsigned o)
+    {
+        const f16* h = a.f[fid].hi;
+        const R v = c16_decode(h[o], ((const int8_t*)h)[a.loff + o]);
+        if constexpr (SC) return v * a.fisc[cls(fid)];
+        else return v;
+    }
+    template <int N> static __device__ __forceinline__ void ldv(const StepArgs<P>& a, int fid, unsigned o, R (&r)[N])
+    {
+        const f16* h = a.f[fid].hi;
+        const Pack<f16, N> hv = ld_pack<N>(h, o);
+        const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h, a.loff + o);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if constexpr (SC) r[j] = c16_decode(hv.v[j], lv.v[j]) * a.fisc[cls(fid)];
+            else r[j] = c16_decode(hv.v[j], lv.v[j]);
+        }
+    }
+    static __device__ __forceinline__ void st(const StepArgs<P>& a, int fid, unsigned o, R v)
+    {
+        f16* h = a.f[fid].hi;
+        f16 hh;
+        int8_t l;
+        if constexpr (SC) c16_encode(v * a.fsc[cls(fid)], hh, l, a.zero);
+        else c16_encode(v, hh, l, a.zero);
+        h[o] = hh;
+        ((int8_t*)h)[a.loff + o] = l;
+    }
+    template <int N>
+    static __device__ __forceinline__ void stv(const StepArgs<P>& a, int fid, unsigned o, const R (&r)[N])
+    {
+        f16* h = a.f[fid].hi;
+        Pack<f16, N> hv;
+        Pack<int8_t, N> lv;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if constexpr (SC) c16_encode(r[j] * a.fsc[cls(fid)], hv.v[j], lv.v[j], a.zero);
+            else c16_encode(r[j], hv.v[j], lv.v[j], a.zero);
+        }
+        st_pack<N>(h, o, hv);
+        st_pack<N>((int8_t*)h, a.loff + o, lv);
+    }
+};
+template <> struct FA<PolC16> : FAComp<PolC16, false> {};
+template <> struct FA<PolC16F> : FAComp<PolC16F, true> {};
 
 // Slab-local index along one axis, -1 outside the CPML slabs.
 __device__ __forceinline__ int slab_index(int i, int n, int lo, int hi)
@@ -571,14 +680,14 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
 
     // lanes past the row end (wide segments only) read the row's last cells; their
     // values are never used
-    auto LD = [&](int fid, unsigned off, R (&r)[N]) { P::template ldv<N>(a.f[fid], off - L.sh, r); };
+    auto LD = [&](int fid, unsigned off, R (&r)[N]) { FA<P>::template ldv<N>(a, fid, off - L.sh, r); };
     auto LDC = [&](int cid, unsigned off, R (&r)[N]) { P::template ldcv<N>(a.c[cid], off - L.sh, r); };
     auto ST = [&](int fid, unsigned off, const R (&r)[N]) {
-        if (all) P::template stv<N>(a.f[fid], off, r);
+        if (all) FA<P>::template stv<N>(a, fid, off, r);
         else {
 #pragma unroll
             for (int j = 0; j < N; ++j)
-                if (upd[j]) P::st(a.f[fid], off + j, r[j]);
+                if (upd[j]) FA<P>::st(a, fid, off + j, r[j]);
         }
     };
 
@@ -630,15 +739,15 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             LD(PW_TZZ, o + PL, tzz_n);
             LD(PW_P, o + PL, p_n);
             LD(PW_TXX, o, txx_c);
-            seg_right(L, txx_c, txx_r, o + N, [&](unsigned q) { return P::ld(a.f[PW_TXX], q); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
+            seg_right(L, txx_c, txx_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_TXX, q); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
             LD(PW_TYY, o, tyy_c);
             LD(PW_TYY, o + W, tyy_q);
             LD(PW_TXY, o, txy_c);
-            seg_left(L, txy_c, txy_l, o - 1, [&](unsigned q) { return P::ld(a.f[PW_TXY], q); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
+            seg_left(L, txy_c, txy_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXY, q); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
             LD(PW_TXY, o - W, txy_h);
-            seg_left(L, txz_c, txz_l, o - 1, [&](unsigned q) { return P::ld(a.f[PW_TXZ], q); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
+            seg_left(L, txz_c, txz_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXZ, q); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
             LD(PW_TYZ, o - W, tyz_h);
-            seg_right(L, p_c, p_r, o + N, [&](unsigned q) { return P::ld(a.f[PW_P], q); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
+            seg_right(L, p_c, p_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_P, q); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
             LD(PW_P, o + W, p_q);
 #pragma unroll
             for (int j = 0; j < N; ++j) {
@@ -651,7 +760,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             }
         } else {
             // N == 1: neighbours two cells out along x and y from L1/L2, along z from the carries
-            auto F = [&](int fid, long d) { return P::ld(a.f[fid], (unsigned)((long)o + d)); };
+            auto F = [&](int fid, long d) { return FA<P>::ld(a, fid, (unsigned)((long)o + d)); };
             const long X = 1, Y = (long)W;
             R txz_p, tyz_p, tzz_nn, p_nn;
             txz_p = F(PW_TXZ, (long)PL);
@@ -831,14 +940,14 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
 
     // lanes past the row end (wide segments only) read the row's last cells; their
     // values are never used
-    auto LD = [&](int fid, unsigned off, R (&r)[N]) { P::template ldv<N>(a.f[fid], off - L.sh, r); };
+    auto LD = [&](int fid, unsigned off, R (&r)[N]) { FA<P>::template ldv<N>(a, fid, off - L.sh, r); };
     auto LDC = [&](int cid, unsigned off, R (&r)[N]) { P::template ldcv<N>(a.c[cid], off - L.sh, r); };
     auto ST = [&](int fid, unsigned off, const R (&r)[N]) {
-        if (all) P::template stv<N>(a.f[fid], off, r);
+        if (all) FA<P>::template stv<N>(a, fid, off, r);
         else {
 #pragma unroll
             for (int j = 0; j < N; ++j)
-                if (upd[j]) P::st(a.f[fid], off + j, r[j]);
+                if (upd[j]) FA<P>::st(a, fid, off + j, r[j]);
         }
     };
 
@@ -888,14 +997,14 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
             LD(PW_VUY, o + PL, vuy_n);
             LD(PW_VUZ, o, vuz_c);
             LD(PW_VWZ, o, vwz_c);
-            seg_left(L, vux_c, vux_l, o - 1, [&](unsigned q) { return P::ld(a.f[PW_VUX], q); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
+            seg_left(L, vux_c, vux_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VUX, q); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
             LD(PW_VUX, o + W, vux_q);
             LD(PW_VUY, o - W, vuy_h);
-            seg_right(L, vuy_c, vuy_r, o + N, [&](unsigned q) { return P::ld(a.f[PW_VUY], q); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
-            seg_right(L, vuz_c, vuz_r, o + N, [&](unsigned q) { return P::ld(a.f[PW_VUZ], q); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
+            seg_right(L, vuy_c, vuy_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUY, q); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
+            seg_right(L, vuz_c, vuz_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUZ, q); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
             LD(PW_VUZ, o + W, vuz_q);
             LD(PW_VWX, o, vwx_c);
-            seg_left(L, vwx_c, vwx_l, o - 1, [&](unsigned q) { return P::ld(a.f[PW_VWX], q); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
+            seg_left(L, vwx_c, vwx_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VWX, q); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
             LD(PW_VWY, o, vwy_c);
             LD(PW_VWY, o - W, vwy_h);
 #pragma unroll
@@ -907,7 +1016,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                 dyz[j] = vuy_n[j] - vuy_c[j]; dzy[j] = vuz_q[j] - vuz_c[j];
             }
         } else {
-            auto F = [&](int fid, long d) { return P::ld(a.f[fid], (unsigned)((long)o + d)); };
+            auto F = [&](int fid, long d) { return FA<P>::ld(a, fid, (unsigned)((long)o + d)); };
             const long X = 1, Y = (long)W;
             const R vuz_p = F(PW_VUZ, (long)PL), vwz_p = F(PW_VWZ, (long)PL);
             const R vux_nn = F(PW_VUX, 2 * (long)PL), vuy_nn = F(PW_VUY, 2 * (long)PL);
