@@ -36,6 +36,24 @@ def workload(n, np_):
     return pw.layered([0, n // 2], [upper, lower], perturb=0.05, corr_cells=8.0, clamp=3.0, seed=0)
 
 
+def large_workload(n, np_, seed=3):
+    """The 1024^3 model: 8 horizontal layers, porosity U[0.05, 0.3], permeability
+    log-uniform in [1e-15, 1e-12] m^2, dry moduli from a porosity trend (stiffer
+    when less porous), layer tops spread with a seeded jitter, each cell's Kb,
+    mu and phi perturbed by 5 % (8-cell correlation)."""
+    import numpy as np
+    import mixedprecisionfaracoustic_amd as pw
+    rng = np.random.default_rng(seed)
+    tops = [0] + sorted(int(v) for v in rng.choice(np.arange(np_ + 40, n - np_ - 40), 7, replace=False))
+    rocks = []
+    for _ in range(8):
+        phi = float(rng.uniform(0.05, 0.3))
+        kb = float(35e9 * (1 - phi / 0.4) ** 2 + 2e9)
+        rocks.append(pw.rock(Ks=35e9, Kb=kb, mu=0.8 * kb, Kf=2.25e9, rhos=2650.0, rhof=1000.0, phi=phi,
+                             kappa=float(10 ** rng.uniform(-15, -12)), eta=1e-3, tortuosity=2.0, pore_m=8.0))
+    return pw.layered(tops, rocks, perturb=0.05, corr_cells=8.0, clamp=3.0, seed=seed)
+
+
 def receivers(n, np_):
     import numpy as np
     xs = np.linspace(np_, n - 1 - np_, 256).round().astype(np.int64)
@@ -50,10 +68,10 @@ def receivers(n, np_):
 
 
 def measured_traffic(precision, kernel, n):
-    """HBM bytes per launch of the kernel from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (profiles/hbm_traffic_512.md), or None."""
+    """HBM bytes per launch of the kernel on the n^3 grid from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/hbm_traffic_<n>.md), or None."""
     import re
-    path = os.path.join(HERE, "profiles", "hbm_traffic_512.md")
+    path = os.path.join(HERE, "profiles", f"hbm_traffic_{n}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -381,6 +399,9 @@ def main():
     ap.add_argument("--rank-planes", type=int, default=64)
     ap.add_argument("--hbm-ceiling", action="store_true",
                     help="run the streaming-kernel bandwidth probe (the practical HBM roof) instead")
+    ap.add_argument("--large", action="store_true",
+                    help="the 1024^3 eight-layer model (NP 32, dx 4 m, dt 0.4 ms, 20 Hz) instead of 512^3; "
+                         "no CPU baseline (its FP64 base fields alone take 77 GB of host memory)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
@@ -398,14 +419,20 @@ def main():
     import torch
     import mixedprecisionfaracoustic_amd as pw
 
-    n, np_, h, dt = args.n, args.np, 5.0, 0.5e-3
+    if args.large:
+        n, np_, h, dt, f0 = 1024, 32, 4.0, 0.4e-3, 20.0
+        spec = large_workload(n, np_)
+        name, medium = "biot_eight_layer", "eight-layer perturbed Biot medium generated on the GPU, seed 3"
+    else:
+        n, np_, h, dt, f0 = args.n, args.np, 5.0, 0.5e-3, 25.0
+        spec = workload(n, np_)
+        name, medium = "biot_two_layer", "two-layer perturbed Biot medium generated on the GPU, seed 0"
     nt = args.warmup + args.steps + args.prof_steps
     stream = torch.cuda.Stream()
     solver = pw.Solver(n, n, n, np_, h, dt, nt, precision=args.precision, cpml_r=1e-4, cpml_v0=4000.0,
                        stream=stream.cuda_stream)
-    spec = workload(n, np_)
     solver.set_medium_layered(spec)
-    solver.set_source(n // 2, n // 2, n // 2, 25.0, (1.0, 1.0, 1.0, -1.0))
+    solver.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
     fields, idx = receivers(n, np_)
     solver.set_receivers(fields, idx)
     snap_steps = [s for s in range(500, nt, 500)] or [nt - 1]
@@ -471,9 +498,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtype,
-        "data": "synthetic (two-layer perturbed Biot medium generated on the GPU, seed 0)",
-        "config": {"workload": f"biot_two_layer_{n}^3_{args.precision}", "grid": [n, n, n], "cpml": np_,
-                   "dx_m": h, "dt_s": dt, "f0_hz": 25.0, "receivers": len(idx), "precision": args.precision,
+        "data": f"synthetic ({medium})",
+        "config": {"workload": f"{name}_{n}^3_{args.precision}", "grid": [n, n, n], "cpml": np_,
+                   "dx_m": h, "dt_s": dt, "f0_hz": f0, "receivers": len(idx), "precision": args.precision,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -494,7 +521,7 @@ def main():
         out["volume_snapshots"] = {"every": args.volume_every, "in_timed_steps": len(timed),
                                    "bytes_each": int(cells * 4)}
     solver.close()
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.large:
         base = None
         s2 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
         s2.set_medium_layered(spec)

@@ -69,22 +69,6 @@ def test_512_workload_linear_and_chunk_exact(prec, tol):
         assert exact > 0.99
 
 
-def config3_layers(nz, np_, seed=3):
-    """8 horizontal layers: porosity U[0.05, 0.3], permeability log-uniform
-    [1e-15, 1e-12] m^2, dry moduli from a porosity trend (stiffer when less
-    porous), layer tops evenly spread with a seeded jitter."""
-    rng = np.random.default_rng(seed)
-    tops = [0] + sorted(int(v) for v in rng.choice(np.arange(np_ + 40, nz - np_ - 40), 7, replace=False))
-    rocks = []
-    for _ in range(8):
-        phi = float(rng.uniform(0.05, 0.3))
-        kb = float(35e9 * (1 - phi / 0.4) ** 2 + 2e9)
-        mu = 0.8 * kb
-        rocks.append(pw.rock(Ks=35e9, Kb=kb, mu=mu, Kf=2.25e9, rhos=2650.0, rhof=1000.0, phi=phi,
-                             kappa=float(10 ** rng.uniform(-15, -12)), eta=1e-3, tortuosity=2.0, pore_m=8.0))
-    return pw.layered(tops, rocks, perturb=0.05, corr_cells=8.0, clamp=3.0, seed=seed)
-
-
 def test_1024_workload_runs_linear():
     n, np_, nt = 1024, 32, 60
     s = pw.Solver(n, n, n, np_, 4.0, 0.4e-3, nt, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
@@ -92,7 +76,7 @@ def test_1024_workload_runs_linear():
     print(f"1024^3 FP32: {info['device_bytes'] / 1e9:.1f} GB on the device, "
           f"{info['bytes_per_cell_step']:.1f} B per cell-step")
     assert info["device_bytes"] < 150e9
-    s.set_medium_layered(config3_layers(n, np_))
+    s.set_medium_layered(bench.large_workload(n, np_))
     c = n // 2
     near = [x + c * n + c * n * n for x in range(c - 24, c + 25, 2)]
     s.set_receivers(["txx"] * len(near) + ["vux"] * len(near) + ["p"] * len(near), near * 3)
