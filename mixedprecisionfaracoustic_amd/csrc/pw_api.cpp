@@ -325,9 +325,13 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     env_int("PW_TY", &ty);
     s->b.ty = (ty == 1 || ty == 2 || ty == 4 || ty == 8) ? ty : 4;
     // cells per lane along x: wider row segments per wave
-    int vx = 1;
+    int vx = 0;
     env_int("PW_VX", &vx);
-    s->b.vx = (vx == 1 || vx == 2 || vx == 4) ? vx : 1;
+    const bool vx_set = vx == 1 || vx == 2 || vx == 4;
+    s->b.vx = vx_set ? vx : 1;
+    // comp16's velocity kernel runs fastest with 2 cells per lane (its FP16 and
+    // int8 rows then load whole cache lines); every other kernel with one
+    s->b.vx_vel = vx_set ? vx : (s->prec == PW_COMP16 ? 2 : 1);
     int xcd = 1;
     env_int("PW_XCD_MAP", &xcd);
     s->b.xcd_map = xcd ? 1 : 0;

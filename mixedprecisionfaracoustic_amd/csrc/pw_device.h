@@ -56,7 +56,8 @@ struct DevBuffers {
     double dt, h;
     int zc;          // z cells per block in the step kernels
     int ty;          // rows per tile in the step kernels (block 64 x ty)
-    int vx;          // cells per lane along x in the step kernels (1, 2, 4)
+    int vx;          // cells per lane along x in the stress kernel (1, 2, 4)
+    int vx_vel;      // ... and in the velocity kernel
     int order;       // spatial order of the staggered differences: 2 or 4
     float sc_stress, sc_vel, sc_vw;  // comp16f: stored value = physical * scale (powers of two)
     int xcd_map;     // 1: XCD-aware block -> tile mapping in the step kernels

@@ -1720,9 +1720,11 @@ template <class F> static hipError_t dispatch(int prec, F&& f)
 hipError_t launch_velocity(const DevBuffers& b, hipStream_t st, int zb, int ze)
 {
     if (ze <= zb) return hipSuccess;
+    DevBuffers bv = b;
+    bv.vx = b.vx_vel;  // the velocity kernel's own row-segment width
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
-        const StepArgs<P> a = range_args<P>(b, zb, ze);
+        const StepArgs<P> a = range_args<P>(bv, zb, ze);
         if (b.order == 4) {
             k_velocity<P, 1, 4><<<a.nblocks, dim3(64, a.ty), 0, st>>>(a);
             return hipGetLastError();
