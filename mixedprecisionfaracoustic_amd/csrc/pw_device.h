@@ -55,7 +55,9 @@ struct DevBuffers {
     void* tab[3];    // per axis: n entries of {c_full, e_full, c_half, e_half} (float or double)
     double dt, h;
     int zc;          // z cells per block in the step kernels
-    int ty;          // rows per tile in the step kernels (block 64 x ty)
+    int ty;          // waves per block in the step kernels (block 64 x ty)
+    int wx;          // waves side by side along x in a stress-kernel tile (1 or 2)
+    int wx_vel;      // ... and in a velocity-kernel tile
     int vx;          // cells per lane along x in the stress kernel (1, 2, 4)
     int vx_vel;      // ... and in the velocity kernel
     int order;       // spatial order of the staggered differences: 2 or 4

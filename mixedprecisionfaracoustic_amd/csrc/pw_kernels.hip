@@ -284,7 +284,8 @@ template <class P> struct StepArgs {
     int lo[3], hi[3];
     int zc;
     int gx, gy, nblocks;   // tile grid (x tiles, y tiles) and total blocks of the 1-D launch
-    int ty;                // rows per tile (block = 64 x ty threads)
+    int ty;                // waves per block (block = 64 x ty threads)
+    int wx;                // waves side by side along x: a tile is 64*vx*wx x ty/wx cells
     int vx;                // cells per lane along x (tile = 64*vx x ty cells)
     unsigned zero;         // 0, opaque to the compiler (round32)
     int zb, ze;            // planes [zb, ze) updated by this launch
@@ -327,8 +328,9 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     a.ty = b.ty > 0 ? b.ty : 4;
     a.vx = (b.vx > 0 && b.order != 4) ? b.vx : 1;  // 4th order: one cell per lane
     a.zero = 0u;
-    a.gx = (b.g.nx + 64 * a.vx - 1) / (64 * a.vx);
-    a.gy = (b.g.ny + a.ty - 1) / a.ty;
+    a.wx = (b.wx == 2 && a.ty % 2 == 0) ? 2 : 1;
+    a.gx = (b.g.nx + 64 * a.vx * a.wx - 1) / (64 * a.vx * a.wx);
+    a.gy = (b.g.ny + a.ty / a.wx - 1) / (a.ty / a.wx);
     const int halo = b.order == 4 ? 2 : 1;  // the outermost layers that are never updated
     a.zb = halo;
     a.ze = b.g.nz - halo;
@@ -659,8 +661,9 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
     using R = typename P::R;
     static_assert(ORD == 2 || (ORD == 4 && N == 1), "4th order runs one cell per lane");
     constexpr int HALO = ORD / 2;  // the outermost HALO layers are never updated
-    const int xb = tl.bx * 64 * N;
-    const int iy = tl.by * a.ty + threadIdx.y;
+    // wave threadIdx.y owns one row segment of the tile (wx segments side by side)
+    const int xb = (tl.bx * a.wx + (int)threadIdx.y % a.wx) * 64 * N;
+    const int iy = tl.by * (a.ty / a.wx) + (int)threadIdx.y / a.wx;
     if (iy < HALO || iy > a.ny - 1 - HALO || xb > a.nx - 1 - HALO) return;  // the whole wave (one row)
     const int z0 = a.zb + tl.bz * a.zc;
     const int z1 = min(z0 + a.zc, a.ze);
@@ -919,8 +922,9 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     using R = typename P::R;
     static_assert(ORD == 2 || (ORD == 4 && N == 1), "4th order runs one cell per lane");
     constexpr int HALO = ORD / 2;
-    const int xb = tl.bx * 64 * N;
-    const int iy = tl.by * a.ty + threadIdx.y;
+    // wave threadIdx.y owns one row segment of the tile (wx segments side by side)
+    const int xb = (tl.bx * a.wx + (int)threadIdx.y % a.wx) * 64 * N;
+    const int iy = tl.by * (a.ty / a.wx) + (int)threadIdx.y / a.wx;
     if (iy < HALO || iy > a.ny - 1 - HALO || xb > a.nx - 1 - HALO) return;  // the whole wave (one row)
     const int z0 = a.zb + tl.bz * a.zc;
     const int z1 = min(z0 + a.zc, a.ze);
@@ -1721,7 +1725,8 @@ hipError_t launch_velocity(const DevBuffers& b, hipStream_t st, int zb, int ze)
 {
     if (ze <= zb) return hipSuccess;
     DevBuffers bv = b;
-    bv.vx = b.vx_vel;  // the velocity kernel's own row-segment width
+    bv.vx = b.vx_vel;  // the velocity kernel's own row-segment width and tile shape
+    bv.wx = b.wx_vel;
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
         const StepArgs<P> a = range_args<P>(bv, zb, ze);

@@ -222,6 +222,8 @@ SCHEDULES = (
     {"PW_SLAB": "3"},                 # z-slab interleave of velocity and stress launches
     {"PW_SLAB": "8", "PW_VX": "2"},
     {"PW_FUSED": "1"},                # one fused launch per step
+    {"PW_WX": "2"},                   # two waves side by side along x (128-cell tiles)
+    {"PW_WX": "1", "PW_VX": "1"},     # every kernel one wave per 64-cell row
 )
 
 
@@ -233,7 +235,7 @@ def test_step_schedules_are_bit_identical(layered, monkeypatch):
     for prec in ("fp32", "comp16", "comp16f", "bf16", "fp64"):
         outs = []
         for env in SCHEDULES:
-            for k in ("PW_FUSED", "PW_SLAB", "PW_VX", "PW_TY"):
+            for k in ("PW_FUSED", "PW_SLAB", "PW_VX", "PW_TY", "PW_WX"):
                 monkeypatch.delenv(k, raising=False)
             for k, v in env.items():
                 monkeypatch.setenv(k, v)
