@@ -123,6 +123,15 @@ def test_config_validation_errors(monkeypatch):
     with pytest.raises(pw.PorowaveError) as ei:
         pw.Solver(16, 40, 40, 6, 5.0, 5e-4, 10, space_order=4)  # 4th order needs 2*np+5 cells
     assert ei.value.status == 1
+    # 1152 x 1100 x 1200 padded cells (1.52e9) fit 32-bit offsets for FP32, but a
+    # compensated field's residual bytes, addressed 2 B per cell past its FP16
+    # values, would not
+    with pytest.raises(pw.PorowaveError) as ei:
+        pw.Solver(1100, 1100, 1200, 20, 5.0, 5e-4, 10, precision="comp16")
+    assert ei.value.status == 6 and "2^32 / 3" in str(ei.value)
+    with pytest.raises(pw.PorowaveError) as ei:
+        pw.Solver(1700, 1700, 1600, 20, 5.0, 5e-4, 10, precision="fp32")
+    assert ei.value.status == 6 and "2^32 padded cells" in str(ei.value)
     monkeypatch.setenv("PW_FUSED", "1")
     with pytest.raises(pw.PorowaveError) as ei:
         pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, space_order=4)
