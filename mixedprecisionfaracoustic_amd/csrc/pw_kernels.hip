@@ -324,7 +324,6 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     a.nx = b.g.nx; a.ny = b.g.ny; a.nz = b.g.nz; a.pitch = b.g.pitch;
     a.plane = (unsigned)b.g.plane;
     a.zc = b.zc;
-    a.gx = (b.g.nx + 63) / 64;
     a.ty = b.ty > 0 ? b.ty : 4;
     a.vx = (b.vx > 0 && b.order != 4) ? b.vx : 1;  // 4th order: one cell per lane
     a.zero = 0u;
