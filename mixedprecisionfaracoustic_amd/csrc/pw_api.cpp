@@ -183,6 +183,7 @@ struct Layout {
     float isp = 1.0f;
     std::vector<size_t> off;
     size_t zero_bytes = 0, total = 0;
+    size_t pad = 0;   // bytes between consecutive arrays
 };
 
 void plan_layout(const pw_config& c, int fused, Layout& L)
@@ -223,16 +224,22 @@ void plan_layout(const pw_config& c, int fused, Layout& L)
     const size_t rb = prec == PW_FP64 ? 8 : 4;
     L.off.clear();
     L.total = 0;
+    // 8 KiB between consecutive arrays: the same cell of every array then no
+    // longer sits a power-of-two distance apart (256 MiB per FP16 field at
+    // 512^3), which measured 0.8 % (FP32) to 1.5 % (comp16) faster
+    int pad = 8192;
+    env_int("PW_POOL_PAD", &pad);
+    L.pad = align256((size_t)std::max(0, pad));
     auto take = [&](size_t bytes) {
         L.off.push_back(L.total);
-        L.total += align256(bytes);
+        L.total += align256(bytes) + L.pad;
     };
     const bool c16 = has_lo(prec);
     const int nbuf = fused ? 2 : 1;  // the fused step reads time n and writes time n+1
     for (int b2 = 0; b2 < nbuf; ++b2) {
         // compensated fields: FP16 values, then the residual bytes at the same
         // offset (align256(2 * cells)) for every field
-        for (int k = 0; k < PW_NFIELDS; ++k) take(c16 ? align256(cells * 2) + cells : cells * sb);
+        for (int k = 0; k < PW_NFIELDS; ++k) take(c16 ? align256(cells * 2) + L.pad + cells : cells * sb);
         for (int ax = 0; ax < 3; ++ax)
             for (int k = 0; k < kMemPerAxis; ++k) take((size_t)g.slab_cells[ax] * mb);
     }
@@ -303,7 +310,7 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     s->state_bytes_total = L.zero_bytes;
     char* base = (char*)s->pool;
     size_t i = 0;
-    const size_t loff = align256((size_t)g.cells * 2);  // residual bytes after a field's FP16 values
+    const size_t loff = align256((size_t)g.cells * 2) + L.pad;  // residual bytes after a field's FP16 values
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.f[k] = base + off[i++];
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = c16 ? (int8_t*)((char*)s->b.f[k] + loff) : nullptr;
     for (int ax = 0; ax < 3; ++ax)
