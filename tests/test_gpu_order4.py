@@ -90,6 +90,25 @@ def test_order4_reduced_precisions(case4):
             assert e[c] <= tol, (prec, c, e[c])
 
 
+def test_order4_tilings_are_bit_identical(case4, monkeypatch):
+    """The 4th-order kernels tile like the 2nd-order ones (BF16's velocity kernel
+    puts two waves side by side by default): one or two waves per tile row, and
+    64 x 2 or 64 x 8 tiles, give the same bits."""
+    grid, base, nt, recs, _, _ = case4
+    for prec in ("bf16", "fp32", "comp16"):
+        outs = []
+        for env in ({}, {"PW_WX": "1"}, {"PW_WX": "2"}, {"PW_TY": "8"}, {"PW_TY": "2", "PW_WX": "2"}):
+            for k in ("PW_WX", "PW_TY"):
+                monkeypatch.delenv(k, raising=False)
+            for k, v in env.items():
+                monkeypatch.setenv(k, v)
+            outs.append((env, run_gpu4(grid, nt, prec, recs, base)))
+        for env, g in outs[1:]:
+            assert np.array_equal(g["seis"], outs[0][1]["seis"]), (prec, env)
+            for f in FIELDS:
+                assert np.array_equal(g["fields"][f], outs[0][1]["fields"][f]), (prec, env, f)
+
+
 def test_order4_converges_faster():
     """Homogeneous sandstone, the same physical source (moment scaled by the
     cell volume) and receiver (120 m from the source along x) on grids of 10 m
