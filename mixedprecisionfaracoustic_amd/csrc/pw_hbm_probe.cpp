@@ -12,6 +12,8 @@
 //   read 12      12 reads, one value per thread written at the end
 //   step-like    the velocity kernel's mix with 4-byte accesses: 17 arrays read
 //                (13 wavefields + 4 coefficients), 6 of them rewritten in place
+//   step-like, interleaved  the same accesses with the 17 arrays stored as one:
+//                each 64-value row segment of array k at (row * 17 + k) * 64
 // Rates count the bytes each pattern must move (every array once), timed
 // with HIP events over repeated launches, best over a few grid sizes.
 //
@@ -90,6 +92,19 @@ __global__ __launch_bounds__(kThreads) void k_step_like(StepLike a, size_t n)
     }
 }
 
+// the same 17 arrays interleaved by 64-value row segments in one buffer
+__global__ __launch_bounds__(kThreads) void k_step_like_aosoa(float* __restrict__ buf, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const size_t base = (i / 64) * 17 * 64 + (i % 64);
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 6; k < 17; ++k) acc += buf[base + (size_t)k * 64];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) buf[base + (size_t)k * 64] = buf[base + (size_t)k * 64] * 0.5f + acc * 1e-3f;
+    }
+}
+
 template <class F>
 double best_rate(F launch, double bytes, int iters, const int* grids, int ngrids, int* best_grid)
 {
@@ -137,7 +152,7 @@ int main(int argc, char** argv)
     for (int r = 0; r < kMaxIn; ++r) in.a[r] = buf[r];
     float4* out = buf[kMaxIn];
 
-    struct Row { const char* name; double rate; int grid; } rows[6];
+    struct Row { const char* name; double rate; int grid; } rows[7];
     int g = 0;
     rows[0] = {"copy", best_rate([&](int b) { k_stream<1><<<b, kThreads>>>(in, out, n4); }, 2 * arr, iters, grids, ngrids, &g), g};
     rows[1] = {"stream_2r1w", best_rate([&](int b) { k_stream<2><<<b, kThreads>>>(in, out, n4); }, 3 * arr, iters, grids, ngrids, &g), g};
@@ -157,6 +172,13 @@ int main(int argc, char** argv)
     rows[5] = {"step_like", best_rate([&](int b) { k_step_like<<<b, kThreads>>>(sl, n); }, 23 * arr, iters, grids, ngrids, &g), g};
 
     for (int r = 0; r < 5; ++r) CHECK(hipFree(extra[r]));
+    // the interleaved layout of the same 17 arrays (n is a multiple of 64)
+    float* inter = nullptr;
+    CHECK(hipMalloc(&inter, 17 * n * sizeof(float)));
+    k_fill<<<4096, kThreads>>>((float4*)inter, 17 * n4, 1e-3f);
+    rows[6] = {"step_like_interleaved",
+               best_rate([&](int b) { k_step_like_aosoa<<<b, kThreads>>>(inter, n); }, 23 * arr, iters, grids, ngrids, &g), g};
+    CHECK(hipFree(inter));
     // the 2:1 stream of arrays filled with 1 and 2 must give 3
     for (int r = 0; r < 3; ++r) k_fill<<<4096, kThreads>>>(buf[r], n4, 1.0f + r);
     k_stream<2><<<1024, kThreads>>>(in, out, n4);
@@ -168,7 +190,7 @@ int main(int argc, char** argv)
     }
     for (const Row& r : rows) std::printf("%-14s %6d blocks %8.1f GB/s\n", r.name, r.grid, r.rate);
     std::printf("{");
-    for (int i = 0; i < 6; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
+    for (int i = 0; i < 7; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
     std::printf(", \"unit\": \"GB/s\", \"array_bytes\": %.0f, \"iters\": %d}\n", arr, iters);
     for (int r = 0; r <= kMaxIn; ++r) CHECK(hipFree(buf[r]));
     CHECK(hipFree(red));
