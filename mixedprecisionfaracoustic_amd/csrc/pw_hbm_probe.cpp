@@ -14,6 +14,8 @@
 //                (13 wavefields + 4 coefficients), 6 of them rewritten in place
 //   step-like, interleaved  the same accesses with the 17 arrays stored as one:
 //                each 64-value row segment of array k at (row * 17 + k) * 64
+//   step-like, out-of-place  17 arrays read, the 6 results written to 6 other
+//                arrays instead of in place
 // Rates count the bytes each pattern must move (every array once), timed
 // with HIP events over repeated launches, best over a few grid sizes.
 //
@@ -92,6 +94,22 @@ __global__ __launch_bounds__(kThreads) void k_step_like(StepLike a, size_t n)
     }
 }
 
+// 17 arrays read, 6 results written to separate arrays
+struct StepLikeOut {
+    const float* ro[17];
+    float* wo[6];
+};
+__global__ __launch_bounds__(kThreads) void k_step_like_out(StepLikeOut a, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 6; k < 17; ++k) acc += a.ro[k][i];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a.wo[k][i] = a.ro[k][i] * 0.5f + acc * 1e-3f;
+    }
+}
+
 // the same 17 arrays interleaved by 64-value row segments in one buffer
 __global__ __launch_bounds__(kThreads) void k_step_like_aosoa(float* __restrict__ buf, size_t n)
 {
@@ -152,7 +170,7 @@ int main(int argc, char** argv)
     for (int r = 0; r < kMaxIn; ++r) in.a[r] = buf[r];
     float4* out = buf[kMaxIn];
 
-    struct Row { const char* name; double rate; int grid; } rows[7];
+    struct Row { const char* name; double rate; int grid; } rows[8];
     int g = 0;
     rows[0] = {"copy", best_rate([&](int b) { k_stream<1><<<b, kThreads>>>(in, out, n4); }, 2 * arr, iters, grids, ngrids, &g), g};
     rows[1] = {"stream_2r1w", best_rate([&](int b) { k_stream<2><<<b, kThreads>>>(in, out, n4); }, 3 * arr, iters, grids, ngrids, &g), g};
@@ -170,6 +188,16 @@ int main(int argc, char** argv)
     for (int r = 0; r < 6; ++r) sl.ro[r] = (const float*)buf[6 + r];
     for (int r = 0; r < 5; ++r) sl.ro[6 + r] = extra[r];
     rows[5] = {"step_like", best_rate([&](int b) { k_step_like<<<b, kThreads>>>(sl, n); }, 23 * arr, iters, grids, ngrids, &g), g};
+    // out of place: the 17 inputs as above, the 6 outputs in buffers 12 and the 5 extras
+    StepLikeOut so;
+    for (int r = 0; r < 12; ++r) so.ro[r] = (const float*)buf[r];
+    for (int r = 0; r < 5; ++r) so.ro[12 + r] = extra[r];
+    float* outs[6];
+    for (int r = 0; r < 6; ++r) CHECK(hipMalloc(&outs[r], n * sizeof(float)));
+    for (int r = 0; r < 6; ++r) so.wo[r] = outs[r];
+    rows[7] = {"step_like_out_of_place",
+               best_rate([&](int b) { k_step_like_out<<<b, kThreads>>>(so, n); }, 23 * arr, iters, grids, ngrids, &g), g};
+    for (int r = 0; r < 6; ++r) CHECK(hipFree(outs[r]));
 
     for (int r = 0; r < 5; ++r) CHECK(hipFree(extra[r]));
     // the interleaved layout of the same 17 arrays (n is a multiple of 64)
@@ -190,7 +218,7 @@ int main(int argc, char** argv)
     }
     for (const Row& r : rows) std::printf("%-14s %6d blocks %8.1f GB/s\n", r.name, r.grid, r.rate);
     std::printf("{");
-    for (int i = 0; i < 7; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
+    for (int i = 0; i < 8; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
     std::printf(", \"unit\": \"GB/s\", \"array_bytes\": %.0f, \"iters\": %d}\n", arr, iters);
     for (int r = 0; r <= kMaxIn; ++r) CHECK(hipFree(buf[r]));
     CHECK(hipFree(red));
