@@ -81,9 +81,10 @@ def measured_traffic(precision, kernel, n):
     return None
 
 
-def cpu_baseline(n, np_, spec_base, dt, h, steps):
+def cpu_baseline(n, np_, spec_base, dt, h, steps, bytes_per_cell_step):
     """The oracle (OpenMP C restatement of Float/main.cu) on the same grid and medium,
-    slab-CPML mode, timed on a bounded number of steps."""
+    slab-CPML mode, timed on a bounded number of steps; its effective GB/s is priced
+    at the same bytes per cell-step as the GPU line."""
     import numpy as np
     from oracle import pw_oracle as O
     grid = {"nx": n, "ny": n, "nz": n, "np": np_, "h": h, "dt": dt, "rc": 1e-4, "v0": 4000.0, "f0": 25.0,
@@ -93,7 +94,9 @@ def cpu_baseline(n, np_, spec_base, dt, h, steps):
     sec, thr = O.time_f32(grid, base, 1, steps)
     total = time.perf_counter() - t0
     cells = float(n) ** 3
-    return {"value": round(cells * steps / sec / 1e9, 5), "unit": "Gcell-steps/s", "cores": thr, "kind": "port",
+    rate = cells * steps / sec / 1e9
+    return {"value": round(rate, 5), "unit": "Gcell-steps/s", "cores": thr, "kind": "port",
+            "effective_gb_s": round(rate * bytes_per_cell_step, 2),
             "sample": f"oracle FP32 (OpenMP C restatement of Float/main.cu FD_V/FD_T/Source, slab CPML) on "
                       f"the same {n}^3 medium: {steps} timed steps after 1 warm-up step, {sec:.2f} s timed "
                       f"({total:.1f} s with set-up)"}
@@ -527,7 +530,7 @@ def main():
         s2.set_medium_layered(spec)
         base = s2.base_fields()
         s2.close()
-        out["cpu_baseline"] = cpu_baseline(n, np_, base, dt, h, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(n, np_, base, dt, h, args.cpu_steps, info["bytes_per_cell_step"])
         del base
     print(json.dumps(out), flush=True)
     return 0
