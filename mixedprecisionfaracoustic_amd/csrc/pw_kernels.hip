@@ -10,12 +10,15 @@
 //
 // Layout: one padded SoA array per wavefield (row pitch = nx rounded to 64
 // elements).  A block is a 64 x 4 (x, y) column tile that marches up a chunk of
-// z; each wave owns one 64-cell x row, so every load is a coalesced 256 B (FP32)
-// row segment.  x neighbours come from the adjacent lane (one DPP move; only
+// z (8 planes from 256 planes up); each wave owns one 64-cell x row, so every
+// load is a coalesced 256 B (FP32) row segment.  (Tunable per kernel: N cells
+// per lane, wx waves side by side along x; comp16's velocity kernel uses N = 2,
+// BF16's wx = 2.)  x neighbours come from the adjacent lane (one DPP move; only
 // the edge lane loads), y neighbours hit L1/L2, and z neighbours are carried in
 // registers from the previous iteration (each value is loaded once per block).
 // A plane's loads are issued ahead of its CPML memory-variable stores, which
-// the compiler cannot prove disjoint from them.
+// the compiler cannot prove disjoint from them.  Each field puts one pointer
+// into the kernels' scalar registers (FA below).
 //
 // Also here: the fused one-launch step (opt-in), the batched kernels of the
 // shot server (one launch per kernel over many shots), the rescale, snapshot,
