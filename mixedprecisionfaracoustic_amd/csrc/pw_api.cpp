@@ -333,12 +333,13 @@ pw_status pw::solver_create(const pw_config* cfg, int fused, int slab, pw_solver
     int ty = 4;
     env_int("PW_TY", &ty);
     s->b.ty = (ty == 1 || ty == 2 || ty == 4 || ty == 8) ? ty : 4;
-    // waves side by side along x per tile: BF16's velocity kernel runs fastest
-    // with two (128-cell tiles), every other kernel with one
+    // waves side by side along x per tile: BF16's velocity kernel and FP32's
+    // stress kernel run fastest with two (128-cell tiles), every other kernel
+    // with one
     int wx = 0;
     env_int("PW_WX", &wx);
     const bool wx_set = wx == 1 || wx == 2;
-    s->b.wx = wx_set ? wx : 1;
+    s->b.wx = wx_set ? wx : (s->prec == PW_FP32 ? 2 : 1);
     s->b.wx_vel = wx_set ? wx : (s->prec == PW_BF16 ? 2 : 1);
     // cells per lane along x: wider row segments per wave
     int vx = 0;
