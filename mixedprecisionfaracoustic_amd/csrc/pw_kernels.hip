@@ -13,7 +13,7 @@
 // z (8 planes from 256 planes up); each wave owns one 64-cell x row, so every
 // load is a coalesced 256 B (FP32) row segment.  (Tunable per kernel: N cells
 // per lane, wx waves side by side along x; comp16's velocity kernel uses N = 2,
-// BF16's wx = 2.)  x neighbours come from the adjacent lane (one DPP move; only
+// BF16's velocity and FP32's stress kernel wx = 2.)  x neighbours come from the adjacent lane (one DPP move; only
 // the edge lane loads), y neighbours hit L1/L2, and z neighbours are carried in
 // registers from the previous iteration (each value is loaded once per block).
 // A plane's loads are issued ahead of its CPML memory-variable stores, which
