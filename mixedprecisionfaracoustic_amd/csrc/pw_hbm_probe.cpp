@@ -16,12 +16,16 @@
 //                each 64-value row segment of array k at (row * 17 + k) * 64
 //   step-like, out-of-place  17 arrays read, the 6 results written to 6 other
 //                arrays instead of in place
+//   comp16-like  the velocity kernel's compensated-FP16 mix: 13 fields of 2-byte
+//                values and 1-byte residuals (6 rewritten in place) and 15
+//                2-byte coefficient arrays, 1 or 2 cells per thread
 // Rates count the bytes each pattern must move (every array once), timed
 // with HIP events over repeated launches, best over a few grid sizes.
 //
 // Output: one line per pattern, "pattern blocks GB/s", then a JSON summary.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 
@@ -110,6 +114,49 @@ __global__ __launch_bounds__(kThreads) void k_step_like_out(StepLikeOut a, size_
     }
 }
 
+// compensated-FP16 mix; N consecutive cells per thread (one 2N- / N-byte access)
+template <class T, int N> struct alignas(sizeof(T) * N) Vec { T v[N]; };
+struct Comp16Like {
+    uint16_t* hi[13];
+    int8_t* lo[13];
+    const uint16_t* co[15];
+};
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_comp16_like(Comp16Like a, size_t n)
+{
+    using H = Vec<uint16_t, N>;
+    using L = Vec<int8_t, N>;
+    for (size_t c = blockIdx.x * (size_t)kThreads + threadIdx.x; c * N < n; c += (size_t)gridDim.x * kThreads) {
+        const size_t i = c * N;
+        unsigned acc[N] = {};
+#pragma unroll
+        for (int k = 6; k < 13; ++k) {
+            const H h = *(const H*)(a.hi[k] + i);
+            const L l = *(const L*)(a.lo[k] + i);
+#pragma unroll
+            for (int j = 0; j < N; ++j) acc[j] += h.v[j] + (unsigned)l.v[j];
+        }
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+            const H h = *(const H*)(a.co[k] + i);
+#pragma unroll
+            for (int j = 0; j < N; ++j) acc[j] += h.v[j];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            H h = *(const H*)(a.hi[k] + i);
+            L l = *(const L*)(a.lo[k] + i);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                h.v[j] = (uint16_t)(h.v[j] + acc[j]);
+                l.v[j] = (int8_t)(l.v[j] + (int)acc[j]);
+            }
+            *(H*)(a.hi[k] + i) = h;
+            *(L*)(a.lo[k] + i) = l;
+        }
+    }
+}
+
 // the same 17 arrays interleaved by 64-value row segments in one buffer
 __global__ __launch_bounds__(kThreads) void k_step_like_aosoa(float* __restrict__ buf, size_t n)
 {
@@ -170,7 +217,7 @@ int main(int argc, char** argv)
     for (int r = 0; r < kMaxIn; ++r) in.a[r] = buf[r];
     float4* out = buf[kMaxIn];
 
-    struct Row { const char* name; double rate; int grid; } rows[8];
+    struct Row { const char* name; double rate; int grid; } rows[10];
     int g = 0;
     rows[0] = {"copy", best_rate([&](int b) { k_stream<1><<<b, kThreads>>>(in, out, n4); }, 2 * arr, iters, grids, ngrids, &g), g};
     rows[1] = {"stream_2r1w", best_rate([&](int b) { k_stream<2><<<b, kThreads>>>(in, out, n4); }, 3 * arr, iters, grids, ngrids, &g), g};
@@ -200,6 +247,23 @@ int main(int argc, char** argv)
     for (int r = 0; r < 6; ++r) CHECK(hipFree(outs[r]));
 
     for (int r = 0; r < 5; ++r) CHECK(hipFree(extra[r]));
+    // the compensated-FP16 mix: 13 x 3 B + 15 x 2 B read, 6 x 3 B written per cell
+    {
+        Comp16Like c16;
+        void* blocks[13 + 13 + 15];
+        int nb = 0;
+        for (int k = 0; k < 13; ++k) {
+            CHECK(hipMalloc(&blocks[nb], n * 2)); c16.hi[k] = (uint16_t*)blocks[nb++];
+            CHECK(hipMalloc(&blocks[nb], n)); c16.lo[k] = (int8_t*)blocks[nb++];
+        }
+        for (int k = 0; k < 15; ++k) { CHECK(hipMalloc(&blocks[nb], n * 2)); c16.co[k] = (const uint16_t*)blocks[nb++]; }
+        for (int k = 0; k < nb; ++k) CHECK(hipMemset(blocks[k], 0, k < 26 && (k % 2) ? n : n * 2));
+        CHECK(hipDeviceSynchronize());
+        const double bytes = (13.0 * 3 + 15.0 * 2 + 6.0 * 3) * (double)n;
+        rows[8] = {"comp16_like", best_rate([&](int b) { k_comp16_like<1><<<b, kThreads>>>(c16, n); }, bytes, iters, grids, ngrids, &g), g};
+        rows[9] = {"comp16_like_2cells", best_rate([&](int b) { k_comp16_like<2><<<b, kThreads>>>(c16, n); }, bytes, iters, grids, ngrids, &g), g};
+        for (int k = 0; k < nb; ++k) CHECK(hipFree(blocks[k]));
+    }
     // the interleaved layout of the same 17 arrays (n is a multiple of 64)
     float* inter = nullptr;
     CHECK(hipMalloc(&inter, 17 * n * sizeof(float)));
@@ -218,7 +282,7 @@ int main(int argc, char** argv)
     }
     for (const Row& r : rows) std::printf("%-14s %6d blocks %8.1f GB/s\n", r.name, r.grid, r.rate);
     std::printf("{");
-    for (int i = 0; i < 8; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
+    for (int i = 0; i < 10; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
     std::printf(", \"unit\": \"GB/s\", \"array_bytes\": %.0f, \"iters\": %d}\n", arr, iters);
     for (int r = 0; r <= kMaxIn; ++r) CHECK(hipFree(buf[r]));
     CHECK(hipFree(red));
