@@ -356,6 +356,44 @@ def rank_share_main(args):
     return 0
 
 
+def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_steps):
+    """The same workload in another storage precision, timed the same way: its
+    throughput, step time, kernel times and dominant-kernel roofline."""
+    nt = warmup + steps + prof_steps
+    stream = torch.cuda.Stream()
+    s = pw.Solver(n, n, n, np_, h, dt, nt, precision=precision, cpml_r=1e-4, cpml_v0=4000.0,
+                  stream=stream.cuda_stream)
+    s.set_medium_layered(spec)
+    s.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
+    fields, idx = receivers(n, np_)
+    s.set_receivers(fields, idx)
+    info = s.info()
+    s.run(warmup)
+    s.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    s.run(steps)
+    ev1.record(stream)
+    s.synchronize()
+    torch.cuda.synchronize()
+    ms_total = ev0.elapsed_time(ev1)
+    s.set_profiling(True)
+    s.run(min(prof_steps, nt - s.step))
+    ms_v, ms_t, launches = s.kernel_times()
+    s.close()
+    cells = float(n) ** 3
+    avg_v, avg_t = ms_v / launches, ms_t / launches
+    kname, kbytes, kms = (("k_velocity", info["bytes_velocity"] * cells, avg_v) if ms_v >= ms_t
+                          else ("k_stress", info["bytes_stress"] * cells, avg_t))
+    achieved = kbytes / (kms / 1e3) / 1e9
+    return {"value": round(cells * steps / (ms_total / 1e3) / 1e9, 4), "ms_per_step": round(ms_total / steps, 4),
+            "kernel_ms": {"k_velocity": round(avg_v, 4), "k_stress": round(avg_t, 4)},
+            "roofline": {"kernel": kname, "achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4)},
+            "bytes_per_cell_step": round(info["bytes_per_cell_step"], 2)}
+
+
 def hbm_ceiling():
     """Run the standalone bandwidth probe built next to the library (plain copy
     and 2/6/12-read streaming kernels over 256 MiB arrays) as a child process,
@@ -383,6 +421,8 @@ def main():
     ap.add_argument("--np", type=int, default=20)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "comp16", "comp16f", "bf16", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-comp16", action="store_true",
+                    help="skip the compensated-FP16 run reported beside the FP32 line")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--prof-steps", type=int, default=40)
     ap.add_argument("--serving", action="store_true", help="replay the shot-serving trace instead")
@@ -524,6 +564,11 @@ def main():
         out["volume_snapshots"] = {"every": args.volume_every, "in_timed_steps": len(timed),
                                    "bytes_each": int(cells * 4)}
     solver.close()
+    if args.precision == "fp32" and not args.no_comp16:
+        # the metric's comparison: compensated FP16 (the reference Half format) on the same workload
+        c16 = side_run(pw, torch, "comp16", n, np_, h, dt, f0, spec, args.steps, args.warmup, args.prof_steps)
+        c16["ratio_vs_fp32"] = round(c16["value"] / gcell, 4)
+        out["comp16"] = c16
     if not args.no_cpu_baseline and not args.large:
         base = None
         s2 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
