@@ -16,6 +16,8 @@
 //                each 64-value row segment of array k at (row * 17 + k) * 64
 //   step-like, out-of-place  17 arrays read, the 6 results written to 6 other
 //                arrays instead of in place
+//   stress-like  the stress kernel's FP32 mix: 18 arrays read (6 velocities,
+//                7 stresses, 5 coefficients), the 7 stresses rewritten in place
 //   comp16-like  the velocity kernel's compensated-FP16 mix: 13 fields of 2-byte
 //                values and 1-byte residuals (6 rewritten in place) and 15
 //                2-byte coefficient arrays, 1 or 2 cells per thread
@@ -111,6 +113,22 @@ __global__ __launch_bounds__(kThreads) void k_step_like_out(StepLikeOut a, size_
         for (int k = 6; k < 17; ++k) acc += a.ro[k][i];
 #pragma unroll
         for (int k = 0; k < 6; ++k) a.wo[k][i] = a.ro[k][i] * 0.5f + acc * 1e-3f;
+    }
+}
+
+// 18 reads of separate arrays, the first 7 rewritten in place
+struct StressLike {
+    float* rw[7];
+    const float* ro[11];
+};
+__global__ __launch_bounds__(kThreads) void k_stress_like(StressLike a, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) acc += a.ro[k][i];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) a.rw[k][i] = a.rw[k][i] * 0.5f + acc * 1e-3f;
     }
 }
 
@@ -217,7 +235,7 @@ int main(int argc, char** argv)
     for (int r = 0; r < kMaxIn; ++r) in.a[r] = buf[r];
     float4* out = buf[kMaxIn];
 
-    struct Row { const char* name; double rate; int grid; } rows[10];
+    struct Row { const char* name; double rate; int grid; } rows[11];
     int g = 0;
     rows[0] = {"copy", best_rate([&](int b) { k_stream<1><<<b, kThreads>>>(in, out, n4); }, 2 * arr, iters, grids, ngrids, &g), g};
     rows[1] = {"stream_2r1w", best_rate([&](int b) { k_stream<2><<<b, kThreads>>>(in, out, n4); }, 3 * arr, iters, grids, ngrids, &g), g};
@@ -245,6 +263,19 @@ int main(int argc, char** argv)
     rows[7] = {"step_like_out_of_place",
                best_rate([&](int b) { k_step_like_out<<<b, kThreads>>>(so, n); }, 23 * arr, iters, grids, ngrids, &g), g};
     for (int r = 0; r < 6; ++r) CHECK(hipFree(outs[r]));
+    // the stress kernel's mix: buffers 0-6 rewritten, 7-11 and the 5 extras and one more read
+    {
+        float* more = nullptr;
+        CHECK(hipMalloc(&more, n * sizeof(float)));
+        k_fill<<<4096, kThreads>>>((float4*)more, n4, 1e-3f);
+        StressLike st;
+        for (int r = 0; r < 7; ++r) st.rw[r] = (float*)buf[r];
+        for (int r = 0; r < 5; ++r) st.ro[r] = (const float*)buf[7 + r];
+        for (int r = 0; r < 5; ++r) st.ro[5 + r] = extra[r];
+        st.ro[10] = more;
+        rows[10] = {"stress_like", best_rate([&](int b) { k_stress_like<<<b, kThreads>>>(st, n); }, 25 * arr, iters, grids, ngrids, &g), g};
+        CHECK(hipFree(more));
+    }
 
     for (int r = 0; r < 5; ++r) CHECK(hipFree(extra[r]));
     // the compensated-FP16 mix: 13 x 3 B + 15 x 2 B read, 6 x 3 B written per cell
@@ -282,7 +313,7 @@ int main(int argc, char** argv)
     }
     for (const Row& r : rows) std::printf("%-14s %6d blocks %8.1f GB/s\n", r.name, r.grid, r.rate);
     std::printf("{");
-    for (int i = 0; i < 10; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
+    for (int i = 0; i < 11; ++i) std::printf("%s\"%s\": %.1f", i ? ", " : "", rows[i].name, rows[i].rate);
     std::printf(", \"unit\": \"GB/s\", \"array_bytes\": %.0f, \"iters\": %d}\n", arr, iters);
     for (int r = 0; r <= kMaxIn; ++r) CHECK(hipFree(buf[r]));
     CHECK(hipFree(red));
