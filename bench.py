@@ -412,6 +412,59 @@ def hbm_ceiling():
     return 0
 
 
+def host_job_main(args):
+    """The whole job as a caller that holds its medium in host memory runs it:
+    create the solver, hand over the nine cell-centred FP64 base volumes
+    (pw_set_medium: upload from pageable memory plus the coefficient
+    precompute), set source, receivers and snapshots, run every step, and read
+    the seismograms and snapshots back. Its rate includes the PCIe transfers
+    and set-up; the bench line's value does not (inputs resident in HBM)."""
+    import numpy as np
+    import torch
+    import mixedprecisionfaracoustic_amd as pw
+    n, np_, h, dt, f0 = args.n, args.np, 5.0, 0.5e-3, 25.0
+    nt = args.job_steps
+    spec = workload(n, np_)
+    s0 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
+    s0.set_medium_layered(spec)
+    base = s0.base_fields()  # the caller's host buffers (not timed)
+    s0.close()
+    fields, idx = receivers(n, np_)
+    snap_steps = [s for s in range(500, nt, 500)] or [nt - 1]
+    cells = float(n) ** 3
+    out = {"metric": "host-to-host job rate (PCIe and set-up included)", "unit": "Gcell-steps/s",
+           "config": {"workload": f"biot_two_layer_{n}^3", "steps": nt, "receivers": len(idx),
+                      "snapshots": len(snap_steps), "medium_bytes": int(base.nbytes)}}
+    for prec in args.job_precisions.split(","):
+        torch.cuda.synchronize()
+        t = [time.perf_counter()]
+        s = pw.Solver(n, n, n, np_, h, dt, nt, precision=prec, cpml_r=1e-4, cpml_v0=4000.0)
+        t.append(time.perf_counter())
+        s.set_medium(base)
+        t.append(time.perf_counter())
+        s.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
+        s.set_receivers(fields, idx)
+        s.set_snapshots(snap_steps, ["txx"] * len(snap_steps), n // 2)
+        s.run(nt)
+        s.synchronize()
+        t.append(time.perf_counter())
+        seis = s.seismogram()
+        snaps = s.snapshots()
+        t.append(time.perf_counter())
+        s.close()
+        t.append(time.perf_counter())
+        job = t[-1] - t[0]
+        out[prec] = {"value": round(cells * nt / job / 1e9, 4), "job_s": round(job, 3),
+                     "stepping_rate": round(cells * nt / (t[3] - t[2]) / 1e9, 4),
+                     "create_s": round(t[1] - t[0], 3), "set_medium_s": round(t[2] - t[1], 3),
+                     "run_s": round(t[3] - t[2], 3), "outputs_s": round(t[4] - t[3], 3),
+                     "close_s": round(t[5] - t[4], 3),
+                     "finite": bool(np.isfinite(seis).all() and np.isfinite(snaps).all())}
+        print(f"host job {prec}: {out[prec]}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -445,6 +498,10 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="the 1024^3 eight-layer model (NP 32, dx 4 m, dt 0.4 ms, 20 Hz) instead of 512^3; "
                          "no CPU baseline (its FP64 base fields alone take 77 GB of host memory)")
+    ap.add_argument("--host-job", action="store_true",
+                    help="time whole jobs from host buffers (set-up, medium upload, run, read-back) instead")
+    ap.add_argument("--job-steps", type=int, default=2000)
+    ap.add_argument("--job-precisions", default="fp32,comp16")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
@@ -453,6 +510,8 @@ def main():
         return hbm_ceiling()
     if args.rank_share:
         return rank_share_main(args)
+    if args.host_job:
+        return host_job_main(args)
     if args.serving:
         return serving_main(args)
     if args.precision_report:
