@@ -7,6 +7,9 @@
   FP32 against the Float oracle (same rule as the other parity tests), FP64
   against the FP64 oracle (1e-10), comp16 against the Half path (1e-3, floor
   rule).
+* Every tiling (1, 2 or 4 cells per lane, 1 or 2 waves per row, 2- to 8-row
+  tiles) bit-identical to the default on grids whose x extent ends inside the
+  last 64-, 128- and 256-cell tile, with the wave crossing those tile edges.
 * A source in the CPML corner cell (1, 1, 1).
 * Receivers on the edge planes, which are never updated: their traces stay 0.
 * No receivers; a zero-step run; running past nt is an error that leaves the
@@ -68,6 +71,48 @@ def test_ragged_grids_match_oracle(shape, np_):
     for k, (f, i) in enumerate(recs):
         if i % nx in (0, nx - 1):
             assert not g32["seis"][k].any(), (f, i)
+
+
+TILINGS = (
+    {"PW_WX": "1", "PW_VX": "1"},  # one wave per 64-cell row in every kernel
+    {"PW_VX": "2"},                # 128-cell row segments, 2 cells per lane
+    {"PW_VX": "4", "PW_TY": "8"},  # 256-cell row segments, 8-row tiles
+    {"PW_WX": "2"},                # two waves per 128-cell row
+    {"PW_WX": "2", "PW_TY": "8"},
+    {"PW_TY": "2"},
+)
+
+
+@pytest.mark.parametrize("shape,np_,src", [((300, 11, 19), 3, (270, 5, 9)), ((130, 9, 13), 2, (100, 4, 6))])
+def test_ragged_tilings_are_bit_identical(shape, np_, src, monkeypatch):
+    """Row tails: 300 cells end 44 cells into the fifth 64-cell tile, the third
+    128-cell one and the second 256-cell one; 130 cells leave 2 cells (both
+    edge or CPML) in the last tile of every width. The source sits near the high
+    x end so that the wave crosses those tile edges within the run."""
+    grid, base, _ = hetero_case(shape, np_, src=src)
+    nt = 80
+    recs = line_receivers(grid)
+    bad = []
+    for prec in ("fp32", "comp16", "comp16f", "bf16", "fp64"):
+        outs = []
+        for env in ({},) + TILINGS:
+            for k in ("PW_VX", "PW_TY", "PW_WX"):
+                monkeypatch.delenv(k, raising=False)
+            for k, v in env.items():
+                monkeypatch.setenv(k, "2" if (prec == "fp64" and k == "PW_VX" and v == "4") else v)
+            g, _ = cases.run_gpu(grid, nt, prec, recs, base=base, fields=FIELDS)
+            outs.append(g)
+        ref = outs[0]
+        assert np.abs(ref["seis"]).max() > 0, prec
+        # the wave has reached the tile edges at 256 (or 128) cells
+        edge = 256 if grid["nx"] == 300 else 128
+        assert np.abs(ref["fields"]["vux"][:, :, edge - 2:edge + 2]).max() > 0, prec
+        for env, g in zip(TILINGS, outs[1:]):
+            same = np.array_equal(g["seis"], ref["seis"]) and all(
+                np.array_equal(g["fields"][f], ref["fields"][f]) for f in FIELDS)
+            if not same:
+                bad.append((prec, env, rel_l2(g["seis"], ref["seis"])))
+    assert not bad, f"tilings differing from the default: {bad}"
 
 
 def test_source_in_cpml_corner_matches_oracle():
