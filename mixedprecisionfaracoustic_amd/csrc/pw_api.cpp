@@ -951,7 +951,8 @@ extern "C" pw_status pw_set_volume_snapshots(pw_solver* s, int n, const int* ste
     if (e != hipSuccess) {
         free_volumes(s);
         (void)hipGetLastError();
-        return fail(PW_ERR_OUT_OF_MEMORY, std::string("pw_set_volume_snapshots: ") + hipGetErrorString(e));
+        return fail(e == hipErrorOutOfMemory ? PW_ERR_OUT_OF_MEMORY : PW_ERR_HIP,
+                    std::string("pw_set_volume_snapshots: ") + hipGetErrorString(e));
     }
     s->vol_step.assign(step, step + n);
     s->vol_field.assign(field, field + n);

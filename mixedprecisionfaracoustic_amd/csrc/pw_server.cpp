@@ -25,6 +25,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "pw_device.h"
@@ -361,6 +362,8 @@ void worker_main(pw_server* srv)
         // 3. retire finished shots
         const double tw = srv->clock();
         std::vector<Shot*> done;
+        // each retired shot's own status: a failed read-back fails that shot only
+        std::vector<std::pair<pw_status, std::string>> outcome;
         if (st == PW_OK) {
             for (Shot* q : srv->active) {
                 q->it += K;
@@ -368,16 +371,12 @@ void worker_main(pw_server* srv)
             }
             for (Shot* q : done) {
                 q->seis.assign((size_t)q->s->nrec * q->req.nt, 0.0f);
-                if (q->s->nrec > 0) {
-                    pw_status g = pw_get_seismogram(q->s, q->seis.data());
-                    if (g != PW_OK) {
-                        st = g;
-                        msg = pw_last_error();
-                    }
-                }
+                pw_status g = q->s->nrec > 0 ? pw_get_seismogram(q->s, q->seis.data()) : PW_OK;
+                outcome.emplace_back(g, g == PW_OK ? std::string() : std::string(pw_last_error()));
             }
         } else {
             done = srv->active;  // a failed round fails every shot in it
+            outcome.assign(done.size(), {st, msg});
         }
         for (Shot* q : done) teardown_shot(q);
         const double tend = srv->clock();
@@ -390,8 +389,9 @@ void worker_main(pw_server* srv)
         srv->stats.graph_s += eng.graph_s;
         eng.launches = eng.graphs_built = 0;
         eng.graph_s = 0.0;
-        for (Shot* q : done) {
-            finish(srv, q, st, msg);
+        for (size_t i = 0; i < done.size(); ++i) {
+            Shot* q = done[i];
+            finish(srv, q, outcome[i].first, outcome[i].second);
             srv->used -= q->bytes;
             srv->active.erase(std::find(srv->active.begin(), srv->active.end(), q));
         }
