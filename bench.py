@@ -12,7 +12,7 @@ Synthetic medium generated on the GPU (no data files).
 A "step" is one pass of the hot path: source -> velocity kernel -> stress
 kernel -> receiver gather.  Inputs are resident in HBM before timing starts.
 
-Prints one JSON line (rank 0).  --gpus > 1 is refused: this build targets one GPU.
+Prints one JSON line (rank 0).  --gpus > 1 is refused: the library runs on one GPU.
 """
 import argparse
 import json
@@ -504,7 +504,7 @@ def main():
     ap.add_argument("--job-precisions", default="fp32,comp16")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         return 2
     if args.hbm_ceiling:
         return hbm_ceiling()

@@ -41,7 +41,7 @@ typedef enum {
     PW_ERR_OUT_OF_MEMORY = 3, /* device allocation refused */
     PW_ERR_STATE = 4,         /* call out of order (e.g. run before set_medium) */
     PW_ERR_UNSTABLE = 5,      /* Courant condition violated by the medium */
-    PW_ERR_UNSUPPORTED = 6,   /* valid request this build does not implement */
+    PW_ERR_UNSUPPORTED = 6,   /* valid request the library does not implement */
     PW_ERR_IO = 7,            /* file could not be written */
     PW_ERR_TIMEOUT = 8        /* pw_server_wait: the shot did not finish in time */
 } pw_status;

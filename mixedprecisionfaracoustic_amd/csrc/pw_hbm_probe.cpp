@@ -1,7 +1,7 @@
 // pw_hbm_probe -- measured HBM bandwidth of plain streaming kernels on this GPU.
 //
 // The step kernels are HBM-bound, so the rate that a kernel of the simplest
-// possible shape reaches is the practical roof they are judged against, next
+// possible shape reaches is the practical roof they are measured against, next
 // to the datasheet figure.  Each pattern below streams separate 256 MiB arrays
 // with 16-byte loads and stores in a grid-stride loop:
 //   copy         1 read : 1 write

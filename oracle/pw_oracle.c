@@ -1,6 +1,6 @@
 /*
  * pw_oracle.c -- CPU restatement of the reference's Biot staggered-grid FD path
- * (Float/main.cu, Half/main.cu).  TEST INFRASTRUCTURE ONLY: the checker for the
+ * (Float/main.cu, Half/main.cu).  TEST INFRASTRUCTURE ONLY: the parity oracle of the
  * HIP product, loaded by tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg; never by the product.  Pinning status per function is listed
  * in pw_oracle.h.

@@ -2,7 +2,7 @@
  * pw_oracle.h -- CPU restatement of the reference's 3-D Biot poroelastic
  * staggered-grid finite-difference path (Float/main.cu and Half/main.cu).
  *
- * TEST INFRASTRUCTURE ONLY.  This code is the parity checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This code is the parity oracle: only tests/,
  * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
  * The product library never links, loads or calls it.
  *

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 N, NP, NT = 128, 20, 1000
 
 
-def config0():
+def homogeneous_128():
     grid = {"nx": N, "ny": N, "nz": N, "np": NP, "h": 5.0, "dt": 0.5e-3, "rc": 1e-4, "v0": 4000.0, "f0": 30.0,
             "src": (N // 2, N // 2, N // 2), "src_w": (1.0, 1.0, 1.0, -1.0)}
     b = pw.biot_from_rock(cases.sandstone())
@@ -28,13 +28,13 @@ def config0():
     return grid, base, recs
 
 
-def test_config0_fp32_matches_float_oracle():
+def test_homogeneous_128_fp32_matches_float_oracle():
     """Every channel within 1e-5 of the Float oracle, or within twice the Float
     oracle's own distance to FP64 where that is larger: vwx here is four orders
     of magnitude below vux, a difference of nearly equal terms that FP32 itself
     resolves only to ~1e-5.  The FP64 reference is the GPU FP64 run, which is
     bit-identical to the FP64 oracle at FP32 output (tests/test_gpu_parity.py)."""
-    grid, base, recs = config0()
+    grid, base, recs = homogeneous_128()
     gpu, _ = cases.run_gpu(grid, NT, "fp32", recs, base=base)
     g64, _ = cases.run_gpu(grid, NT, "fp64", recs, base=base)
     ref = cases.run_oracle("f32", grid, NT, recs, base=base)
@@ -42,7 +42,7 @@ def test_config0_fp32_matches_float_oracle():
         k = [i for i, (g, _) in enumerate(recs) if g == f]
         e = rel_l2(gpu["seis"][k], ref["seis"][k])
         floor = rel_l2(ref["seis"][k], g64["seis"][k])
-        print(f"config0 fp32 {f}: rel-L2 {e:.2e} (Float oracle vs FP64: {floor:.2e}), "
+        print(f"homogeneous 128^3 fp32 {f}: rel-L2 {e:.2e} (Float oracle vs FP64: {floor:.2e}), "
               f"max |ref| {np.abs(ref['seis'][k]).max():.3e}")
         assert np.abs(ref["seis"][k]).max() > 0
         assert e <= max(1e-5, 2.0 * floor), f
