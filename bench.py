@@ -81,14 +81,14 @@ def measured_traffic(precision, kernel, n):
     return None
 
 
-def cpu_baseline(n, np_, spec_base, dt, h, steps, bytes_per_cell_step):
+def cpu_baseline(n, np_, spec_base, dt, h, steps, bytes_per_cell_step, order=2):
     """The oracle (OpenMP C restatement of Float/main.cu) on the same grid and medium,
     slab-CPML mode, timed on a bounded number of steps; its effective GB/s is priced
     at the same bytes per cell-step as the GPU line."""
     import numpy as np
     from oracle import pw_oracle as O
     grid = {"nx": n, "ny": n, "nz": n, "np": np_, "h": h, "dt": dt, "rc": 1e-4, "v0": 4000.0, "f0": 25.0,
-            "src": (n // 2, n // 2, n // 2), "src_w": (1.0, 1.0, 1.0, -1.0)}
+            "src": (n // 2, n // 2, n // 2), "src_w": (1.0, 1.0, 1.0, -1.0), "order": order}
     base = spec_base.astype(np.float32)
     t0 = time.perf_counter()
     sec, thr = O.time_f32(grid, base, 1, steps)
@@ -97,7 +97,8 @@ def cpu_baseline(n, np_, spec_base, dt, h, steps, bytes_per_cell_step):
     rate = cells * steps / sec / 1e9
     return {"value": round(rate, 5), "unit": "Gcell-steps/s", "cores": thr, "kind": "port",
             "effective_gb_s": round(rate * bytes_per_cell_step, 2),
-            "sample": f"oracle FP32 (OpenMP C restatement of Float/main.cu FD_V/FD_T/Source, slab CPML) on "
+            "sample": f"oracle FP32 (OpenMP C restatement of Float/main.cu FD_V/FD_T/Source, slab CPML"
+                      f"{', 4th-order differences' if order == 4 else ''}) on "
                       f"the same {n}^3 medium: {steps} timed steps after 1 warm-up step, {sec:.2f} s timed "
                       f"({total:.1f} s with set-up)"}
 
@@ -356,13 +357,13 @@ def rank_share_main(args):
     return 0
 
 
-def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_steps):
+def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_steps, order=2):
     """The same workload in another storage precision, timed the same way: its
     throughput, step time, kernel times and dominant-kernel roofline."""
     nt = warmup + steps + prof_steps
     stream = torch.cuda.Stream()
     s = pw.Solver(n, n, n, np_, h, dt, nt, precision=precision, cpml_r=1e-4, cpml_v0=4000.0,
-                  stream=stream.cuda_stream)
+                  stream=stream.cuda_stream, space_order=order)
     s.set_medium_layered(spec)
     s.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
     fields, idx = receivers(n, np_)
@@ -498,6 +499,8 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="the 1024^3 eight-layer model (NP 32, dx 4 m, dt 0.4 ms, 20 Hz) instead of 512^3; "
                          "no CPU baseline (its FP64 base fields alone take 77 GB of host memory)")
+    ap.add_argument("--space-order", type=int, default=2, choices=[2, 4],
+                    help="staggered differences of 2nd (the reference's) or 4th order")
     ap.add_argument("--host-job", action="store_true",
                     help="time whole jobs from host buffers (set-up, medium upload, run, read-back) instead")
     ap.add_argument("--job-steps", type=int, default=2000)
@@ -532,7 +535,7 @@ def main():
     nt = args.warmup + args.steps + args.prof_steps
     stream = torch.cuda.Stream()
     solver = pw.Solver(n, n, n, np_, h, dt, nt, precision=args.precision, cpml_r=1e-4, cpml_v0=4000.0,
-                       stream=stream.cuda_stream)
+                       stream=stream.cuda_stream, space_order=args.space_order)
     solver.set_medium_layered(spec)
     solver.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
     fields, idx = receivers(n, np_)
@@ -601,12 +604,14 @@ def main():
         "vs_baseline": None,
         "dtype": dtype,
         "data": f"synthetic ({medium})",
-        "config": {"workload": f"{name}_{n}^3_{args.precision}", "grid": [n, n, n], "cpml": np_,
+        "config": {"workload": f"{name}_{n}^3_{args.precision}" + ("_order4" if args.space_order == 4 else ""),
+                   "grid": [n, n, n], "cpml": np_, "space_order": args.space_order,
                    "dx_m": h, "dt_s": dt, "f0_hz": f0, "receivers": len(idx), "precision": args.precision,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(args.precision, kname, n),
+                     # profiled for the 2nd-order step only
+                     "traffic": measured_traffic(args.precision, kname, n) if args.space_order == 2 else None,
                      "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 4)},
         "step_hbm_gbs": round(step_bytes * args.steps / sec / 1e9, 1),
         "bytes_per_cell_step": round(info["bytes_per_cell_step"], 2),
@@ -625,16 +630,19 @@ def main():
     solver.close()
     if args.precision == "fp32" and not args.no_comp16:
         # the metric's comparison: compensated FP16 (the reference Half format) on the same workload
-        c16 = side_run(pw, torch, "comp16", n, np_, h, dt, f0, spec, args.steps, args.warmup, args.prof_steps)
+        c16 = side_run(pw, torch, "comp16", n, np_, h, dt, f0, spec, args.steps, args.warmup, args.prof_steps,
+                       order=args.space_order)
         c16["ratio_vs_fp32"] = round(c16["value"] / gcell, 4)
         out["comp16"] = c16
     if not args.no_cpu_baseline and not args.large:
         base = None
-        s2 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
+        s2 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0,
+                       space_order=args.space_order)
         s2.set_medium_layered(spec)
         base = s2.base_fields()
         s2.close()
-        out["cpu_baseline"] = cpu_baseline(n, np_, base, dt, h, args.cpu_steps, info["bytes_per_cell_step"])
+        out["cpu_baseline"] = cpu_baseline(n, np_, base, dt, h, args.cpu_steps, info["bytes_per_cell_step"],
+                                           order=args.space_order)
         del base
     print(json.dumps(out), flush=True)
     return 0
