@@ -466,6 +466,34 @@ def host_job_main(args):
     return 0
 
 
+def medium_digest_main(args):
+    """sha256 of each benchmark medium's nine cell-centred FP64 base fields,
+    exactly as pw_get_base_fields returns them ([field][z][y][x], little-endian
+    doubles): the record that a generated medium is the one measured.  The
+    512^3 two-layer model (also the one the shot-serving trace crops from);
+    with --large also the 1024^3 eight-layer model (64 GB of host memory)."""
+    import hashlib
+    import mixedprecisionfaracoustic_amd as pw
+    todo = [("biot_two_layer_512", 512, 20, 5.0, 0.5e-3, workload(512, 20))]
+    if args.large:
+        todo.append(("biot_eight_layer_1024", 1024, 32, 4.0, 0.4e-3, large_workload(1024, 32)))
+    out = {"metric": "medium_digest", "layout": "9 x [nz-2np][ny-2np][nx-2np] float64 little-endian"}
+    rec_path = os.path.join(HERE, "tests", "golden", "medium_digests.json")
+    recorded = json.load(open(rec_path)) if os.path.exists(rec_path) else {}
+    for name, n, np_, h, dt, spec in todo:
+        s = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
+        s.set_medium_layered(spec)
+        base = s.base_fields()
+        s.close()
+        out[name] = {"sha256": hashlib.sha256(base).hexdigest(), "bytes": int(base.nbytes)}
+        if name in recorded:
+            out[name]["matches_recorded"] = out[name]["sha256"] == recorded[name]["sha256"]
+        print(f"{name}: {out[name]}", file=sys.stderr, flush=True)
+        del base
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -501,6 +529,8 @@ def main():
                          "no CPU baseline (its FP64 base fields alone take 77 GB of host memory)")
     ap.add_argument("--space-order", type=int, default=2, choices=[2, 4],
                     help="staggered differences of 2nd (the reference's) or 4th order")
+    ap.add_argument("--medium-digest", action="store_true",
+                    help="print the sha256 of the workload media (with --large, also the 1024^3 one) instead")
     ap.add_argument("--host-job", action="store_true",
                     help="time whole jobs from host buffers (set-up, medium upload, run, read-back) instead")
     ap.add_argument("--job-steps", type=int, default=2000)
@@ -515,6 +545,8 @@ def main():
         return rank_share_main(args)
     if args.host_job:
         return host_job_main(args)
+    if args.medium_digest:
+        return medium_digest_main(args)
     if args.serving:
         return serving_main(args)
     if args.precision_report:

@@ -13,7 +13,9 @@ Properties:
   relative L2 (FP32) and bit-exactly on nearly every sample;
 * a run split into chunks is bit-identical to one run (the step counter, the
   source time function and the receiver slots continue across calls);
-* seismograms are finite and carry signal.
+* seismograms are finite and carry signal;
+* the generated 512^3 medium is bit-identical to the recorded one (sha256 of its
+  FP64 base fields in tests/golden/medium_digests.json).
 """
 import os
 import sys
@@ -91,3 +93,18 @@ def test_1024_workload_runs_linear():
     doubled = s.seismogram()
     s.close()
     linearity(once, doubled, 1e-6)
+
+
+def test_512_medium_matches_recorded_digest():
+    """The benchmark's 512^3 two-layer medium, generated on the GPU from its
+    seed, hashes to the recorded sha256 (the 1024^3 one is recorded beside it;
+    its 64 GB of base fields are checked by bench.py --medium-digest --large)."""
+    import hashlib
+    import json
+    rec = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "medium_digests.json")))
+    s = pw.Solver(512, 512, 512, 20, 5.0, 0.5e-3, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
+    s.set_medium_layered(bench.workload(512, 20))
+    base = s.base_fields()
+    s.close()
+    assert base.nbytes == rec["biot_two_layer_512"]["bytes"]
+    assert hashlib.sha256(base).hexdigest() == rec["biot_two_layer_512"]["sha256"]
