@@ -91,15 +91,23 @@ def cpu_baseline(n, np_, spec_base, dt, h, steps, bytes_per_cell_step, order=2):
             "src": (n // 2, n // 2, n // 2), "src_w": (1.0, 1.0, 1.0, -1.0), "order": order}
     base = spec_base.astype(np.float32)
     t0 = time.perf_counter()
-    sec, thr = O.time_f32(grid, base, 1, steps)
+    sec, thr = O.time_f32(grid, base, 2, steps)
     total = time.perf_counter() - t0
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     cells = float(n) ** 3
     rate = cells * steps / sec / 1e9
-    return {"value": round(rate, 5), "unit": "Gcell-steps/s", "cores": thr, "kind": "port",
+    return {"value": round(rate, 5), "unit": "Gcell-steps/s", "cores": thr, "kind": "port", "cpu_model": model,
             "effective_gb_s": round(rate * bytes_per_cell_step, 2),
             "sample": f"oracle FP32 (OpenMP C restatement of Float/main.cu FD_V/FD_T/Source, slab CPML"
                       f"{', 4th-order differences' if order == 4 else ''}) on "
-                      f"the same {n}^3 medium: {steps} timed steps after 1 warm-up step, {sec:.2f} s timed "
+                      f"the same {n}^3 medium: {steps} timed steps after 2 warm-up steps, {sec:.2f} s timed "
                       f"({total:.1f} s with set-up)"}
 
 
@@ -505,7 +513,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-comp16", action="store_true",
                     help="skip the compensated-FP16 run reported beside the FP32 line")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--prof-steps", type=int, default=40)
     ap.add_argument("--serving", action="store_true", help="replay the shot-serving trace instead")
     ap.add_argument("--precision-report", action="store_true",
