@@ -365,10 +365,11 @@ def rank_share_main(args):
     return 0
 
 
-def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_steps, order=2):
+def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_steps, order=2, total_steps=0):
     """The same workload in another storage precision, timed the same way: its
-    throughput, step time, kernel times and dominant-kernel roofline."""
-    nt = warmup + steps + prof_steps
+    throughput, step time, kernel times and dominant-kernel roofline; then,
+    untimed, on to total_steps for the accuracy comparison."""
+    nt = max(warmup + steps + prof_steps, total_steps)
     stream = torch.cuda.Stream()
     s = pw.Solver(n, n, n, np_, h, dt, nt, precision=precision, cpml_r=1e-4, cpml_v0=4000.0,
                   stream=stream.cuda_stream, space_order=order)
@@ -376,6 +377,8 @@ def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_
     s.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
     fields, idx = receivers(n, np_)
     s.set_receivers(fields, idx)
+    snap_steps = [k for k in range(500, nt, 500)] or [nt - 1]
+    s.set_snapshots(snap_steps, ["txx"] * len(snap_steps), n // 2)
     info = s.info()
     s.run(warmup)
     s.synchronize()
@@ -391,6 +394,11 @@ def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_
     s.set_profiling(True)
     s.run(min(prof_steps, nt - s.step))
     ms_v, ms_t, launches = s.kernel_times()
+    s.set_profiling(False)
+    if s.step < nt:
+        s.run(nt - s.step)  # untimed
+        s.synchronize()
+    outputs = (s.seismogram(), s.snapshots(), s.step)
     s.close()
     cells = float(n) ** 3
     avg_v, avg_t = ms_v / launches, ms_t / launches
@@ -400,7 +408,43 @@ def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_
     return {"value": round(cells * steps / (ms_total / 1e3) / 1e9, 4), "ms_per_step": round(ms_total / steps, 4),
             "kernel_ms": {"k_velocity": round(avg_v, 4), "k_stress": round(avg_t, 4)},
             "roofline": {"kernel": kname, "achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4)},
-            "bytes_per_cell_step": round(info["bytes_per_cell_step"], 2)}
+            "bytes_per_cell_step": round(info["bytes_per_cell_step"], 2)}, outputs
+
+
+def fp64_outputs(pw, n, np_, h, dt, f0, spec, nt, order):
+    """The same run in FP64 (untimed): the reference for every line's accuracy."""
+    s = pw.Solver(n, n, n, np_, h, dt, nt, precision="fp64", cpml_r=1e-4, cpml_v0=4000.0, space_order=order)
+    s.set_medium_layered(spec)
+    s.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
+    fields, idx = receivers(n, np_)
+    s.set_receivers(fields, idx)
+    snap_steps = [k for k in range(500, nt, 500)] or [nt - 1]
+    s.set_snapshots(snap_steps, ["txx"] * len(snap_steps), n // 2)
+    s.run(nt)
+    out = (s.seismogram(), s.snapshots())
+    s.close()
+    return out
+
+
+def accuracy(outputs, ref, fields):
+    """Relative L2 of each seismogram channel and of the last txx snapshot
+    against the FP64 run, with the FP64 seismogram's peak (before the first
+    arrival at the receiver line it is rounding noise, and so is the error)."""
+    import numpy as np
+    seis, snaps = outputs
+    rseis, rsnaps = ref
+
+    def rel(a, b):
+        nb = float(np.linalg.norm(b.astype(np.float64)))
+        return float(np.linalg.norm(a.astype(np.float64) - b.astype(np.float64)) / nb) if nb > 0 else None
+
+    chans = {}
+    for ch in sorted(set(fields)):
+        k = [i for i, f in enumerate(fields) if f == ch]
+        e = rel(seis[k], rseis[k])
+        chans[ch] = None if e is None else float(f"{e:.3g}")
+    return {"seis_rel_l2": chans, "fp64_seis_peak": float(f"{float(np.abs(rseis).max()):.3g}"),
+            "snapshot_txx_rel_l2": float(f"{rel(snaps[-1], rsnaps[-1]):.3g}")}
 
 
 def hbm_ceiling():
@@ -572,7 +616,11 @@ def main():
         n, np_, h, dt, f0 = args.n, args.np, 5.0, 0.5e-3, 25.0
         spec = workload(n, np_)
         name, medium = "biot_two_layer", "two-layer perturbed Biot medium generated on the GPU, seed 0"
-    nt = args.warmup + args.steps + args.prof_steps
+    # the default line also checks FP32 and comp16 against an FP64 run; those
+    # runs go on, untimed, to 1000 steps: the first arrival reaches the
+    # receiver line (226 cells from the source) near step 780
+    acc_steps = 1000 if (args.precision == "fp32" and not args.no_comp16 and not args.large) else 0
+    nt = max(args.warmup + args.steps + args.prof_steps, acc_steps)
     stream = torch.cuda.Stream()
     solver = pw.Solver(n, n, n, np_, h, dt, nt, precision=args.precision, cpml_r=1e-4, cpml_v0=4000.0,
                        stream=stream.cuda_stream, space_order=args.space_order)
@@ -582,7 +630,8 @@ def main():
     solver.set_receivers(fields, idx)
     snap_steps = [s for s in range(500, nt, 500)] or [nt - 1]
     solver.set_snapshots(snap_steps, ["txx"] * len(snap_steps), n // 2)
-    vol_steps = list(range(args.volume_every - 1, nt, args.volume_every)) if args.volume_every > 0 else []
+    nt_run = args.warmup + args.steps + args.prof_steps
+    vol_steps = list(range(args.volume_every - 1, nt_run, args.volume_every)) if args.volume_every > 0 else []
     if vol_steps:
         # full 3-D txx volumes, copied to pinned host memory while the run goes on
         solver.set_volume_snapshots(vol_steps, ["txx"] * len(vol_steps))
@@ -613,6 +662,10 @@ def main():
     solver.run(nprof)
     ms_v, ms_t, launches = solver.kernel_times()
     solver.set_profiling(False)
+    if acc_steps and solver.step < nt:
+        solver.run(nt - solver.step)  # untimed
+        solver.synchronize()
+    main_outputs = (solver.seismogram(), solver.snapshots(), solver.step)
 
     finite = bool(np.isfinite(seis).all())
     cells = float(n) ** 3
@@ -670,10 +723,16 @@ def main():
     solver.close()
     if args.precision == "fp32" and not args.no_comp16:
         # the metric's comparison: compensated FP16 (the reference Half format) on the same workload
-        c16 = side_run(pw, torch, "comp16", n, np_, h, dt, f0, spec, args.steps, args.warmup, args.prof_steps,
-                       order=args.space_order)
+        c16, c16_outputs = side_run(pw, torch, "comp16", n, np_, h, dt, f0, spec, args.steps, args.warmup,
+                                    args.prof_steps, order=args.space_order, total_steps=acc_steps)
         c16["ratio_vs_fp32"] = round(c16["value"] / gcell, 4)
         out["comp16"] = c16
+        if acc_steps and main_outputs[2] == nt and c16_outputs[2] == nt:
+            # both lines' accuracy against the FP64 run of the same steps
+            ref = fp64_outputs(pw, n, np_, h, dt, f0, spec, nt, args.space_order)
+            out["accuracy_vs_fp64"] = accuracy(main_outputs[:2], ref, fields)
+            c16["accuracy_vs_fp64"] = accuracy(c16_outputs[:2], ref, fields)
+            del ref
     if not args.no_cpu_baseline and not args.large:
         base = None
         s2 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0,
