@@ -1,6 +1,6 @@
 """bench.py's host-side contract, without a GPU: the byte accounting behind
-`roofline` and `reference_scheme`, the serving trace generator, and the
-one-GPU guard."""
+`roofline` and `reference_scheme`, the serving trace generator, the accuracy
+figures against FP64, and the one-GPU guard."""
 import json
 import os
 import subprocess
@@ -57,3 +57,24 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     line = json.loads(r.stdout.strip().splitlines()[-1])
     assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+def test_accuracy_against_fp64_per_channel():
+    """Relative L2 per receiver channel and of the last snapshot; a channel
+    whose FP64 trace is all zero (no arrival yet) reports None."""
+    import numpy as np
+    rng = np.random.default_rng(1)
+    ref_seis = rng.standard_normal((6, 40)).astype(np.float32)
+    ref_seis[4:] = 0.0
+    ref_snap = rng.standard_normal((2, 5, 7)).astype(np.float32)
+    seis = ref_seis.copy()
+    seis[0:2] *= np.float32(1.5)   # txx: error 0.5
+    seis[2:4] += ref_seis[2:4]      # p: error 1
+    snap = ref_snap.copy()
+    snap[-1] *= np.float32(0.75)    # last snapshot: error 0.25
+    acc = bench.accuracy((seis, snap), (ref_seis, ref_snap), ["txx", "txx", "p", "p", "vwx", "vwx"])
+    assert acc["seis_rel_l2"]["txx"] == pytest.approx(0.5, rel=1e-6)
+    assert acc["seis_rel_l2"]["p"] == pytest.approx(1.0, rel=1e-6)
+    assert acc["seis_rel_l2"]["vwx"] is None
+    assert acc["snapshot_txx_rel_l2"] == pytest.approx(0.25, rel=1e-6)
+    assert acc["fp64_seis_peak"] == pytest.approx(float(np.abs(ref_seis).max()), rel=1e-3)
