@@ -6,6 +6,33 @@ import mixedprecisionfaracoustic_amd as pw
 from oracle import pw_oracle as O
 
 
+# each field's position along (x, y, z): 1 = staggered to the half-cell point
+STAGGER = {"p": (0, 0, 0), "txx": (0, 0, 0), "tyy": (0, 0, 0), "tzz": (0, 0, 0),
+           "vux": (1, 0, 0), "vwx": (1, 0, 0), "vuy": (0, 1, 0), "vwy": (0, 1, 0), "vuz": (0, 0, 1), "vwz": (0, 0, 1),
+           "txy": (1, 1, 0), "txz": (1, 0, 1), "tyz": (0, 1, 1)}
+
+
+def mirror_deviation(a, name):
+    """For a run whose medium and explosive source are symmetric about the
+    middle plane of every axis (odd extents, source in the middle cell): a field
+    at whole cells along an axis is even there, F[i] = F[n-1-i], and one
+    staggered to i + 1/2 is odd, F[i] = -F[n-2-i].  Returns the largest
+    deviation over the three axes relative to the field's peak, the smallest
+    fraction of bit-exact cells, and the peak.  a is [z][y][x]."""
+    a = np.asarray(a, np.float64)
+    peak = float(np.abs(a).max())
+    worst, exact = 0.0, 1.0
+    for ax, stag in zip((2, 1, 0), STAGGER[name]):  # x, y, z
+        if stag:
+            b = np.take(a, range(a.shape[ax] - 1), axis=ax)
+            m = -np.flip(b, axis=ax)
+        else:
+            b, m = a, np.flip(a, axis=ax)
+        worst = max(worst, float(np.abs(b - m).max()) / peak if peak > 0 else 0.0)
+        exact = min(exact, float(np.mean(b == m)))
+    return worst, exact, peak
+
+
 def rel_l2(got, ref):
     got = np.asarray(got, np.float64)
     ref = np.asarray(ref, np.float64)

@@ -10,9 +10,10 @@
 * Every tiling (1, 2 or 4 cells per lane, 1 or 2 waves per row, 2- to 8-row
   tiles) bit-identical to the default on grids whose x extent ends inside the
   last 64-, 128- and 256-cell tile, with the wave crossing those tile edges.
-* x-mirror symmetry: homogeneous medium, odd x extent, explosive source
-  on the middle plane; the tile edges (x = 64, 128) are not symmetric about it,
-  so a wrong neighbour at a wave's edge lane breaks the symmetry.
+* Mirror symmetry about the middle plane of every axis: homogeneous medium,
+  odd extents, explosive source in the middle cell; the x tile edges, y block
+  edges and z chunk edges are not symmetric about it, so a wrong neighbour or
+  carry at any of them breaks the symmetry.
 * A source in the CPML corner cell (1, 1, 1).
 * Receivers on the edge planes, which are never updated: their traces stay 0.
 * No receivers; a zero-step run; running past nt is an error that leaves the
@@ -118,42 +119,33 @@ def test_ragged_tilings_are_bit_identical(shape, np_, src, monkeypatch):
     assert not bad, f"tilings differing from the default: {bad}"
 
 
-# fields even in x about the source plane, and those staggered to x + 1/2 (odd)
-EVEN_X = ("vuz", "vwz", "tzz", "p", "txx")
-ODD_X = ("vux", "vwx", "txz")
-
-
 @pytest.mark.parametrize("prec,order", [("fp32", 2), ("fp32", 4), ("fp64", 4), ("comp16", 2), ("comp16", 4),
                                         ("bf16", 2)])
-def test_x_mirror_symmetry(prec, order):
-    """Homogeneous sandstone, nx = 131, source at x = 65: every even field
-    satisfies F[x] = F[130 - x] and every field staggered to x + 1/2 satisfies
-    F[i] = -F[129 - i].  Negation and round-to-nearest are symmetric, so the same
-    update at mirrored cells gives mirrored bits; the x tile edges at 64 and 128
-    (lanes 63 / 0 of a wave) sit at different places relative to the source than
-    their mirror images, and a wrong neighbour there is an O(1) error.  The
-    bound is 1e-6 of the field's peak rather than 0: the update spreads values
-    one to two cells per step, so values near 1e-35 of the peak already reach
-    the x CPML, whose profile need not be mirror-exact."""
-    nx, ny, nz, np_, sx = 131, 22, 26, 4, 65
+def test_mirror_symmetry(prec, order):
+    """Homogeneous sandstone on a 131 x 99 x 99 grid, explosive source in the
+    middle cell (65, 49, 49): every field is even or odd about the middle plane
+    of each axis (cases.mirror_deviation).  Negation and round-to-nearest are
+    symmetric, so the same update at mirrored cells gives mirrored bits.  The
+    tile edges along x (64, 128: lanes 63 / 0 of a wave), the 4-row block edges
+    along y and the 4-plane chunk edges along z sit at different places relative
+    to the source than their mirror images, so a wrong neighbour or carry at any
+    of them is an O(1) error.  In 120 steps the P wave moves about 40 cells, short
+    of the CPML 45 or more cells out, whose profiles (the reference's) are not
+    mirror-exact.  The bound is 1e-6 of the field's peak rather than 0: the
+    update spreads values one to two cells per step, so values near 1e-25 of
+    the peak do reach the CPML."""
+    nx, ny, nz, np_ = 131, 99, 99, 4
     grid = {"nx": nx, "ny": ny, "nz": nz, "np": np_, "h": 5.0, "dt": 0.5e-3, "rc": 1e-4, "v0": 4000.0,
-            "f0": 30.0, "src": (sx, 10, 13), "src_w": (1.0, 1.0, 1.0, -1.0), "order": order}
+            "f0": 30.0, "src": (nx // 2, ny // 2, nz // 2), "src_w": (1.0, 1.0, 1.0, -1.0), "order": order}
     b = pw.biot_from_rock(cases.sandstone())
     ni = (nz - 2 * np_, ny - 2 * np_, nx - 2 * np_)
     base = np.broadcast_to(b[:, None, None, None], (9,) + ni).copy()
-    # 120 steps: the P wave (0.33 cells per step) has moved about 25 cells past
-    # the source wavelet's onset, far from the x CPML 61 cells away
-    g, _ = cases.run_gpu(grid, 120, prec, [], base=base, fields=EVEN_X + ODD_X)
-    for f in EVEN_X + ODD_X:
-        a = g["fields"][f].astype(np.float64)
-        m = a[:, :, ::-1] if f in EVEN_X else -a[:, : , : nx - 1][:, :, ::-1]
-        if f in ODD_X:
-            a = a[:, :, : nx - 1]
-        peak = np.abs(a).max()
+    names = tuple(cases.STAGGER)
+    g, _ = cases.run_gpu(grid, 120, prec, [], base=base, fields=names)
+    for f in names:
+        dev, exact, peak = cases.mirror_deviation(g["fields"][f], f)
+        print(f"{prec} order {order} {f}: max mirror deviation {dev:.1e} of the peak, {exact:.4f} of cells bit-exact")
         assert peak > 0, f
-        dev = np.abs(a - m).max() / peak
-        print(f"{prec} order {order} {f}: max mirror deviation {dev:.1e} of the peak, "
-              f"{np.mean(a == m):.4f} of cells bit-exact")
         assert dev <= 1e-6, (prec, order, f, dev)
 
 
