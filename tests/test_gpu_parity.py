@@ -203,6 +203,19 @@ def test_layered_generator_is_deterministic(layered):
     assert 0.01 < np.std(top / 9e9 - 1) < 0.08
 
 
+def test_host_medium_equals_generated_medium(layered):
+    """pw_set_medium with the base fields in host memory (what a caller holding
+    its own model does) gives the same run, bit for bit, as generating the same
+    medium on the device with pw_set_medium_layered."""
+    grid, spec, nt, recs, gpu32, base = layered
+    for prec in ("fp32", "comp16"):
+        g_dev = gpu32 if prec == "fp32" else cases.run_gpu(grid, nt, prec, recs, spec=spec, fields=FIELDS)[0]
+        g_host, _ = cases.run_gpu(grid, nt, prec, recs, base=base, fields=FIELDS)
+        assert np.array_equal(g_dev["seis"], g_host["seis"]), prec
+        for f in FIELDS:
+            assert np.array_equal(g_dev["fields"][f], g_host["fields"][f]), (prec, f)
+
+
 def test_snapshots_match_fields(borehole):
     grid, base, nt, recs, _ = borehole
     snaps = [(nt - 1, "txx"), (nt - 1, "vuz"), (50, "p")]
