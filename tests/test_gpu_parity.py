@@ -216,6 +216,45 @@ def test_host_medium_equals_generated_medium(layered):
             assert np.array_equal(g_dev["fields"][f], g_host["fields"][f]), (prec, f)
 
 
+def test_two_handles_run_concurrently(layered):
+    """Two handles driven at once from two host threads, each on the stream
+    the library created for it, give the same bits as each run alone (one
+    handle is not thread-safe; distinct handles may run concurrently)."""
+    import threading
+    grid, spec, nt, recs, gpu32, base = layered
+    solo16, _ = cases.run_gpu(grid, nt, "comp16", recs, spec=spec, fields=FIELDS)
+    handles = {}
+    for prec in ("fp32", "comp16"):
+        h = cases.make_solver(grid, nt, prec, spec=spec)
+        h.set_receivers([f for f, _ in recs], [i for _, i in recs])
+        handles[prec] = h
+    errs = []
+
+    def drive(h):
+        try:
+            done = 0
+            while done < nt:
+                k = min(7, nt - done)
+                h.run(k)
+                done += k
+            h.synchronize()
+        except Exception as e:  # reported below, on the test's thread
+            errs.append(e)
+
+    threads = [threading.Thread(target=drive, args=(h,)) for h in handles.values()]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errs, errs
+    for prec, solo in (("fp32", gpu32), ("comp16", solo16)):
+        h = handles[prec]
+        assert np.array_equal(h.seismogram(), solo["seis"]), prec
+        for f in FIELDS:
+            assert np.array_equal(h.field(f), solo["fields"][f]), (prec, f)
+        h.close()
+
+
 def test_snapshots_match_fields(borehole):
     grid, base, nt, recs, _ = borehole
     snaps = [(nt - 1, "txx"), (nt - 1, "vuz"), (50, "p")]
