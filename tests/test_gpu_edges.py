@@ -159,6 +159,36 @@ def test_source_in_cpml_corner_matches_oracle():
     check(g32, ref32, recs, 1e-5, "fp32 corner source", floor=floor)
 
 
+def test_api_misuse_is_refused_and_the_handle_stays_usable():
+    """Calls out of order or with bad arguments fail with their status code and
+    a message, before touching the device state; the handle then runs normally."""
+    grid, base, nt = hetero_case((20, 18, 22), 3, nt=12)
+    s = pw.Solver(20, 18, 22, 3, 5.0, 0.5e-3, nt, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
+    with pytest.raises(pw.PorowaveError) as ei:
+        s.run(1)                                         # before any medium: PW_ERR_STATE
+    assert ei.value.status == 4 and "no medium" in str(ei.value)
+    s.set_medium(base)
+    for call, status in [
+        (lambda: s.base_fields(), 4),                    # only generated media are kept
+        (lambda: s.set_source(0, 9, 11, 30.0), 1),       # x = 0 is never updated
+        (lambda: s.set_source(10, 9, 11, -1.0), 1),      # f0 must be positive
+        (lambda: s.set_receivers(["txx"], [20 * 18 * 22]), 1),  # one past the last cell
+        (lambda: s.field(99), 1),                        # no such field
+    ]:
+        with pytest.raises(pw.PorowaveError) as ei:
+            call()
+        assert ei.value.status == status and str(ei.value), (status, str(ei.value))
+    with pytest.raises(ValueError):
+        s.set_medium(base[:, :-1])                       # wrong interior shape (checked in Python)
+    assert s.step == 0
+    s.set_source(10, 9, 11, 30.0, (1.0, 1.0, 1.0, -1.0))
+    s.set_receivers(["txx", "p"], [10 + 9 * 20 + 11 * 20 * 18, 11 + 9 * 20 + 11 * 20 * 18])
+    s.run(nt)
+    seis = s.seismogram()
+    assert seis.shape == (2, nt) and np.isfinite(seis).all() and np.abs(seis).max() > 0
+    s.close()
+
+
 def test_no_receivers_zero_steps_and_overrun():
     grid, base, nt = hetero_case((40, 36, 44), 6, nt=50)
     s = cases.make_solver(grid, nt, "fp32", base=base)
