@@ -2,7 +2,8 @@
 reference shot (192^3 borehole, 3 kHz) run by the program that replaces
 Float/main.cu:main(), its text files read back and compared with the snapshot
 files the reference committed (tests/golden/float_snapshots.npz) and with the
-same run through the Python binding.  A comp16 run writes binary files."""
+same run through the Python binding.  A comp16 run and a 4th-order run write
+binary files that equal the same runs through the binding."""
 import os
 import subprocess
 
@@ -68,3 +69,19 @@ def test_cli_comp16_binary_outputs(tmp_path):
         assert np.array_equal(data, seis[c]), c
     h, data = pw.read_bin(str(tmp_path / "txx.bin"))
     assert h["step"] == nt - 1 and h["kind"] == 1 and np.isfinite(data).all() and np.abs(data).max() > 0
+
+
+def test_cli_fourth_order_matches_the_binding(tmp_path):
+    """--order 4: the program's seismogram files equal the same 4th-order run
+    through the Python binding, bit for bit."""
+    nt = 150
+    run_cli(["--order", "4", "--nt", str(nt), "--format", "bin"], tmp_path)
+    s = pw.reference_solver("fp32", nt=nt, space_order=4)
+    s.run(nt)
+    seis = s.reference_seismograms()
+    s.close()
+    for c in SIS:
+        h, data = pw.read_bin(str(tmp_path / f"{c}.bin"))
+        assert (h["nr"], h["nc"]) == (192, nt), c
+        assert np.array_equal(data, seis[c]), c
+    assert np.abs(seis["sisp"]).max() > 0
