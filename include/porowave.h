@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PW_ABI_VERSION 1
+#define PW_ABI_VERSION 2
 
 typedef struct pw_solver pw_solver;
 
@@ -53,9 +53,11 @@ typedef enum {
     PW_COMP16 = 1, /* Half/main.cu: FP16 value + int8 residual, FP16 coefficients */
     PW_BF16 = 2,   /* BF16 wavefields + memory variables, FP16 coefficients */
     PW_FP64 = 3,   /* FP64 oracle run */
-    PW_COMP16F = 4 /* the comp16 state (FP16 value + int8 residual; stresses scaled by 2^es,
-                      velocities by 2^(es-em) as Half/main.cu:793-807 chooses them) with the
-                      Float update: FP32 cell-centred coefficients, FP32 memory variables */
+    PW_COMP16F = 4 /* the comp16 state (FP16 value + int8 residual, residual rounded to
+                      nearest; stresses stored times 2^(es+6), velocities 2^(es+6-em) with
+                      em of Half/main.cu:793-807) under an FP32 update of the Float
+                      program's equations: FP32 cell-centred coefficients, FP32 memory
+                      variables, hardware reciprocals */
 } pw_precision;
 
 /* Wavefield ids for receivers, snapshots and pw_get_field. */
@@ -91,7 +93,8 @@ typedef struct pw_config {
     int precision;          /* pw_precision */
     double cpml_r;          /* CPML reflection coefficient (RC; reference 1e-6) */
     double cpml_v0;         /* CPML velocity v0 (reference 4000 m/s) */
-    int half_es;            /* comp16/bf16: WAVEFIELD_ES_INT (reference 0) */
+    int half_es;            /* comp16/bf16: WAVEFIELD_ES_INT (reference 0); comp16f
+                               stores stresses times 2^(half_es + 6) */
     double half_dyn_end;    /* comp16/bf16: DYNAMIC_SCALE_END (reference 16) */
     int half_dyn_interval;  /* comp16/bf16: DYNAMIC_SCALE_INTERVAL (reference 100) */
     int device;             /* HIP device ordinal */
@@ -110,6 +113,13 @@ typedef struct pw_config {
                                outermost plane is held like every edge plane: the
                                ghost plane of one rank's z-slab in a decomposed
                                grid.  No counterpart in the reference. */
+    /* Kernel tiling of the two step kernels (no counterpart in the reference,
+       whose kernels take one thread per cell).  0 selects the measured default
+       of the precision; every setting gives bit-identical results. */
+    int tile_cells;         /* cells of an x row per lane: 1, 2 or 4 (order 4: 1) */
+    int tile_waves_x;       /* waves side by side along x in a tile: 1 or 2 */
+    int tile_rows;          /* waves per block: 1, 2, 4 or 8 */
+    int tile_planes;        /* z planes each block marches through, >= 1 */
 } pw_config;
 
 /* pw_config.cpml_open bits */
@@ -236,10 +246,8 @@ typedef struct pw_info {
     int precision;
     int em, epml;               /* comp16/bf16 exponents (Half/main.cu:793-843) */
     double bytes_per_cell_step; /* algorithmic HBM bytes per cell-step, this grid */
-    double bytes_velocity;      /* ... of which the velocity kernel (per cell); the
-                                   fused step kernel when fused != 0 */
-    double bytes_stress;        /* ... of which the stress kernel (0 when fused) */
-    int fused;                  /* 1: one fused launch per step (opt-in, PW_FUSED=1) */
+    double bytes_velocity;      /* ... of which the velocity kernel (per cell) */
+    double bytes_stress;        /* ... of which the stress kernel (per cell) */
     double vp_max;              /* fastest P velocity of the medium [m/s] */
     double courant;             /* vp_max * dt / h * sqrt(3) */
     long long device_bytes;     /* device memory held by the handle */
@@ -398,7 +406,8 @@ pw_status pw_server_wait(pw_server* srv, long long id, double timeout_s);
 pw_status pw_server_status(pw_server* srv, long long id, pw_shot_status* out);
 /* Copies a finished shot's seismogram, [nrec][nt] floats in physical units. */
 pw_status pw_server_result(pw_server* srv, long long id, float* seis);
-/* Forgets a finished or failed shot and its host-side result. */
+/* Forgets a finished or failed shot and its host-side result (PW_ERR_STATE while
+ * the shot is queued or running, or while another thread waits on it). */
 pw_status pw_server_release(pw_server* srv, long long id);
 pw_status pw_server_get_stats(pw_server* srv, pw_server_stats* out);
 /* Seconds since pw_server_create (the clock of pw_shot_status). */

@@ -40,13 +40,13 @@ class Config(C.Structure):
                 ("cpml_r", C.c_double), ("cpml_v0", C.c_double),
                 ("half_es", C.c_int), ("half_dyn_end", C.c_double), ("half_dyn_interval", C.c_int),
                 ("device", C.c_int), ("stream", C.c_void_p), ("check_cfl", C.c_int), ("space_order", C.c_int),
-                ("cpml_open", C.c_int)]
+                ("cpml_open", C.c_int), ("tile_cells", C.c_int), ("tile_waves_x", C.c_int), ("tile_rows", C.c_int),
+                ("tile_planes", C.c_int)]
 
 
 class Info(C.Structure):
     _fields_ = [("precision", C.c_int), ("em", C.c_int), ("epml", C.c_int),
                 ("bytes_per_cell_step", C.c_double), ("bytes_velocity", C.c_double), ("bytes_stress", C.c_double),
-                ("fused", C.c_int),
                 ("vp_max", C.c_double), ("courant", C.c_double),
                 ("device_bytes", C.c_longlong), ("slab_lo", C.c_int * 3), ("slab_hi", C.c_int * 3)]
 
@@ -266,11 +266,15 @@ class Solver:
     cpml_open: faces without an absorbing layer, any of "x_lo", "x_hi", "y_lo",
     "y_hi", "z_lo", "z_hi" (or the PW_FACE_* bit mask); e.g. ("z_lo", "z_hi")
     is one interior rank's z-slab, whose outermost planes are its ghost planes.
+    tiling: optional dict of the step kernels' tile shape, keys "cells" (per
+    lane along x: 1, 2, 4), "waves_x" (1, 2), "rows" (waves per block: 1, 2, 4,
+    8), "planes" (z planes per block); omitted keys take the precision's
+    default.  Every tiling gives bit-identical results.
     """
 
     def __init__(self, nx, ny, nz, np_, h, dt, nt, precision="fp32", cpml_r=1e-6, cpml_v0=4000.0,
                  device=0, check_cfl=True, half_es=0, half_dyn_end=16.0, half_dyn_interval=100, stream=None,
-                 space_order=2, cpml_open=0):
+                 space_order=2, cpml_open=0, tiling=None):
         L = lib()
         cfg = Config()
         L.pw_config_default(C.byref(cfg))
@@ -284,6 +288,10 @@ class Solver:
         cfg.check_cfl = 1 if check_cfl else 0
         cfg.space_order = int(space_order)
         cfg.cpml_open = face_mask(cpml_open)
+        for k, v in (tiling or {}).items():
+            if k not in ("cells", "waves_x", "rows", "planes"):
+                raise ValueError(f"unknown tiling key {k!r}")
+            setattr(cfg, "tile_" + k, int(v))
         h_ = C.c_void_p()
         _check(L.pw_create(C.byref(cfg), C.byref(h_)))
         self._h = h_
@@ -431,7 +439,7 @@ class Solver:
         _check(lib().pw_get_info(self._h, C.byref(i)))
         return {"precision": self.precision, "em": i.em, "epml": i.epml,
                 "bytes_per_cell_step": i.bytes_per_cell_step, "bytes_velocity": i.bytes_velocity,
-                "bytes_stress": i.bytes_stress, "fused": i.fused, "vp_max": i.vp_max, "courant": i.courant,
+                "bytes_stress": i.bytes_stress, "vp_max": i.vp_max, "courant": i.courant,
                 "device_bytes": i.device_bytes, "slab_lo": list(i.slab_lo), "slab_hi": list(i.slab_hi)}
 
 

@@ -37,10 +37,21 @@ constexpr int kMaxCoef = 22;
 inline bool is_scaled(int prec) { return prec == PW_COMP16 || prec == PW_BF16; }
 // the compensated FP16 state: FP16 "hi" arrays plus int8 "lo" residual arrays
 inline bool has_lo(int prec) { return prec == PW_COMP16 || prec == PW_COMP16F; }
+// comp16f fields kept in FP32 (bit k = field k; a build-time mask, 0 by default)
+#ifndef PW_C16F_PLAIN
+#define PW_C16F_PLAIN 0
+#endif
+__host__ __device__ constexpr bool c16f_plain(int k) { return (PW_C16F_PLAIN >> k) & 1; }
+inline bool field_has_lo(int prec, int k) { return has_lo(prec) && !(prec == PW_COMP16F && c16f_plain(k)); }
 inline int ncoef(int prec) { return is_scaled(prec) ? (int)SC_N : (int)PC_N; }
-// Bytes of one stored wavefield value / coefficient / memory variable.
+// comp16f forms H from H - 2mu and mu and holds no H array
+inline bool coef_used(int prec, int k) { return k < ncoef(prec) && !(prec == PW_COMP16F && k == PC_HH); }
+// Bytes of one stored wavefield value of field k / coefficient / memory variable.
 inline int state_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : has_lo(prec) ? 3 : 2; }
+inline int field_bytes(int prec, int k) { return has_lo(prec) && !field_has_lo(prec, k) ? 4 : state_bytes(prec); }
 inline int coef_bytes(int prec) { return prec == PW_FP64 ? 8 : (prec == PW_FP32 || prec == PW_COMP16F) ? 4 : 2; }
+// memory-variable slots in use per axis (the rest are null)
+inline int mem_slots(int prec) { return prec == PW_COMP16 ? 16 : 8; }
 inline int mem_bytes(int prec) { return prec == PW_FP64 ? 8 : (prec == PW_FP32 || prec == PW_COMP16F) ? 4 : 2; }
 
 // Device pointers of one solver.  For comp16, f[] holds the FP16 "hi" arrays
@@ -63,19 +74,11 @@ struct DevBuffers {
     int order;       // spatial order of the staggered differences: 2 or 4
     float sc_stress, sc_vel, sc_vw;  // comp16f: stored value = physical * scale (powers of two)
     int xcd_map;     // 1: XCD-aware block -> tile mapping in the step kernels
-    int slab;        // >0: interleave velocity/stress launches over z-slabs of this many planes
-    int fused;       // 1: one fused launch per step over double-buffered state
-    // write side of the fused step (time n+1); swapped with f/lo/mem after each step
-    void* fo[PW_NFIELDS];
-    int8_t* loo[PW_NFIELDS];
-    void* memo[3][kMemPerAxis];
 };
 
 // Step kernels over the planes [zb, ze) (the updated planes are 1 .. nz-2).
 hipError_t launch_velocity(const DevBuffers& b, hipStream_t st, int zb, int ze);
 hipError_t launch_stress(const DevBuffers& b, hipStream_t st, int zb, int ze);
-// One fused step (velocity then stress) reading f/lo/mem and writing fo/loo/memo.
-hipError_t launch_fused(const DevBuffers& b, hipStream_t st);
 hipError_t launch_source(const DevBuffers& b, hipStream_t st, unsigned off, const double w[4], double amp);
 hipError_t launch_gather(const DevBuffers& b, hipStream_t st, const int* rfield, const unsigned* roff, int nrec,
                          float* seis, int nt, int it, float s_stress, float s_vel);

@@ -24,9 +24,11 @@ pw_status fail(pw_status st, const std::string& msg);
                               std::string(#expr) + ": " + hipGetErrorString(e_));          \
     } while (0)
 
-// Number of CPML memory variables per damped (cell, axis): 4 in the velocity
-// kernel and 4 in the stress kernel.
-constexpr int kMemPerAxis = 8;
+// CPML memory-variable slots per damped (cell, axis): 4 terms in the velocity
+// kernel and 4 in the stress kernel (slots 0-7), one variable each; comp16
+// keeps the reference Half program's pair per term (Half/main.cu:201-240): psi
+// in slots 0-7 and the previous derivative in slots 8-15.
+constexpr int kMemPerAxis = 16;
 
 // Grid geometry shared by every kernel.  Arrays are padded SoA: row pitch is
 // nx rounded up to 64 elements, plane = pitch * ny.

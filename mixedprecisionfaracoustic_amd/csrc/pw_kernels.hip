@@ -20,9 +20,8 @@
 // the compiler cannot prove disjoint from them.  Each field puts one pointer
 // into the kernels' scalar registers (FA below).
 //
-// Also here: the fused one-launch step (opt-in), the batched kernels of the
-// shot server (one launch per kernel over many shots), the rescale, snapshot,
-// decode and fill side kernels.
+// Also here: the batched kernels of the shot server (one launch per kernel over
+// many shots), the rescale, snapshot, decode and fill side kernels.
 //
 // CPML: the memory variables exist only on the boundary slabs of each axis (in
 // the interior the reference's psi is identically zero).  Each term keeps ONE
@@ -63,7 +62,7 @@ __device__ __forceinline__ float c16_unit(float hf)
     return __uint_as_float((E - 18u) << 23);
 }
 // The same unit from the FP16 bit pattern: biased FP16 exponent e5 (0 for
-// zero/subnormal) gives 2^(max(e5, 1) - 32), i.e. float bits (max(e5,1)+95) << 23
+// zero/subnormal) gives 2^(max(e5, 1) - 33), i.e. float bits (max(e5,1)+94) << 23
 // with the 127 bias: e5 = 1 -> 2^-32, the subnormal value.
 __device__ __forceinline__ float c16_unit_bits(unsigned short hb)
 {
@@ -92,6 +91,24 @@ __device__ __forceinline__ void c16_encode(float v_in, f16& hi, int8_t& lo, unsi
     lo = (int8_t)q;
 }
 
+// comp16f's encode: the same format, with the residual rounded to nearest
+// (ties to even) instead of half away from zero.  The inverse unit comes from
+// the FP32 exponent field E of the FP16 value, 2^(145 - max(E, 113)); the
+// clamped residual plus 1.5 * 2^23 is an FP32 whose low byte holds the rounded
+// residual in two's complement.
+__device__ __forceinline__ void c16_encode_rn(float v_in, f16& hi, int8_t& lo, unsigned z)
+{
+    const float v = round32(v_in, z);
+    const f16 h = (f16)v;
+    const float hf = (float)h;
+    const unsigned e = __float_as_uint(hf) & 0x7f800000u;
+    const unsigned eb = e > (113u << 23) ? e : (113u << 23);
+    const float iu = __uint_as_float((272u << 23) - eb);
+    const float q = __builtin_amdgcn_fmed3f((v - hf) * iu, -127.0f, 127.0f);
+    hi = h;
+    lo = (int8_t)(__float_as_uint(q + 12582912.0f) & 0xffu);
+}
+
 // --------------------------------------------------------------------------
 // Storage policies
 //   ld / st        one value
@@ -106,6 +123,17 @@ template <class F> struct requires_zero<F, std::void_t<decltype(F::z)>> : std::t
 
 // N consecutive elements, aligned to their total size: one global load/store
 template <class T, int N> struct alignas(sizeof(T) * N) Pack { T v[N]; };
+
+// Update arithmetic of a policy:
+//  kFormRef   the reference Float program's (Float/main.cu:64-295): staggered
+//             coefficients formed on the fly with its float/double steps,
+//             derivatives divided by h
+//  kFormHalf  the reference Half program's (Half/main.cu:156-371): FP16
+//             coefficients prescaled by dt and 2^em
+//  kFormFast  this build's FP32 form of the same update: undivided differences
+//             with dt/h and the storage scales folded into launch constants,
+//             hardware reciprocals instead of IEEE divides
+enum { kFormRef = 0, kFormHalf = 1, kFormFast = 2 };
 template <int N, class T> __device__ __forceinline__ Pack<T, N> ld_pack(const T* p, unsigned o)
 {
     return *(const Pack<T, N>*)(p + o);
@@ -146,7 +174,9 @@ template <> __device__ __forceinline__ bf16 narrow<bf16, float>(float v, unsigne
 
 struct PolF32 {
     using R = float; using Coef = float; using Mem = float;
+    static constexpr int form = kFormRef;
     static constexpr bool scaled = false;
+    static constexpr bool pair_cpml = false;
     struct Field { float* p; };
     static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return f.p[o]; }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v) { f.p[o] = v; }
@@ -154,11 +184,12 @@ struct PolF32 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned) { m[o] = v; }
-    static __device__ __forceinline__ R rt(const Field&, R v) { return v; }  // value as stored
 };
 struct PolF64 {
     using R = double; using Coef = double; using Mem = double;
+    static constexpr int form = kFormRef;
     static constexpr bool scaled = false;
+    static constexpr bool pair_cpml = false;
     struct Field { double* p; };
     static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return f.p[o]; }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v) { f.p[o] = v; }
@@ -166,11 +197,13 @@ struct PolF64 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned) { m[o] = v; }
-    static __device__ __forceinline__ R rt(const Field&, R v) { return v; }
 };
 struct PolC16 {
     using R = float; using Coef = f16; using Mem = f16;
+    static constexpr int form = kFormHalf;
     static constexpr bool scaled = true;
+    // the reference Half program's CPML: (psi, previous derivative) FP16 pairs
+    static constexpr bool pair_cpml = true;
     struct Field { f16* hi; int8_t* lo; unsigned z; };
     static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return c16_decode(f.hi[o], f.lo[o]); }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v)
@@ -206,45 +239,36 @@ struct PolC16 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return (float)c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return (float)m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned z) { m[o] = (f16)round32(v, z); }
-    static __device__ __forceinline__ R rt(const Field& f, R v)
-    {
-        f16 h;
-        int8_t l;
-        c16_encode(v, h, l, f.z);
-        return c16_decode(h, l);
-    }
 };
-// comp16 storage under the physical (Float) update: values are stored times a
-// power-of-two scale per field class (exact), coefficients and memory variables
-// are FP32.
+// comp16f: the comp16 state (FP16 value + int8 residual, residual rounded to
+// nearest) under this build's FP32 update (kFormFast).  Stored values are
+// physical values times a power-of-two scale per field class; coefficients are
+// the cell-centred FP32 medium and memory variables FP32.  With
+// a bit of PW_C16F_PLAIN set, field k is stored as FP32 (a field whose lo is
+// null).  ld / st take and return physical values (side kernels); the
+// step kernels work in stored units through FA<PolC16F>.
 struct PolC16F {
     using R = float; using Coef = float; using Mem = float;
+    static constexpr int form = kFormFast;
     static constexpr bool scaled = false;
+    static constexpr bool pair_cpml = false;
     struct Field { f16* hi; int8_t* lo; unsigned z; float sc, isc; };
-    static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return c16_decode(f.hi[o], f.lo[o]) * f.isc; }
+    static __device__ __forceinline__ R ld(const Field& f, unsigned o)
+    {
+        if (!f.lo) return ((const float*)f.hi)[o] * f.isc;
+        return c16_decode(f.hi[o], f.lo[o]) * f.isc;
+    }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v)
     {
+        if (!f.lo) {
+            ((float*)f.hi)[o] = v * f.sc;
+            return;
+        }
         f16 h;
         int8_t l;
-        c16_encode(v * f.sc, h, l, f.z);
+        c16_encode_rn(v * f.sc, h, l, f.z);
         f.hi[o] = h;
         f.lo[o] = l;
-    }
-    template <int N> static __device__ __forceinline__ void ldv(const Field& f, unsigned o, R (&r)[N])
-    {
-        const Pack<f16, N> h = ld_pack<N>(f.hi, o);
-        const Pack<int8_t, N> l = ld_pack<N>(f.lo, o);
-#pragma unroll
-        for (int j = 0; j < N; ++j) r[j] = c16_decode(h.v[j], l.v[j]) * f.isc;
-    }
-    template <int N> static __device__ __forceinline__ void stv(const Field& f, unsigned o, const R (&r)[N])
-    {
-        Pack<f16, N> h;
-        Pack<int8_t, N> l;
-#pragma unroll
-        for (int j = 0; j < N; ++j) c16_encode(r[j] * f.sc, h.v[j], l.v[j], f.z);
-        st_pack<N>(f.hi, o, h);
-        st_pack<N>(f.lo, o, l);
     }
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     template <int N> static __device__ __forceinline__ void ldcv(const Coef* c, unsigned o, R (&r)[N])
@@ -255,18 +279,13 @@ struct PolC16F {
     }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned) { m[o] = v; }
-    static __device__ __forceinline__ R rt(const Field& f, R v)
-    {
-        f16 h;
-        int8_t l;
-        c16_encode(v * f.sc, h, l, f.z);
-        return c16_decode(h, l) * f.isc;
-    }
 };
 
 struct PolB16 {
     using R = float; using Coef = f16; using Mem = bf16;
+    static constexpr int form = kFormHalf;
     static constexpr bool scaled = true;
+    static constexpr bool pair_cpml = false;
     struct Field { bf16* p; unsigned z; };
     static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return (float)f.p[o]; }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v) { f.p[o] = (bf16)round32(v, f.z); }
@@ -274,7 +293,6 @@ struct PolB16 {
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return (float)c[o]; }
     static __device__ __forceinline__ R ldm(const Mem* m, unsigned o) { return (float)m[o]; }
     static __device__ __forceinline__ void stm(Mem* m, unsigned o, R v, unsigned z) { m[o] = (bf16)round32(v, z); }
-    static __device__ __forceinline__ R rt(const Field& f, R v) { return (float)(bf16)round32(v, f.z); }
 };
 
 template <class P> struct StepArgs {
@@ -295,10 +313,25 @@ template <class P> struct StepArgs {
     int xcd_map;           // 1: XCD-aware block -> tile mapping
     typename P::R dt, h;
     // compensated fields: bytes from a field's FP16 values to its residual bytes
-    // (the same for every field); comp16f: storage scale of the velocity,
-    // relative-velocity and stress classes and their inverses
+    // (the same for every field)
     unsigned loff;
-    float fsc[3], fisc[3];
+    // kFormFast launch constants (KF_*): dt/h, the factors of the staggered
+    // means and the class storage scales folded together
+    float kf[9];
+};
+
+// kFormFast constants; g = dt/h, s_u, s_w, s_t = storage scales of the solid
+// velocity, relative fluid velocity and stress classes
+enum {
+    KF_DT2 = 0,  // dt/2
+    KF_WS,       // 4 g s_w/s_t
+    KF_WP,       // 2 g s_w/s_t
+    KF_U,        // 2 g s_u/s_t
+    KF_UW,       // 2 s_u/s_w
+    KF_G,        // g s_t/s_u
+    KF_G2,       // 2 g s_t/s_u
+    KF_G4,       // 4 g s_t/s_u
+    KF_W         // s_u/s_w
 };
 
 template <class P> __host__ typename P::Field make_field(const DevBuffers& b, int k);
@@ -309,7 +342,7 @@ template <> __host__ PolB16::Field make_field<PolB16>(const DevBuffers& b, int k
 template <> __host__ PolC16F::Field make_field<PolC16F>(const DevBuffers& b, int k)
 {
     const float sc = k <= PW_VUZ ? b.sc_vel : (k <= PW_VWZ ? b.sc_vw : b.sc_stress);
-    return {(f16*)b.f[k], b.lo[k], 0u, sc, 1.0f / sc};
+    return {(f16*)b.f[k], b.lo[k], 0u, sc, 1.0f / sc};  // lo null: an FP32 field
 }
 
 template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
@@ -343,11 +376,16 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     // the pool stores each compensated field's residual bytes right after its
     // FP16 values, at one offset for all fields
     a.loff = b.lo[0] ? (unsigned)((const char*)b.lo[0] - (const char*)b.f[0]) : 0u;
-    const float sc[3] = {b.sc_vel, b.sc_vw, b.sc_stress};
-    for (int k = 0; k < 3; ++k) {
-        a.fsc[k] = sc[k];
-        a.fisc[k] = 1.0f / sc[k];
-    }
+    const double g = b.dt / b.h, su = b.sc_vel, sw = b.sc_vw, stt = b.sc_stress;
+    a.kf[KF_DT2] = (float)(0.5 * b.dt);
+    a.kf[KF_WS] = (float)(4.0 * g * sw / stt);
+    a.kf[KF_WP] = (float)(2.0 * g * sw / stt);
+    a.kf[KF_U] = (float)(2.0 * g * su / stt);
+    a.kf[KF_UW] = (float)(2.0 * su / sw);
+    a.kf[KF_G] = (float)(g * stt / su);
+    a.kf[KF_G2] = (float)(2.0 * g * stt / su);
+    a.kf[KF_G4] = (float)(4.0 * g * stt / su);
+    a.kf[KF_W] = (float)(su / sw);
     return a;
 }
 
@@ -397,16 +435,14 @@ template <> struct FA<PolB16> {
         st_pack<N>(a.f[fid].p, o, t);
     }
 };
-// comp16 (SC = false) and comp16f (SC = true, values stored times the class scale)
-template <class P, bool SC> struct FAComp {
+// comp16: the field's FP16 values at f[fid].hi, its residual bytes a.loff bytes after them
+template <> struct FA<PolC16> {
     using R = float;
-    static __device__ __forceinline__ int cls(int fid) { return fid <= PW_VUZ ? 0 : (fid <= PW_VWZ ? 1 : 2); }
+    using P = PolC16;
     static __device__ __forceinline__ R ld(const StepArgs<P>& a, int fid, unsigned o)
     {
         const f16* h = a.f[fid].hi;
-        const R v = c16_decode(h[o], ((const int8_t*)h)[a.loff + o]);
-        if constexpr (SC) return v * a.fisc[cls(fid)];
-        else return v;
+        return c16_decode(h[o], ((const int8_t*)h)[a.loff + o]);
     }
     template <int N> static __device__ __forceinline__ void ldv(const StepArgs<P>& a, int fid, unsigned o, R (&r)[N])
     {
@@ -414,18 +450,14 @@ template <class P, bool SC> struct FAComp {
         const Pack<f16, N> hv = ld_pack<N>(h, o);
         const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h, a.loff + o);
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            if constexpr (SC) r[j] = c16_decode(hv.v[j], lv.v[j]) * a.fisc[cls(fid)];
-            else r[j] = c16_decode(hv.v[j], lv.v[j]);
-        }
+        for (int j = 0; j < N; ++j) r[j] = c16_decode(hv.v[j], lv.v[j]);
     }
     static __device__ __forceinline__ void st(const StepArgs<P>& a, int fid, unsigned o, R v)
     {
         f16* h = a.f[fid].hi;
         f16 hh;
         int8_t l;
-        if constexpr (SC) c16_encode(v * a.fsc[cls(fid)], hh, l, a.zero);
-        else c16_encode(v, hh, l, a.zero);
+        c16_encode(v, hh, l, a.zero);
         h[o] = hh;
         ((int8_t*)h)[a.loff + o] = l;
     }
@@ -436,16 +468,69 @@ template <class P, bool SC> struct FAComp {
         Pack<f16, N> hv;
         Pack<int8_t, N> lv;
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            if constexpr (SC) c16_encode(r[j] * a.fsc[cls(fid)], hv.v[j], lv.v[j], a.zero);
-            else c16_encode(r[j], hv.v[j], lv.v[j], a.zero);
-        }
+        for (int j = 0; j < N; ++j) c16_encode(r[j], hv.v[j], lv.v[j], a.zero);
         st_pack<N>(h, o, hv);
         st_pack<N>((int8_t*)h, a.loff + o, lv);
     }
 };
-template <> struct FA<PolC16> : FAComp<PolC16, false> {};
-template <> struct FA<PolC16F> : FAComp<PolC16F, true> {};
+// comp16f, in stored units (the class scales are folded into the launch
+// constants); the fields of PW_C16F_PLAIN are FP32
+template <> struct FA<PolC16F> {
+    using R = float;
+    using P = PolC16F;
+    static __device__ __forceinline__ constexpr bool plain(int fid) { return c16f_plain(fid); }
+    static __device__ __forceinline__ R ld(const StepArgs<P>& a, int fid, unsigned o)
+    {
+        if (plain(fid)) return ((const float*)a.f[fid].hi)[o];
+        const f16* h = a.f[fid].hi;
+        return c16_decode(h[o], ((const int8_t*)h)[a.loff + o]);
+    }
+    template <int N> static __device__ __forceinline__ void ldv(const StepArgs<P>& a, int fid, unsigned o, R (&r)[N])
+    {
+        if (plain(fid)) {
+            const Pack<float, N> t = ld_pack<N>((const float*)a.f[fid].hi, o);
+#pragma unroll
+            for (int j = 0; j < N; ++j) r[j] = t.v[j];
+            return;
+        }
+        const f16* h = a.f[fid].hi;
+        const Pack<f16, N> hv = ld_pack<N>(h, o);
+        const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h, a.loff + o);
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = c16_decode(hv.v[j], lv.v[j]);
+    }
+    static __device__ __forceinline__ void st(const StepArgs<P>& a, int fid, unsigned o, R v)
+    {
+        if (plain(fid)) {
+            ((float*)a.f[fid].hi)[o] = v;
+            return;
+        }
+        f16* h = a.f[fid].hi;
+        f16 hh;
+        int8_t l;
+        c16_encode_rn(v, hh, l, a.zero);
+        h[o] = hh;
+        ((int8_t*)h)[a.loff + o] = l;
+    }
+    template <int N>
+    static __device__ __forceinline__ void stv(const StepArgs<P>& a, int fid, unsigned o, const R (&r)[N])
+    {
+        if (plain(fid)) {
+            Pack<float, N> t;
+#pragma unroll
+            for (int j = 0; j < N; ++j) t.v[j] = r[j];
+            st_pack<N>((float*)a.f[fid].hi, o, t);
+            return;
+        }
+        f16* h = a.f[fid].hi;
+        Pack<f16, N> hv;
+        Pack<int8_t, N> lv;
+#pragma unroll
+        for (int j = 0; j < N; ++j) c16_encode_rn(r[j], hv.v[j], lv.v[j], a.zero);
+        st_pack<N>(h, o, hv);
+        st_pack<N>((int8_t*)h, a.loff + o, lv);
+    }
+};
 
 // Slab-local index along one axis, -1 outside the CPML slabs.
 __device__ __forceinline__ int slab_index(int i, int n, int lo, int hi)
@@ -497,24 +582,37 @@ __device__ __forceinline__ typename P::R cpml_with(typename P::Mem* phi, unsigne
 // The four CPML terms of one damped axis.  All four memory variables are loaded
 // before any is stored: a store could alias the next load as far as the
 // compiler knows, so term-by-term code would make four memory round trips.
+// With P::pair_cpml the reference Half form instead (Half/main.cu:201-240):
+// psi = psi*e + (-c)*(e*prev + der) in float without contraction, psi and
+// prev = der stored as FP16, der += psi; prev lives 8 slots after psi.
 template <class P>
 __device__ __forceinline__ void cpml4(typename P::Mem* const* phi, unsigned k, const typename P::R (&c)[4],
                                       const typename P::R (&e)[4], typename P::R* const (&der)[4], unsigned z)
 {
     using R = typename P::R;
-    R m[4];
+    if constexpr (P::pair_cpml) {
+#pragma clang fp contract(off)
+        R ps[4], pv[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) m[q] = P::ldm(phi[q], k);
+        for (int q = 0; q < 4; ++q) {
+            ps[q] = P::ldm(phi[q], k);
+            pv[q] = P::ldm(phi[q + 8], k);
+        }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) *der[q] = cpml_with<P>(phi[q], k, m[q], c[q], e[q], *der[q], z);
-}
-
-// One CPML term in single-variable form (see header comment).
-template <class P>
-__device__ __forceinline__ typename P::R cpml(typename P::Mem* phi, unsigned k, typename P::R c, typename P::R e,
-                                              typename P::R der, unsigned z)
-{
-    return cpml_with<P>(phi, k, P::ldm(phi, k), c, e, der, z);
+        for (int q = 0; q < 4; ++q) {
+            const R d = *der[q];
+            const R nw = ps[q] * e[q] + (-c[q]) * (e[q] * pv[q] + d);
+            P::stm(phi[q], k, nw, z);
+            P::stm(phi[q + 8], k, d, z);
+            *der[q] = d + nw;
+        }
+    } else {
+        R m[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) m[q] = P::ldm(phi[q], k);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) *der[q] = cpml_with<P>(phi[q], k, m[q], c[q], e[q], *der[q], z);
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -545,18 +643,23 @@ __device__ __forceinline__ void cn_pair(R c1, R c2, R rf, R ri, R dt, R& cc1, R&
     cc2 = (R)((double)c2 - q - h);
 }
 
-// Staggered difference divided by the grid step.  The FP32/FP64 forms divide
-// as the reference does; the FP16-stored forms multiply by 1/h (the rounding
-// difference is far below FP16 storage resolution).
+// Staggered difference divided by the grid step as the reference programs do
+// (Float/main.cu:115-128, Half/main.cu:186-199).  kFormFast keeps the
+// undivided difference (1/h is folded into its launch constants); BF16, which
+// has no reference counterpart, multiplies by 1/h.
 template <class P> struct Deriv {
     typename P::R h, ih;
     __device__ __forceinline__ explicit Deriv(typename P::R h_) : h(h_), ih((typename P::R)1 / h_) {}
     __device__ __forceinline__ typename P::R operator()(typename P::R d) const
     {
-        if constexpr (P::scaled) return d * ih;
+        if constexpr (P::form == kFormFast) return d;
+        else if constexpr (P::scaled && !P::pair_cpml) return d * ih;
         else return d / h;
     }
 };
+
+// hardware reciprocal (v_rcp_f32, within 1 ulp); rcp(0) = inf, rcp(inf) = 0
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // --------------------------------------------------------------------------
 // Row segments.  Each lane owns N consecutive cells of one x row (N = 1, 2, 4),
@@ -920,6 +1023,34 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             seg_right(L, c2_c, c2_x, o + N, [&](unsigned q) { return P::ldc(a.c[PC_C2], q); }, [&](R (&r)[N]) { LDC(PC_C2, o + 1, r); });
             LDC(PC_C2, o + W, c2_y);
             LDC(PC_RHOF, o, rf);
+            if constexpr (P::form == kFormFast) {
+                // Float/main.cu:106-113, 160-189 in stored units, with the
+                // staggered means folded: ir = 1/(rho + rho') (rho^-1 = 2 ir),
+                // q2 = 2Q = C2 + C2' - 4 rhof^2 ir, h2 = (C1 + C1') dt/2, so
+                // 2 CC1 = q2 + h2 and 2 CC2 = q2 - h2
+                const R dt2 = a.kf[KF_DT2], kws = a.kf[KF_WS], kwp = a.kf[KF_WP], ku = a.kf[KF_U],
+                        kuw = a.kf[KF_UW];
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    const R rf4 = (R)4 * rf[j] * rf[j];
+                    auto axis = [&](R r1, R c11, R c21, R S, R dp, R w0, R u0, R& w1, R& u1) {
+                        const R ir = fast_rcp(rho_c[j] + r1);
+                        const R sc1 = c1_c[j] + c11;
+                        const R q2 = fmaf(-rf4, ir, c2_c[j] + c21);
+                        const R h2 = sc1 * dt2;
+                        const R af = rf[j] * ir;
+                        const R num = fmaf(q2 - h2, w0, -fmaf(kws * af, S, kwp * dp));
+                        w1 = sc1 == (R)0 ? (R)0 : num * fast_rcp(q2 + h2);
+                        u1 = fmaf(ku * ir, S, u0) - kuw * af * (w1 - w0);
+                    };
+                    axis(rho_x[j], c1_x[j], c2_x[j], sx[j], s1[j], wx0[j], ux0[j], wx1[j], ux1[j]);
+                    axis(rho_y[j], c1_y[j], c2_y[j], sy[j], s2[j], wy0[j], uy0[j], wy1[j], uy1[j]);
+                    axis(rho_n[j], c1_n[j], c2_n[j], sz[j], s3[j], wz0[j], uz0[j], wz1[j], uz1[j]);
+                    rho_c[j] = rho_n[j];
+                    c1_c[j] = c1_n[j];
+                    c2_c[j] = c2_n[j];
+                }
+            } else {
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 const R rix = inv_mean(rho_c[j], rho_x[j]), riy = inv_mean(rho_c[j], rho_y[j]),
@@ -943,9 +1074,12 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
                 c1_c[j] = c1_n[j];
                 c2_c[j] = c2_n[j];
             }
+            }
         } else {
             // Half/main.cu:243-262: vw = Aw*vw - Bt*S - Bp*dp (0 where Aw = Bp = Bt = 0);
-            // vu += rho~^-1*S - rhof*rho^-1*(vw_new - vw_old)
+            // vu += rho~^-1*S - rhof*rho^-1*(vw_new - vw_old), in the reference's
+            // order of float operations, without contraction
+#pragma clang fp contract(off)
             R aw[N], bp[N], bt[N], ri[N], fr[N];
             LDC(SC_AWX, o, aw); LDC(SC_BPX, o, bp); LDC(SC_BTX, o, bt); LDC(SC_RITX, o, ri); LDC(SC_FRX, o, fr);
 #pragma unroll
@@ -1050,11 +1184,25 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         LD(PW_VUY, o + PL, vuy_k);
     }
     // physical form: cell-centred mu at (x, y), (x+1, y), (x, y+1) of this plane
-    R mu_c[N], mu_x[N], mu_y[N];
+    // (kFormFast: their reciprocals, and the cell's own mu in mv_c)
+    constexpr bool FAST = P::form == kFormFast;
+    auto MU = [](R v) -> R {
+        if constexpr (FAST) return fast_rcp(v);
+        else return v;
+    };
+    auto LDMU = [&](unsigned off, R (&r)[N]) {
+        LDC(PC_MU, off, r);
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = MU(r[j]);
+    };
+    auto EMU = [&](unsigned q) { return MU(P::ldc(a.c[PC_MU], q)); };
+    R mu_c[N], mu_x[N], mu_y[N], mv_c[N];
     if constexpr (!P::scaled) {
-        LDC(PC_MU, o, mu_c);
-        seg_right(L, mu_c, mu_x, o + N, [&](unsigned q) { return P::ldc(a.c[PC_MU], q); }, [&](R (&r)[N]) { LDC(PC_MU, o + 1, r); });
-        LDC(PC_MU, o + W, mu_y);
+        LDC(PC_MU, o, mv_c);
+#pragma unroll
+        for (int j = 0; j < N; ++j) mu_c[j] = MU(mv_c[j]);
+        seg_right(L, mu_c, mu_x, o + N, EMU, [&](R (&r)[N]) { LDMU(o + 1, r); });
+        LDMU(o + W, mu_y);
     }
     const R C4A = (R)1.125, C4B = (R)(-1.0 / 24.0);
 
@@ -1141,14 +1289,18 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         LD(PW_TYZ, o, yz0);
         LD(PW_TXZ, o, xz0);
         R p1[N], xx1[N], yy1[N], zz1[N], xy1[N], yz1[N], xz1[N];
-        R mu_n[N], mu_xn[N], mu_yn[N], mu_xy[N];
+        R mu_n[N], mu_xn[N], mu_yn[N], mu_xy[N], mv_n[N];
         R C_[N], M_[N], HH_[N], H2u_[N], mxy[N], myz[N], mxz[N];
         if constexpr (!P::scaled) {
-            LDC(PC_MU, o + PL, mu_n);
-            seg_right(L, mu_n, mu_xn, o + PL + N, [&](unsigned q) { return P::ldc(a.c[PC_MU], q); }, [&](R (&r)[N]) { LDC(PC_MU, o + PL + 1, r); });
-            LDC(PC_MU, o + PL + W, mu_yn);
-            seg_right(L, mu_y, mu_xy, o + W + N, [&](unsigned q) { return P::ldc(a.c[PC_MU], q); }, [&](R (&r)[N]) { LDC(PC_MU, o + W + 1, r); });
-            LDC(PC_C, o, C_); LDC(PC_M, o, M_); LDC(PC_HH, o, HH_); LDC(PC_H2U, o, H2u_);
+            LDC(PC_MU, o + PL, mv_n);
+#pragma unroll
+            for (int j = 0; j < N; ++j) mu_n[j] = MU(mv_n[j]);
+            seg_right(L, mu_n, mu_xn, o + PL + N, EMU, [&](R (&r)[N]) { LDMU(o + PL + 1, r); });
+            LDMU(o + PL + W, mu_yn);
+            seg_right(L, mu_y, mu_xy, o + W + N, EMU, [&](R (&r)[N]) { LDMU(o + W + 1, r); });
+            LDC(PC_C, o, C_); LDC(PC_M, o, M_); LDC(PC_H2U, o, H2u_);
+            // kFormFast forms H as (H - 2 mu) + 2 mu
+            if constexpr (!FAST) LDC(PC_HH, o, HH_);
         } else {
             LDC(SC_C, o, C_); LDC(SC_M, o, M_); LDC(SC_HH, o, HH_); LDC(SC_H2U, o, H2u_);
             LDC(SC_MUXY, o, mxy); LDC(SC_MUYZ, o, myz); LDC(SC_MUXZ, o, mxz);
@@ -1180,7 +1332,29 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                          {&uzz, &wz, &uxz, &uyz}, a.zero);
             }
             const R du = uxx + uyy + uzz, dw = wx + wy + wz;
-            if constexpr (!P::scaled) {
+            if constexpr (FAST) {
+                // Float/main.cu:287-293 in stored units: the shear moduli are
+                // 4 / (sum of the four reciprocals) (0 when any mu is 0: its
+                // reciprocal is inf), H uxx + (H - 2mu)(uyy + uzz) is
+                // (H - 2mu) du + 2mu uxx, and the relative-velocity divergence is
+                // brought to velocity units by KF_W
+                const R g = a.kf[KF_G], g2 = a.kf[KF_G2], g4 = a.kf[KF_G4];
+                const R dwu = dw * a.kf[KF_W];
+                const R C = C_[j], M = M_[j], lam = H2u_[j];
+                const R com = g * fmaf(lam, du, C * dwu);
+                const R gm = g2 * mv_c[j];
+                p1[j] = fmaf(-g, fmaf(M, dwu, C * du), p0[j]);
+                xx1[j] = fmaf(gm, uxx, xx0[j] + com);
+                yy1[j] = fmaf(gm, uyy, yy0[j] + com);
+                zz1[j] = fmaf(gm, uzz, zz0[j] + com);
+                xy1[j] = fmaf(g4 * fast_rcp(mu_c[j] + mu_x[j] + mu_y[j] + mu_xy[j]), uxy + uyx, xy0[j]);
+                yz1[j] = fmaf(g4 * fast_rcp(mu_c[j] + mu_y[j] + mu_n[j] + mu_yn[j]), uyz + uzy, yz0[j]);
+                xz1[j] = fmaf(g4 * fast_rcp(mu_c[j] + mu_x[j] + mu_n[j] + mu_xn[j]), uxz + uzx, xz0[j]);
+                mu_c[j] = mu_n[j];
+                mu_x[j] = mu_xn[j];
+                mu_y[j] = mu_yn[j];
+                mv_c[j] = mv_n[j];
+            } else if constexpr (!P::scaled) {
                 // Float/main.cu:480-491
                 const R muxz = harm4(mu_c[j], mu_x[j], mu_n[j], mu_xn[j]);
                 const R muyz = harm4(mu_c[j], mu_y[j], mu_n[j], mu_yn[j]);
@@ -1198,7 +1372,8 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                 mu_x[j] = mu_xn[j];
                 mu_y[j] = mu_yn[j];
             } else {
-                // Half/main.cu:362-370
+                // Half/main.cu:362-370, without contraction
+#pragma clang fp contract(off)
                 const R C = C_[j], M = M_[j], HH = HH_[j], H2u = H2u_[j];
                 p1[j] = p0[j] - C * du - M * dw;
                 xx1[j] = xx0[j] + H2u * (uyy + uzz) + HH * uxx + C * dw;
@@ -1225,283 +1400,16 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     }
 }
 
-// Occupancy hint for the single-shot step kernels (waves per SIMD), set at build
-// time for tuning; 0 leaves the register allocation to the compiler.
-#ifndef PW_WAVES_PER_EU
-#define PW_WAVES_PER_EU 0
-#endif
-#if PW_WAVES_PER_EU > 0
-#define PW_OCCUPANCY __attribute__((amdgpu_waves_per_eu(PW_WAVES_PER_EU)))
-#else
-#define PW_OCCUPANCY
-#endif
-
-template <class P, int N, int ORD> __global__ __launch_bounds__(512) PW_OCCUPANCY void k_velocity(const StepArgs<P> a)
+// (An occupancy hint of 5 waves per SIMD for comp16f, whose kernels take 102 and
+// 124 VGPRs, spills 12 and 33 of them to scratch and measured 8 % and 3x
+// slower: the register allocation is left to the compiler.)
+template <class P, int N, int ORD> __global__ __launch_bounds__(512) void k_velocity(const StepArgs<P> a)
 {
     velocity_tile<P, N, ORD>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
 }
-template <class P, int N, int ORD> __global__ __launch_bounds__(512) PW_OCCUPANCY void k_stress(const StepArgs<P> a)
+template <class P, int N, int ORD> __global__ __launch_bounds__(512) void k_stress(const StepArgs<P> a)
 {
     stress_tile<P, N, ORD>(a, tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map));
-}
-
-// --------------------------------------------------------------------------
-// Fused step kernel: velocity and stress update of one time step in one launch.
-//
-// The state is double-buffered (read time n from s.f, write time n+1 to g), so
-// each wavefield is read once and written once per step instead of read twice
-// (39 -> 26 touches).  A block owns a 62 x 6 column tile and marches up its z
-// chunk: at plane k it computes the new velocities of plane k+1 (tile cells: all
-// three axes, stored; the one-cell ring around the tile: the axis components
-// the stress stencil reads there, not stored), then the stress update of
-// plane k from registers (own column, planes k-1..k+1) and LDS (x/y neighbours).
-// Ring cells belong to neighbouring tiles and are recomputed here from the same
-// time-n inputs, so they come out bit-identical to the owner's values.  The CPML
-// memory variables are double-buffered likewise (read m, write mo).
-// --------------------------------------------------------------------------
-template <class P> struct FusedArgs {
-    StepArgs<P> s;                        // time n (read): fields, coefficients, memory variables, geometry
-    typename P::Field g[PW_NFIELDS];      // time n+1 (write)
-    typename P::Mem* mo[3][kMemPerAxis];  // memory variables written
-};
-
-template <class P>
-__device__ __forceinline__ typename P::R cpml2(const typename P::Mem* in, typename P::Mem* out, unsigned k,
-                                               typename P::R c, typename P::R e, typename P::R der, bool write,
-                                               unsigned z)
-{
-    using R = typename P::R;
-    const R psi = P::ldm(in, k) - c * der;
-    if (write) P::stm(out, k, e * (psi - c * der), z);
-    return der + psi;
-}
-
-// Velocity update of one axis (AX 0: vux/vwx, 1: vuy/vwy, 2: vuz/vwz) at an
-// updated cell; own: this block owns the cell and writes its memory variables.
-template <class P, int AX>
-__device__ __forceinline__ void v_axis(const FusedArgs<P>& A, int ix, int iy, int iz, bool own, typename P::R& vu1,
-                                       typename P::R& vw1)
-{
-    using R = typename P::R;
-    const StepArgs<P>& a = A.s;
-    const unsigned PL = a.plane, W = (unsigned)a.pitch;
-    const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)iz * PL;
-    const Deriv<P> D(a.h);
-    const R dt = a.dt;
-    const unsigned st = AX == 0 ? 1u : (AX == 1 ? W : PL);
-    // d1: normal stress along AX (forward), d2/d3: shear stresses (backward),
-    // ds: pressure along AX (forward)  (Float/main.cu:115-128)
-    R d1, d2, d3;
-    const R ds = D(P::ld(a.f[PW_P], o + st) - P::ld(a.f[PW_P], o));
-    if (AX == 0) {
-        d1 = D(P::ld(a.f[PW_TXX], o + 1) - P::ld(a.f[PW_TXX], o));
-        d2 = D(P::ld(a.f[PW_TXY], o) - P::ld(a.f[PW_TXY], o - W));
-        d3 = D(P::ld(a.f[PW_TXZ], o) - P::ld(a.f[PW_TXZ], o - PL));
-    } else if (AX == 1) {
-        d1 = D(P::ld(a.f[PW_TYY], o + W) - P::ld(a.f[PW_TYY], o));
-        d2 = D(P::ld(a.f[PW_TXY], o) - P::ld(a.f[PW_TXY], o - 1));
-        d3 = D(P::ld(a.f[PW_TYZ], o) - P::ld(a.f[PW_TYZ], o - PL));
-    } else {
-        d1 = D(P::ld(a.f[PW_TZZ], o + PL) - P::ld(a.f[PW_TZZ], o));
-        d2 = D(P::ld(a.f[PW_TXZ], o) - P::ld(a.f[PW_TXZ], o - 1));
-        d3 = D(P::ld(a.f[PW_TYZ], o) - P::ld(a.f[PW_TYZ], o - W));
-    }
-    R dsv = ds;
-    // CPML: d1 and ds are damped along AX (half-point profile, terms 0 and 1);
-    // d2 along x (AX 1, 2) or y (AX 0); d3 along z (AX 0, 1) or y (AX 2).
-    const int xs = slab_index(ix, a.nx, a.lo[0], a.hi[0]);
-    const int ys = slab_index(iy, a.ny, a.lo[1], a.hi[1]);
-    const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
-    const unsigned kx = xs >= 0 ? ((unsigned)iz * a.ny + iy) * (a.lo[0] + a.hi[0]) + xs : 0u;
-    const unsigned ky = ys >= 0 ? ((unsigned)iz * (a.lo[1] + a.hi[1]) + ys) * W + ix : 0u;
-    const unsigned kz = zs >= 0 ? ((unsigned)zs * a.ny + iy) * W + ix : 0u;
-    const int sa = AX == 0 ? xs : (AX == 1 ? ys : zs);
-    if (sa >= 0) {
-        const Tab4<R> t = a.tab[AX][AX == 0 ? ix : (AX == 1 ? iy : iz)];
-        const unsigned k = AX == 0 ? kx : (AX == 1 ? ky : kz);
-        d1 = cpml2<P>(a.m[AX][0], A.mo[AX][0], k, t.ch, t.eh, d1, own, a.zero);
-        dsv = cpml2<P>(a.m[AX][1], A.mo[AX][1], k, t.ch, t.eh, dsv, own, a.zero);
-    }
-    if (AX == 0) {
-        if (ys >= 0) { const Tab4<R> t = a.tab[1][iy]; d2 = cpml2<P>(a.m[1][2], A.mo[1][2], ky, t.cf, t.ef, d2, own, a.zero); }
-        if (zs >= 0) { const Tab4<R> t = a.tab[2][iz]; d3 = cpml2<P>(a.m[2][2], A.mo[2][2], kz, t.cf, t.ef, d3, own, a.zero); }
-    } else if (AX == 1) {
-        if (xs >= 0) { const Tab4<R> t = a.tab[0][ix]; d2 = cpml2<P>(a.m[0][2], A.mo[0][2], kx, t.cf, t.ef, d2, own, a.zero); }
-        if (zs >= 0) { const Tab4<R> t = a.tab[2][iz]; d3 = cpml2<P>(a.m[2][3], A.mo[2][3], kz, t.cf, t.ef, d3, own, a.zero); }
-    } else {
-        if (xs >= 0) { const Tab4<R> t = a.tab[0][ix]; d2 = cpml2<P>(a.m[0][3], A.mo[0][3], kx, t.cf, t.ef, d2, own, a.zero); }
-        if (ys >= 0) { const Tab4<R> t = a.tab[1][iy]; d3 = cpml2<P>(a.m[1][3], A.mo[1][3], ky, t.cf, t.ef, d3, own, a.zero); }
-    }
-    const R S = d1 + d2 + d3;
-    const R vw0 = P::ld(a.f[PW_VWX + AX], o), vu0 = P::ld(a.f[PW_VUX + AX], o);
-    if constexpr (!P::scaled) {
-        const R ri = inv_mean(P::ldc(a.c[PC_RHO], o), P::ldc(a.c[PC_RHO], o + st));
-        const R c1 = half_sum(P::ldc(a.c[PC_C1], o), P::ldc(a.c[PC_C1], o + st));
-        const R c2 = half_sum(P::ldc(a.c[PC_C2], o), P::ldc(a.c[PC_C2], o + st));
-        const R rf = P::ldc(a.c[PC_RHOF], o);
-        R cc1, cc2;
-        cn_pair(c1, c2, rf, ri, dt, cc1, cc2);
-        vw1 = c1 == (R)0 ? (R)0 : (vw0 * cc2 - dt * (rf * ri * S + dsv)) / cc1;
-        vu1 = vu0 + dt * ri * S - rf * ri * (vw1 - vw0);
-    } else {
-        const R aw = P::ldc(a.c[SC_AWX + AX], o), bp = P::ldc(a.c[SC_BPX + AX], o), bt = P::ldc(a.c[SC_BTX + AX], o);
-        vw1 = (aw == 0.0f && bp == 0.0f && bt == 0.0f) ? 0.0f : aw * vw0 - bt * S - bp * dsv;
-        vu1 = vu0 + P::ldc(a.c[SC_RITX + AX], o) * S - P::ldc(a.c[SC_FRX + AX], o) * (vw1 - vw0);
-    }
-}
-
-// Block of 64 x ROWS threads, one cell per thread: lanes 1..62 of rows
-// 1..ROWS-2 own a 62 x (ROWS-2) tile; lane 0 / 63 and row 0 / ROWS-1 are the
-// one-cell ring around it.  Every thread computes new velocities at its cell,
-// so the ring costs whole waves of uniform work instead of a few divergent
-// lanes: ring row 0 (y - 1) computes the y components the stress stencil reads
-// there (vuy, vwy), ring row ROWS-1 (y + 1) the x and z ones (vux, vuz), the
-// rows in between all three axes (lanes 0 and 63 supply vux/vwx and vuy/vuz).
-// Only owned cells store velocities and memory variables and update stresses.
-template <class P, int ROWS>
-__global__ __launch_bounds__(64 * ROWS) void k_fused(const FusedArgs<P> A)
-{
-    using R = typename P::R;
-    constexpr int OWN_X = 62, OWN_Y = ROWS - 2;
-    const StepArgs<P>& a = A.s;
-    // velocity of the current plane for the tile and its ring: vux, vuy, vuz, vwx, vwy
-    __shared__ R lds[2][5][ROWS * 64];
-    const Tile tl = tile_of_block(a.gx, a.gy, a.nblocks, a.xcd_map);
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int ix = tl.bx * OWN_X - 1 + tx, iy = tl.by * OWN_Y - 1 + ty;
-    const int z0 = a.zb + tl.bz * a.zc;
-    const int z1 = min(z0 + a.zc, a.ze);
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const unsigned PL = a.plane, W = (unsigned)a.pitch;
-    // no early return below: every thread takes part in the barriers
-    const bool in_xy = ix >= 0 && ix <= nx - 1 && iy >= 0 && iy <= ny - 1;
-    const bool up_xy = ix >= 1 && ix <= nx - 2 && iy >= 1 && iy <= ny - 2;
-    const bool own = tx >= 1 && tx <= OWN_X && ty >= 1 && ty <= OWN_Y;
-    // axes this row computes (1 x, 2 y, 4 z): wave-uniform
-    const int rmask = ty == 0 ? 2 : (ty == ROWS - 1 ? 5 : 7);
-    const int lt = ty * 64 + tx;
-
-    // new velocities at plane z for the axes in mask; cells that are never
-    // updated keep their stored value
-    auto v_cell = [&](int z, bool wr, int mask, R* u) {
-        if (z0 >= z1) return;
-        if (up_xy && z >= 1 && z <= nz - 2) {
-            if (mask & 1) v_axis<P, 0>(A, ix, iy, z, wr, u[0], u[3]);
-            if (mask & 2) v_axis<P, 1>(A, ix, iy, z, wr, u[1], u[4]);
-            if (mask & 4) v_axis<P, 2>(A, ix, iy, z, wr, u[2], u[5]);
-            // the stress update reads the velocities as stored (Half: encoded)
-            for (int c = 0; c < 6; ++c) u[c] = P::rt(a.f[PW_VUX + c], u[c]);
-        } else if (in_xy && z >= 0 && z <= nz - 1) {
-            const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z * PL;
-            for (int c = 0; c < 3; ++c)
-                if (mask & (1 << c)) { u[c] = P::ld(a.f[PW_VUX + c], o); u[3 + c] = P::ld(a.f[PW_VWX + c], o); }
-        }
-    };
-    auto store_cell = [&](int z, const R* u) {
-        if (z0 >= z1 || !own || !up_xy || z < 1 || z > nz - 2) return;
-        const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)z * PL;
-        for (int c = 0; c < 6; ++c) P::st(A.g[PW_VUX + c], o, u[c]);
-    };
-    auto put_cell = [&](int buf, const R* u) {
-        lds[buf][0][lt] = u[0]; lds[buf][1][lt] = u[1]; lds[buf][2][lt] = u[2];
-        lds[buf][3][lt] = u[3]; lds[buf][4][lt] = u[4];
-    };
-
-    R prev[6] = {0, 0, 0, 0, 0, 0}, cur[6] = {0, 0, 0, 0, 0, 0}, nxt[6] = {0, 0, 0, 0, 0, 0};
-    // vuz, vwz below the chunk (owned by the block below): only the tile reads them
-    if (own) v_cell(z0 - 1, false, 4, prev);
-    v_cell(z0, own, rmask, cur);
-    store_cell(z0, cur);
-    put_cell(0, cur);
-    __syncthreads();
-
-    for (int k = z0; k < z1; ++k) {
-        const int cb = (k - z0) & 1, nb = cb ^ 1;
-        if (k + 1 < z1) {
-            v_cell(k + 1, own, rmask, nxt);
-            store_cell(k + 1, nxt);
-            put_cell(nb, nxt);
-        } else if (own) {
-            v_cell(k + 1, false, 3, nxt);  // vux, vuy above the chunk (owned by the block above)
-        }
-        // stress update of plane k (Float/main.cu:234-293 / Half/main.cu:297-370)
-        if (own && up_xy) {
-            const unsigned o = (unsigned)ix + (unsigned)iy * W + (unsigned)k * PL;
-            const Deriv<P> D(a.h);
-    const R dt = a.dt;
-            const R vux_l = lds[cb][0][lt - 1], vux_q = lds[cb][0][lt + 64];
-            const R vuy_h = lds[cb][1][lt - 64], vuy_r = lds[cb][1][lt + 1];
-            const R vuz_r = lds[cb][2][lt + 1], vuz_q = lds[cb][2][lt + 64];
-            const R vwx_l = lds[cb][3][lt - 1], vwy_h = lds[cb][4][lt - 64];
-            R uxx = D(cur[0] - vux_l), uyy = D(cur[1] - vuy_h), uzz = D(cur[2] - prev[2]);
-            R wx = D(cur[3] - vwx_l), wy = D(cur[4] - vwy_h), wz = D(cur[5] - prev[5]);
-            R uxy = D(vux_q - cur[0]), uyx = D(vuy_r - cur[1]);
-            R uxz = D(nxt[0] - cur[0]), uzx = D(vuz_r - cur[2]);
-            R uyz = D(nxt[1] - cur[1]), uzy = D(vuz_q - cur[2]);
-            const int xs = slab_index(ix, nx, a.lo[0], a.hi[0]);
-            const int ys = slab_index(iy, ny, a.lo[1], a.hi[1]);
-            const int zs = slab_index(k, nz, a.lo[2], a.hi[2]);
-            if (xs >= 0) {
-                const Tab4<R> t = a.tab[0][ix];
-                const unsigned q = ((unsigned)k * ny + iy) * (a.lo[0] + a.hi[0]) + xs;
-                uxx = cpml2<P>(a.m[0][4], A.mo[0][4], q, t.cf, t.ef, uxx, true, a.zero);
-                wx = cpml2<P>(a.m[0][5], A.mo[0][5], q, t.cf, t.ef, wx, true, a.zero);
-                uyx = cpml2<P>(a.m[0][6], A.mo[0][6], q, t.ch, t.eh, uyx, true, a.zero);
-                uzx = cpml2<P>(a.m[0][7], A.mo[0][7], q, t.ch, t.eh, uzx, true, a.zero);
-            }
-            if (ys >= 0) {
-                const Tab4<R> t = a.tab[1][iy];
-                const unsigned q = ((unsigned)k * (a.lo[1] + a.hi[1]) + ys) * W + ix;
-                uyy = cpml2<P>(a.m[1][4], A.mo[1][4], q, t.cf, t.ef, uyy, true, a.zero);
-                wy = cpml2<P>(a.m[1][5], A.mo[1][5], q, t.cf, t.ef, wy, true, a.zero);
-                uxy = cpml2<P>(a.m[1][6], A.mo[1][6], q, t.ch, t.eh, uxy, true, a.zero);
-                uzy = cpml2<P>(a.m[1][7], A.mo[1][7], q, t.ch, t.eh, uzy, true, a.zero);
-            }
-            if (zs >= 0) {
-                const Tab4<R> t = a.tab[2][k];
-                const unsigned q = ((unsigned)zs * ny + iy) * W + ix;
-                uzz = cpml2<P>(a.m[2][4], A.mo[2][4], q, t.cf, t.ef, uzz, true, a.zero);
-                wz = cpml2<P>(a.m[2][5], A.mo[2][5], q, t.cf, t.ef, wz, true, a.zero);
-                uxz = cpml2<P>(a.m[2][6], A.mo[2][6], q, t.ch, t.eh, uxz, true, a.zero);
-                uyz = cpml2<P>(a.m[2][7], A.mo[2][7], q, t.ch, t.eh, uyz, true, a.zero);
-            }
-            const R du = uxx + uyy + uzz, dw = wx + wy + wz;
-            R p1, xx1, yy1, zz1, xy1, yz1, xz1;
-            if constexpr (!P::scaled) {
-                const R mu_c = P::ldc(a.c[PC_MU], o), mu_x = P::ldc(a.c[PC_MU], o + 1),
-                        mu_y = P::ldc(a.c[PC_MU], o + W), mu_n = P::ldc(a.c[PC_MU], o + PL),
-                        mu_xn = P::ldc(a.c[PC_MU], o + PL + 1), mu_yn = P::ldc(a.c[PC_MU], o + PL + W),
-                        mu_xy = P::ldc(a.c[PC_MU], o + 1 + W);
-                const R muxz = harm4(mu_c, mu_x, mu_n, mu_xn);
-                const R muyz = harm4(mu_c, mu_y, mu_n, mu_yn);
-                const R muxy = harm4(mu_c, mu_x, mu_y, mu_xy);
-                const R C = P::ldc(a.c[PC_C], o), M = P::ldc(a.c[PC_M], o);
-                const R HH = P::ldc(a.c[PC_HH], o), H2u = P::ldc(a.c[PC_H2U], o);
-                p1 = P::ld(a.f[PW_P], o) - dt * (C * du + M * dw);
-                xx1 = P::ld(a.f[PW_TXX], o) + dt * (H2u * (uyy + uzz) + HH * uxx + C * dw);
-                yy1 = P::ld(a.f[PW_TYY], o) + dt * (H2u * (uxx + uzz) + HH * uyy + C * dw);
-                zz1 = P::ld(a.f[PW_TZZ], o) + dt * (H2u * (uxx + uyy) + HH * uzz + C * dw);
-                xy1 = P::ld(a.f[PW_TXY], o) + muxy * dt * (uxy + uyx);
-                yz1 = P::ld(a.f[PW_TYZ], o) + muyz * dt * (uyz + uzy);
-                xz1 = P::ld(a.f[PW_TXZ], o) + muxz * dt * (uxz + uzx);
-            } else {
-                const R C = P::ldc(a.c[SC_C], o), M = P::ldc(a.c[SC_M], o);
-                const R HH = P::ldc(a.c[SC_HH], o), H2u = P::ldc(a.c[SC_H2U], o);
-                p1 = P::ld(a.f[PW_P], o) - C * du - M * dw;
-                xx1 = P::ld(a.f[PW_TXX], o) + H2u * (uyy + uzz) + HH * uxx + C * dw;
-                yy1 = P::ld(a.f[PW_TYY], o) + H2u * (uxx + uzz) + HH * uyy + C * dw;
-                zz1 = P::ld(a.f[PW_TZZ], o) + H2u * (uxx + uyy) + HH * uzz + C * dw;
-                xy1 = P::ld(a.f[PW_TXY], o) + P::ldc(a.c[SC_MUXY], o) * (uxy + uyx);
-                yz1 = P::ld(a.f[PW_TYZ], o) + P::ldc(a.c[SC_MUYZ], o) * (uyz + uzy);
-                xz1 = P::ld(a.f[PW_TXZ], o) + P::ldc(a.c[SC_MUXZ], o) * (uxz + uzx);
-            }
-            P::st(A.g[PW_P], o, p1);
-            P::st(A.g[PW_TXX], o, xx1); P::st(A.g[PW_TYY], o, yy1); P::st(A.g[PW_TZZ], o, zz1);
-            P::st(A.g[PW_TXY], o, xy1); P::st(A.g[PW_TYZ], o, yz1); P::st(A.g[PW_TXZ], o, xz1);
-        }
-        __syncthreads();
-        for (int c = 0; c < 6; ++c) { prev[c] = cur[c]; cur[c] = nxt[c]; }
-    }
 }
 
 // --------------------------------------------------------------------------
@@ -1844,32 +1752,6 @@ hipError_t launch_stress(const DevBuffers& b, hipStream_t st, int zb, int ze)
     });
 }
 
-hipError_t launch_fused(const DevBuffers& b, hipStream_t st)
-{
-    return dispatch(b.prec, [&](auto pol) {
-        using P = decltype(pol);
-        constexpr int ROWS = 8;
-        FusedArgs<P> A;
-        DevBuffers b1 = b;
-        b1.vx = 1;
-        A.s = make_args<P>(b1);
-        // tiles of 62 x (ROWS - 2) owned cells (the kernel's ring takes the rest of the block)
-        A.s.gx = (b.g.nx + 61) / 62;
-        A.s.gy = (b.g.ny + ROWS - 3) / (ROWS - 2);
-        A.s.nblocks = A.s.gx * A.s.gy * ((A.s.ze - A.s.zb + b.zc - 1) / b.zc);
-        for (int k = 0; k < PW_NFIELDS; ++k) {
-            DevBuffers w = b;
-            w.f[k] = b.fo[k];
-            w.lo[k] = b.loo[k];
-            A.g[k] = make_field<P>(w, k);
-        }
-        for (int ax = 0; ax < 3; ++ax)
-            for (int k = 0; k < kMemPerAxis; ++k) A.mo[ax][k] = (typename P::Mem*)b.memo[ax][k];
-        k_fused<P, ROWS><<<A.s.nblocks, dim3(64, ROWS), 0, st>>>(A);
-        return hipGetLastError();
-    });
-}
-
 hipError_t launch_source(const DevBuffers& b, hipStream_t st, unsigned off, const double w[4], double amp)
 {
     return dispatch(b.prec, [&](auto pol) {
@@ -1960,7 +1842,7 @@ hipError_t batch_build(BatchGroup& g, int prec, const BatchShot* shots, int n, c
         std::vector<int> ntv((size_t)n), itv(it, it + n);
         for (int s = 0; s < n; ++s) {
             const BatchShot& b = shots[s];
-            if (b.b->prec != prec || b.b->fused) return hipErrorInvalidValue;
+            if (b.b->prec != prec) return hipErrorInvalidValue;
             ShotDesc<P>& q = d[(size_t)s];
             q.a = make_args<P>(*b.b);
             q.amp = (const R*)b.amp;

@@ -163,7 +163,7 @@ __global__ void k_coef_physical(BaseView v, int nx, int ny, int nz, double dt, C
     c[PC_C2][o] = to_store<T>(v.at(PW_C2, ix, iy, iz));
     c[PC_C][o] = to_store<T>(v.at(PW_CC, ix, iy, iz));
     c[PC_M][o] = to_store<T>(v.at(PW_MM, ix, iy, iz));
-    c[PC_HH][o] = to_store<T>(v.at(PW_HH, ix, iy, iz));
+    if (c[PC_HH]) c[PC_HH][o] = to_store<T>(v.at(PW_HH, ix, iy, iz));  // null: comp16f
     c[PC_H2U][o] = to_store<T>(v.at(PW_H2U, ix, iy, iz));
     c[PC_MU][o] = to_store<T>(v.at(PW_MU, ix, iy, iz));
     (void)ny; (void)nz; (void)dt;

@@ -46,6 +46,7 @@ struct Shot {
     pw_status err = PW_OK;
     std::string errmsg;
     double t_submit = -1, t_admit = -1, t_done = -1;
+    int waiters = 0;       // threads inside pw_server_wait on this shot (release refuses meanwhile)
     size_t bytes = 0;      // device bytes while admitted (estimate used for admission)
     long long cells = 0;
     // while admitted
@@ -67,7 +68,7 @@ size_t shot_bytes(const pw_shot& q)
     c.nx = q.nx; c.ny = q.ny; c.nz = q.nz; c.np = q.np;
     c.h = q.h; c.dt = q.dt; c.nt = q.nt; c.precision = q.precision;
     c.cpml_r = q.cpml_r; c.cpml_v0 = q.cpml_v0;
-    size_t b = solver_pool_bytes(c, 0);
+    size_t b = solver_pool_bytes(c);
     b += al256((size_t)q.nrec * 4) * 2 + al256((size_t)q.nrec * q.nt * 4);
     b += al256((size_t)q.nt * 8) + al256((size_t)q.nt * 8) + al256((size_t)q.nt * 4);
     return b;
@@ -118,7 +119,7 @@ pw_status setup_shot(pw_server* srv, Shot* q)
     c.cpml_r = r.cpml_r; c.cpml_v0 = r.cpml_v0;
     c.device = srv->cfg.device;
     c.stream = srv->stream;
-    pw_status st = solver_create(&c, 0, 0, &q->s);
+    pw_status st = solver_create(&c, &q->s);
     if (st != PW_OK) return st;
     pw_solver* s = q->s;
     const int nxi = r.nx - 2 * r.np, nyi = r.ny - 2 * r.np, nzi = r.nz - 2 * r.np;
@@ -572,22 +573,25 @@ extern "C" pw_status pw_server_set_medium_layered(pw_server* srv, const pw_layer
 extern "C" pw_status pw_server_submit(pw_server* srv, const pw_shot* shot, long long* id)
 {
     if (!srv || !id) return fail(PW_ERR_INVALID, "pw_server_submit: null argument");
+    if (!shot) return fail(PW_ERR_INVALID, "pw_server_submit: null shot");
+    if (shot->nrec < 0 || (shot->nrec > 0 && (!shot->rec_field || !shot->rec_index)))
+        return fail(PW_ERR_INVALID, "pw_server_submit: receivers");
+    // the record is built first (host copies, byte estimate); validation against
+    // the medium and queueing then happen under one lock, so that
+    // pw_server_set_medium cannot replace the medium between the two
     std::unique_ptr<Shot> q(new Shot());
-    {
-        std::lock_guard<std::mutex> lk(srv->mu);
-        pw_status st = check_shot(srv, shot);
-        if (st != PW_OK) return st;
-    }
     q->req = *shot;
     q->rfield.assign(shot->rec_field, shot->rec_field + shot->nrec);
     q->rindex.assign(shot->rec_index, shot->rec_index + shot->nrec);
     q->req.rec_field = nullptr;
     q->req.rec_index = nullptr;
-    q->bytes = shot_bytes(*shot);
     q->cells = (long long)shot->nx * shot->ny * shot->nz;
     {
         std::lock_guard<std::mutex> lk(srv->mu);
+        pw_status st = check_shot(srv, shot);
+        if (st != PW_OK) return st;
         if (srv->stop) return fail(PW_ERR_STATE, "pw_server_submit: server is stopping");
+        q->bytes = shot_bytes(*shot);
         q->id = srv->next_id++;
         q->t_submit = srv->clock();
         *id = q->id;
@@ -605,12 +609,15 @@ extern "C" pw_status pw_server_wait(pw_server* srv, long long id, double timeout
     std::unique_lock<std::mutex> lk(srv->mu);
     Shot* q = find_shot(srv, id);
     if (!q) return fail(PW_ERR_INVALID, "pw_server_wait: unknown shot id");
+    // q stays valid while the lock is released inside the waits: pw_server_release
+    // refuses a shot with waiters
+    q->waiters++;
     auto ready = [&] { return q->state == PW_SHOT_DONE || q->state == PW_SHOT_FAILED; };
-    if (timeout_s < 0) {
-        srv->cv_done.wait(lk, ready);
-    } else if (!srv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
-        return fail(PW_ERR_TIMEOUT, "pw_server_wait: shot " + std::to_string(id) + " not finished");
-    }
+    bool ok = true;
+    if (timeout_s < 0) srv->cv_done.wait(lk, ready);
+    else ok = srv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready);
+    q->waiters--;
+    if (!ok) return fail(PW_ERR_TIMEOUT, "pw_server_wait: shot " + std::to_string(id) + " not finished");
     if (q->state == PW_SHOT_FAILED) return fail(q->err, q->errmsg);
     return PW_OK;
 }
@@ -656,6 +663,7 @@ extern "C" pw_status pw_server_release(pw_server* srv, long long id)
     if (!q) return fail(PW_ERR_INVALID, "pw_server_release: unknown shot id");
     if (q->state != PW_SHOT_DONE && q->state != PW_SHOT_FAILED)
         return fail(PW_ERR_STATE, "pw_server_release: shot still queued or running");
+    if (q->waiters > 0) return fail(PW_ERR_STATE, "pw_server_release: another thread is waiting on the shot");
     srv->shots.erase(id);
     return PW_OK;
 }

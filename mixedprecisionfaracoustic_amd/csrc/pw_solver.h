@@ -63,12 +63,11 @@ struct pw_solver {
 
 namespace pw {
 
-// Creates a solver; fused / slab select the step schedule (pw_create reads
-// them from PW_FUSED / PW_SLAB).
-pw_status solver_create(const pw_config* cfg, int fused, int slab, pw_solver** out);
+// Creates a solver (pw_create).
+pw_status solver_create(const pw_config* cfg, pw_solver** out);
 // Device bytes pw_create allocates for this configuration (state, memory
 // variables, coefficients, tables), without the receiver and snapshot buffers.
-size_t solver_pool_bytes(const pw_config& c, int fused);
+size_t solver_pool_bytes(const pw_config& c);
 // Installs a medium from device-resident FP64 base fields: field k of cell
 // (x, y, z) of the interior at d_base[k] + x + y*row_stride + z*plane_stride.
 pw_status install_medium(pw_solver* s, const double* const* d_base, int nxi, int nyi, int nzi, long long row_stride,

@@ -70,7 +70,7 @@ def layered_case(nx=40, ny=36, nz=44, np_=6, nt=150, perturb=0.05, seed=0):
 def make_solver(grid, nt, precision, base=None, spec=None):
     s = pw.Solver(grid["nx"], grid["ny"], grid["nz"], grid["np"], grid["h"], grid["dt"], nt, precision=precision,
                   cpml_r=grid["rc"], cpml_v0=grid["v0"], space_order=grid.get("order", 2),
-                  cpml_open=grid.get("open", 0))
+                  cpml_open=grid.get("open", 0), tiling=grid.get("tiling"))
     if spec is not None:
         s.set_medium_layered(spec)
     else:

@@ -25,7 +25,7 @@ import pytest
 import cases
 import mixedprecisionfaracoustic_amd as pw
 from cases import rel_l2
-from test_gpu_parity import FIELDS, check, fp32_floor
+from test_gpu_parity import FIELDS, check, check_exact, fp32_floor
 
 pytestmark = pytest.mark.gpu
 
@@ -67,9 +67,9 @@ def test_ragged_grids_match_oracle(shape, np_):
     g64, _ = cases.run_gpu(grid, nt, "fp64", recs, base=base, fields=FIELDS)
     check(g64, ref64, recs, 1e-10, f"fp64 {shape}")
     refh = cases.run_oracle("half", grid, nt, recs, base=base, final=True)
-    floor_h, _ = fp32_floor(grid, nt, recs, base, refh, "Half")
+    _, ref64h = fp32_floor(grid, nt, recs, base, refh, "Half")
     g16, _ = cases.run_gpu(grid, nt, "comp16", recs, base=base, fields=FIELDS)
-    check(g16, refh, recs, 1e-3, f"comp16 {shape} vs Half", floor=floor_h)
+    check_exact(g16, refh, recs, f"comp16 {shape} vs Half", ref64h)
     # edge cells are never updated: their traces are exactly zero
     nx = grid["nx"]
     for k, (f, i) in enumerate(recs):
@@ -78,17 +78,17 @@ def test_ragged_grids_match_oracle(shape, np_):
 
 
 TILINGS = (
-    {"PW_WX": "1", "PW_VX": "1"},  # one wave per 64-cell row in every kernel
-    {"PW_VX": "2"},                # 128-cell row segments, 2 cells per lane
-    {"PW_VX": "4", "PW_TY": "8"},  # 256-cell row segments, 8-row tiles
-    {"PW_WX": "2"},                # two waves per 128-cell row
-    {"PW_WX": "2", "PW_TY": "8"},
-    {"PW_TY": "2"},
+    {"waves_x": 1, "cells": 1},  # one wave per 64-cell row in every kernel
+    {"cells": 2},                # 128-cell row segments, 2 cells per lane
+    {"cells": 4, "rows": 8},     # 256-cell row segments, 8-row tiles
+    {"waves_x": 2},              # two waves per 128-cell row
+    {"waves_x": 2, "rows": 8},
+    {"rows": 2},
 )
 
 
 @pytest.mark.parametrize("shape,np_,src", [((300, 11, 19), 3, (270, 5, 9)), ((130, 9, 13), 2, (100, 4, 6))])
-def test_ragged_tilings_are_bit_identical(shape, np_, src, monkeypatch):
+def test_ragged_tilings_are_bit_identical(shape, np_, src):
     """Row tails: 300 cells end 44 cells into the fifth 64-cell tile, the third
     128-cell one and the second 256-cell one; 130 cells leave 2 cells (both
     edge or CPML) in the last tile of every width. The source sits near the high
@@ -99,12 +99,11 @@ def test_ragged_tilings_are_bit_identical(shape, np_, src, monkeypatch):
     bad = []
     for prec in ("fp32", "comp16", "comp16f", "bf16", "fp64"):
         outs = []
-        for env in ({},) + TILINGS:
-            for k in ("PW_VX", "PW_TY", "PW_WX"):
-                monkeypatch.delenv(k, raising=False)
-            for k, v in env.items():
-                monkeypatch.setenv(k, "2" if (prec == "fp64" and k == "PW_VX" and v == "4") else v)
-            g, _ = cases.run_gpu(grid, nt, prec, recs, base=base, fields=FIELDS)
+        for tiling in ({},) + TILINGS:
+            t = dict(tiling)
+            if prec == "fp64" and t.get("cells") == 4:
+                t["cells"] = 2
+            g, _ = cases.run_gpu(dict(grid, tiling=t), nt, prec, recs, base=base, fields=FIELDS)
             outs.append(g)
         ref = outs[0]
         assert np.abs(ref["seis"]).max() > 0, prec

@@ -2,7 +2,8 @@
 need no CPU reference (the oracle cannot run these sizes in test time; it is
 compared with the GPU at smaller sizes in the other GPU tests):
 
-* 512^3, the two-layer perturbed workload of the benchmark (FP32, comp16);
+* 512^3, the two-layer perturbed workload of the benchmark, in every storage
+  precision (FP32, comp16, comp16f, BF16 and the FP64 run);
 * 1024^3 with 8 random layers, NP 32, dx 4 m, dt 0.4 ms, 20 Hz: the largest
   workload (about 100 GB of FP32 state, coefficients and memory variables).
 
@@ -43,7 +44,8 @@ def linearity(a, b, tol):
     return exact
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("comp16", 1e-5)])
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("comp16", 1e-5), ("comp16f", 1e-5), ("bf16", 1e-5),
+                                      ("fp64", 1e-12)])
 def test_512_workload_linear_and_chunk_exact(prec, tol):
     n, np_, nt = 512, 20, 100
     s = pw.Solver(n, n, n, np_, 5.0, 0.5e-3, nt, precision=prec, cpml_r=1e-4, cpml_v0=4000.0)
@@ -67,7 +69,7 @@ def test_512_workload_linear_and_chunk_exact(prec, tol):
     doubled = s.seismogram()
     s.close()
     exact = linearity(whole, doubled, tol)
-    if prec == "fp32":
+    if prec in ("fp32", "fp64"):
         assert exact > 0.99
 
 

@@ -9,7 +9,7 @@ import pytest
 
 import cases
 from cases import rel_l2
-from test_gpu_parity import FIELDS, check, fp32_floor
+from test_gpu_parity import FIELDS, check, check_exact, fp32_floor
 
 pytestmark = pytest.mark.gpu
 
@@ -63,7 +63,7 @@ def test_comp16_z_slab_matches_half_oracle(layered_open):
     nt, recs, spec, out = layered_open
     g, _, base = out["z slab (interior rank)"]
     refh = cases.run_oracle("half", g, nt, recs, base=base, final=True)
-    floor, _ = fp32_floor(g, nt, recs, base, refh, "Half")
+    _, ref64 = fp32_floor(g, nt, recs, base, refh, "Half")
     g16, _ = cases.run_gpu(g, nt, "comp16", recs, spec=spec, fields=FIELDS)
-    check(g16, refh, recs, 1e-3, "comp16 open z slab vs Half", floor=floor)
+    check_exact(g16, refh, recs, "comp16 open z slab vs Half", ref64)
     assert np.isfinite(g16["seis"]).all()

@@ -20,7 +20,7 @@ FIELDS = ("vux", "vuz", "vwx", "vwz", "txx", "tzz", "txz", "p")
 
 def run_gpu4(grid, nt, prec, recs, base, fields=FIELDS):
     s = pw.Solver(grid["nx"], grid["ny"], grid["nz"], grid["np"], grid["h"], grid["dt"], nt, precision=prec,
-                  cpml_r=grid["rc"], cpml_v0=grid["v0"], space_order=4)
+                  cpml_r=grid["rc"], cpml_v0=grid["v0"], space_order=4, tiling=grid.get("tiling"))
     s.set_medium(base)
     s.set_source(*grid["src"], grid["f0"], grid["src_w"])
     s.set_receivers([f for f, _ in recs], [i for _, i in recs])
@@ -90,19 +90,15 @@ def test_order4_reduced_precisions(case4):
             assert e[c] <= tol, (prec, c, e[c])
 
 
-def test_order4_tilings_are_bit_identical(case4, monkeypatch):
+def test_order4_tilings_are_bit_identical(case4):
     """The 4th-order kernels tile like the 2nd-order ones (BF16's velocity kernel
     puts two waves side by side by default): one or two waves per tile row, and
     64 x 2 or 64 x 8 tiles, give the same bits."""
     grid, base, nt, recs, _, _ = case4
     for prec in ("bf16", "fp32", "comp16"):
         outs = []
-        for env in ({}, {"PW_WX": "1"}, {"PW_WX": "2"}, {"PW_TY": "8"}, {"PW_TY": "2", "PW_WX": "2"}):
-            for k in ("PW_WX", "PW_TY"):
-                monkeypatch.delenv(k, raising=False)
-            for k, v in env.items():
-                monkeypatch.setenv(k, v)
-            outs.append((env, run_gpu4(grid, nt, prec, recs, base)))
+        for t in ({}, {"waves_x": 1}, {"waves_x": 2}, {"rows": 8}, {"rows": 2, "waves_x": 2}):
+            outs.append((t, run_gpu4(dict(grid, tiling=t), nt, prec, recs, base)))
         for env, g in outs[1:]:
             assert np.array_equal(g["seis"], outs[0][1]["seis"]), (prec, env)
             for f in FIELDS:

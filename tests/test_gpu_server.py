@@ -179,3 +179,158 @@ def test_failing_shots_do_not_disturb_others(volume, alone):
         with pytest.raises(pw.PorowaveError) as ei:
             srv.wait(i, timeout=60)
         assert ei.value.status == 3  # PW_ERR_OUT_OF_MEMORY
+
+
+def test_server_shots_match_the_oracle(volume):
+    """Server output against the CPU oracle, not only against the single-shot
+    path: in a mix of four shots, the FP32 shot is within 1e-5 of the Float
+    oracle (or twice the oracle's own distance to FP64 where that is larger) and
+    the comp16 shot within 1e-6 of the Half oracle on the same cropped medium."""
+    shots = shot_list()
+    mix = [dict(shots[0], nt=150), dict(shots[1], nt=160), shots[3], shots[4]]
+    res, st, stats = run_server(volume, mix, max_active=4, round_steps=16)
+    assert stats["completed"] == len(mix) and stats["failed"] == 0
+    for q, got in zip(mix[:2], res[:2]):
+        nx, ny, nz = q["nx"], q["ny"], q["nz"]
+        cx, cy, cz = q["crop"]
+        crop = volume[:, cz:cz + nz - 2 * NP, cy:cy + ny - 2 * NP, cx:cx + nx - 2 * NP]
+        grid = {"nx": nx, "ny": ny, "nz": nz, "np": NP, "h": H, "dt": DT, "rc": 1e-4, "v0": 4000.0,
+                "f0": q["f0"], "src": q["src"], "src_w": q["w"]}
+        recs = list(zip(q["fields"], q["idx"]))
+        r64 = cases.run_oracle("f64", grid, q["nt"], recs, base=crop)["seis"]
+        assert np.abs(r64).max() > 0
+        if q["prec"] == "fp32":
+            ref = cases.run_oracle("f32", grid, q["nt"], recs, base=crop)["seis"]
+            tol = max(1e-5, 2.0 * cases.rel_l2(ref, r64))
+        else:
+            ref = cases.run_oracle("half", grid, q["nt"], recs, base=crop)["seis"]
+            tol = 1e-6
+        e = cases.rel_l2(got, ref)
+        print(f"server {q['prec']} shot vs oracle: rel-L2 {e:.2e} (tolerance {tol:.1e}; "
+              f"oracle vs FP64 {cases.rel_l2(ref, r64):.2e})")
+        assert e <= tol, (q["prec"], e)
+
+
+def test_server_at_trace_grid_sizes_matches_solver():
+    """One shot of each grid size of the serving trace (96^3, 128^3, 192^3 and
+    256^3 total, 20-cell CPML, 500-800 steps, FP32 and comp16 alternating),
+    cropped from a two-layer perturbed volume: every seismogram bit-identical to
+    the same shot run alone through the Solver."""
+    np_ = 20
+    n_vol = 256
+    spec = pw.layered([0, n_vol // 2], [cases.sandstone(), cases.lower_layer()], perturb=0.05, corr_cells=8.0, seed=0)
+    s = pw.Solver(n_vol, n_vol, n_vol, np_, H, DT, 2, cpml_r=1e-4)
+    s.set_medium_layered(spec)
+    vol = s.base_fields()
+    s.close()
+    vi = n_vol - 2 * np_
+    rng = np.random.default_rng(4)
+    shots = []
+    for k, (n, nt) in enumerate(((96, 500), (128, 600), (192, 700), (256, 800))):
+        ni = n - 2 * np_
+        crop = tuple(int(v) for v in rng.integers(0, vi - ni + 1, 3))
+        src = tuple(int(v) for v in rng.integers(np_, n - np_, 3))
+        xs = np.linspace(np_, n - 1 - np_, 32).round().astype(int)
+        idx = [int(x) + (n // 2) * n + (np_ + 10) * n * n for x in xs] * 2
+        shots.append(dict(n=n, nt=nt, crop=crop, src=src, prec="fp32" if k % 2 == 0 else "comp16",
+                          fields=["txx"] * 32 + ["vux"] * 32, idx=idx))
+    with pw.Server(max_active=4, round_steps=32) as srv:
+        srv.set_medium(vol)
+        ids = [srv.submit(q["n"], q["n"], q["n"], np_, H, DT, q["nt"], q["prec"], q["crop"], q["src"], 25.0,
+                          q["fields"], q["idx"], weights=(1.0, 1.0, 1.0, -1.0), cpml_r=1e-4, cpml_v0=4000.0)
+               for q in shots]
+        res = []
+        for i in ids:
+            srv.wait(i, timeout=300)
+            res.append(srv.result(i))
+        stats = srv.stats()
+    assert stats["completed"] == 4 and stats["peak_active"] == 4
+    for q, got in zip(shots, res):
+        n, (cx, cy, cz) = q["n"], q["crop"]
+        ni = n - 2 * np_
+        sv = pw.Solver(n, n, n, np_, H, DT, q["nt"], precision=q["prec"], cpml_r=1e-4, cpml_v0=4000.0)
+        sv.set_medium(vol[:, cz:cz + ni, cy:cy + ni, cx:cx + ni])
+        sv.set_source(*q["src"], 25.0, (1.0, 1.0, 1.0, -1.0))
+        sv.set_receivers(q["fields"], q["idx"])
+        sv.run(q["nt"])
+        ref = sv.seismogram()
+        sv.close()
+        assert np.isfinite(got).all() and np.abs(got).max() > 0, q["n"]
+        assert np.array_equal(got, ref), (q["n"], q["prec"], cases.rel_l2(got, ref))
+
+
+def test_submissions_race_medium_changes(volume):
+    """pw_server_submit checks a shot against the medium and queues it under one
+    lock, and pw_server_set_medium refuses while any shot is queued or running:
+    with one thread replacing the medium (by a smaller volume every other time)
+    while another submits, every accepted shot runs on the medium it was checked
+    against, so it completes and matches the Solver."""
+    import threading
+    shots = shot_list()
+    q = dict(shots[3], nt=40)  # its crop reaches x = 38 of the 44-cell volume
+    ref = solver_run(volume, q)
+    small = np.ascontiguousarray(volume[:, :30, :30, :30])  # too small for the shot's crop
+    accepted, refused_sets = [], [0]
+    with pw.Server(max_active=4, round_steps=8) as srv:
+        srv.set_medium(volume)
+        stop = threading.Event()
+
+        def changer():
+            k = 0
+            while not stop.is_set():
+                try:
+                    srv.set_medium(small if k % 2 else volume)
+                except pw.PorowaveError as e:
+                    assert e.status == 4  # PW_ERR_STATE: shots queued or running
+                    refused_sets[0] += 1
+                k += 1
+
+        th = threading.Thread(target=changer)
+        th.start()
+        try:
+            for _ in range(40):
+                try:
+                    accepted.append(submit(srv, q))
+                except pw.PorowaveError as e:
+                    assert e.status in (1, 4), e  # crop outside the current (small) volume
+        finally:
+            stop.set()
+            th.join()
+        for i in accepted:
+            srv.wait(i, timeout=120)
+            assert np.array_equal(srv.result(i), ref)
+    print(f"accepted {len(accepted)} of 40 submissions; medium changes refused {refused_sets[0]}")
+    assert accepted
+
+
+def test_release_refuses_while_another_thread_waits(volume):
+    """A shot cannot be released (and its record freed) while another thread is
+    inside pw_server_wait on it: release is refused until the waiter has
+    returned, and then succeeds."""
+    import threading
+    import time
+    q = dict(shot_list()[0], nt=400)
+    with pw.Server(max_active=1) as srv:
+        srv.set_medium(volume)
+        i = submit(srv, q)
+        returned = threading.Event()
+
+        def waiter():
+            srv.wait(i, timeout=120)
+            returned.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        refused = 0
+        while True:
+            try:
+                srv.release(i)
+                break
+            except pw.PorowaveError as e:
+                assert e.status == 4  # PW_ERR_STATE: running, or a thread still waits
+                refused += 1
+                time.sleep(0.002)
+        th.join()
+        assert returned.is_set() and refused > 0
+        with pytest.raises(pw.PorowaveError):
+            srv.status(i)  # released

@@ -115,7 +115,7 @@ def test_server_needs_a_device():
         pw.Server()
 
 
-def test_config_validation_errors(monkeypatch):
+def test_config_validation_errors():
     """Bad configurations are refused with their reason before any device call."""
     with pytest.raises(pw.PorowaveError) as ei:
         pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, space_order=3)
@@ -132,10 +132,17 @@ def test_config_validation_errors(monkeypatch):
     with pytest.raises(pw.PorowaveError) as ei:
         pw.Solver(1700, 1700, 1600, 20, 5.0, 5e-4, 10, precision="fp32")
     assert ei.value.status == 6 and "2^32 padded cells" in str(ei.value)
-    monkeypatch.setenv("PW_FUSED", "1")
     with pytest.raises(pw.PorowaveError) as ei:
-        pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, space_order=4)
-    assert ei.value.status == 6 and "fused" in str(ei.value)  # PW_ERR_UNSUPPORTED
+        pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, tiling={"cells": 3})
+    assert ei.value.status == 1 and "tile_cells" in str(ei.value)
+    with pytest.raises(pw.PorowaveError) as ei:
+        pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, space_order=4, tiling={"cells": 2})
+    assert ei.value.status == 6 and "one cell per lane" in str(ei.value)  # PW_ERR_UNSUPPORTED
+    with pytest.raises(pw.PorowaveError) as ei:
+        pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, tiling={"rows": 1, "waves_x": 2})
+    assert ei.value.status == 1 and "tile_rows" in str(ei.value)
+    with pytest.raises(ValueError):
+        pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, tiling={"zc": 4})
 
 
 def test_binary_files_round_trip(tmp_path):
