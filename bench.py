@@ -1,7 +1,7 @@
 """Benchmark: Gcell-steps/s of the Biot staggered-grid step on one MI355X.
 
 Workload (default): 512^3 total grid, dx = 5 m, dt = 0.5 ms, 20-cell CPML
-(R = 1e-4, v0 = 4000 m/s), two layers split at z = 256 (upper: water-saturated
+(R = 1e-4, d0 from the medium's fastest P velocity), two layers split at z = 256 (upper: water-saturated
 sandstone Ks 35 GPa, Kb 7, mu 9, Kf 2.25, rho_s 2650, rho_f 1000, phi 0.2,
 kappa 1e-12, eta 1e-3, tortuosity 2; lower: Kb 15, mu 14, phi 0.1, kappa 1e-14),
 Kb/mu/phi perturbed by 5 % smoothed Gaussian noise (sigma 8 cells, seed 0),
@@ -25,6 +25,8 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-steps/s & HBM GB/s, FP32 vs comp-FP16 @512^3; shot p50/p99 latency"
+# CPML velocity: 0 lets the library take d0 from the medium's fastest P velocity
+CPML_V0 = 0.0
 
 
 def workload(n, np_):
@@ -81,13 +83,14 @@ def measured_traffic(precision, kernel, n):
     return None
 
 
-def cpu_baseline(n, np_, spec_base, dt, h, steps, bytes_per_cell_step, order=2):
+def cpu_baseline(n, np_, spec_base, dt, h, steps, bytes_per_cell_step, order=2, v0=4000.0):
     """The oracle (OpenMP C restatement of Float/main.cu) on the same grid and medium,
     slab-CPML mode, timed on a bounded number of steps; its effective GB/s is priced
-    at the same bytes per cell-step as the GPU line."""
+    at the same bytes per cell-step as the GPU line.  v0: the CPML velocity the
+    library took from the medium."""
     import numpy as np
     from oracle import pw_oracle as O
-    grid = {"nx": n, "ny": n, "nz": n, "np": np_, "h": h, "dt": dt, "rc": 1e-4, "v0": 4000.0, "f0": 25.0,
+    grid = {"nx": n, "ny": n, "nz": n, "np": np_, "h": h, "dt": dt, "rc": 1e-4, "v0": float(v0), "f0": 25.0,
             "src": (n // 2, n // 2, n // 2), "src_w": (1.0, 1.0, 1.0, -1.0), "order": order}
     base = spec_base.astype(np.float32)
     t0 = time.perf_counter()
@@ -114,7 +117,7 @@ def cpu_baseline(n, np_, spec_base, dt, h, steps, bytes_per_cell_step, order=2):
 def serving_trace(nshots, rate, seed, n_vol, np_):
     """Shot requests: Poisson arrivals (mean rate per second), total grid N uniform
     in {96, 128, 192, 256}, source uniform in the non-CPML interior, 500-1500
-    steps, FP32 or comp16 with equal odds, medium cropped from the interior of
+    steps, FP32 or compensated FP16 (comp16f) with equal odds, medium cropped from the interior of
     the 512^3 two-layer model at a random origin, 64 receivers x (txx, vux) on
     a line along x at iz = np + 10, iy = N / 2."""
     import numpy as np
@@ -129,7 +132,7 @@ def serving_trace(nshots, rate, seed, n_vol, np_):
         crop = tuple(int(v) for v in rng.integers(0, vol_i - ni + 1, 3))
         src = tuple(int(v) for v in rng.integers(np_, n - np_, 3))
         nt = int(rng.integers(500, 1501))
-        prec = "fp32" if rng.random() < 0.5 else "comp16"
+        prec = "fp32" if rng.random() < 0.5 else "comp16f"
         xs = np.linspace(np_, n - 1 - np_, 64).round().astype(np.int64)
         iz, iy = np_ + 10, n // 2
         fields = ["txx"] * 64 + ["vux"] * 64
@@ -139,6 +142,7 @@ def serving_trace(nshots, rate, seed, n_vol, np_):
 
 
 def run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=False, **server_kw):
+    """(server_kw: pw.Server keyword arguments; none gives the server's defaults.)"""
     """Replays the trace against one server: each request is submitted at its
     arrival time; returns per-shot timings and the server's statistics."""
     import threading
@@ -157,7 +161,7 @@ def run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=False, **server_kw):
                 time.sleep(delay)
             prec = "fp32" if force_fp32 else q["prec"]
             ids[k] = srv.submit(q["n"], q["n"], q["n"], np_, h, dt, q["nt"], prec, q["crop"], q["src"], 25.0,
-                                q["fields"], q["idx"], weights=(1.0, 1.0, 1.0, -1.0), cpml_r=1e-4, cpml_v0=4000.0)
+                                q["fields"], q["idx"], weights=(1.0, 1.0, 1.0, -1.0), cpml_r=1e-4, cpml_v0=CPML_V0)
 
     th = threading.Thread(target=feeder)
     th.start()
@@ -194,7 +198,7 @@ def serving_main(args):
     trace = serving_trace(args.shots, args.rate, args.seed, n_vol, np_)
     print(f"serving: {len(trace)} shots over {trace[-1]['t']:.1f} s of arrivals", file=sys.stderr, flush=True)
     batched, seis_b = run_serving(trace, n_vol, np_, h, dt, spec, max_active=args.slots, round_steps=args.round_steps,
-                                  use_graphs=not args.no_graphs, batched=True)
+                                  use_graphs=not args.no_graphs, batched=True, fill_cells=args.fill_cells)
     print("  batched:", json.dumps(batched), file=sys.stderr, flush=True)
     seq, seis_s = run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=True, max_active=1, round_steps=64,
                               use_graphs=False, batched=False)
@@ -210,11 +214,12 @@ def serving_main(args):
            "higher_is_better": True, "data": "synthetic (Poisson trace, seeded)",
            "config": {"workload": "shot_serving_trace", "shots": args.shots, "rate_per_s": args.rate,
                       "seed": args.seed, "grids": [96, 128, 192, 256], "steps": [500, 1500],
-                      "precisions": "fp32/comp16 50/50", "slots": args.slots, "round_steps": args.round_steps,
-                      "graphs": not args.no_graphs},
+                      "precisions": "fp32/comp16f 50/50", "slots": args.slots, "round_steps": args.round_steps,
+                      "graphs": not args.no_graphs, "fill_cells": args.fill_cells},
            "serving": batched, "sequential_fp32": seq,
            "throughput_ratio": round(batched["gcell_steps_per_s"] / seq["gcell_steps_per_s"], 3),
-           "p50_ratio": round(seq["p50_latency_s"] / batched["p50_latency_s"], 3),
+           "p50_ratio": round(batched["p50_latency_s"] / seq["p50_latency_s"], 3),
+           "p99_ratio": round(batched["p99_latency_s"] / seq["p99_latency_s"], 3),
            "fp32_shots_identical_to_sequential": all(e == 0.0 for e in err)}
     print(json.dumps(out), flush=True)
     return 0
@@ -235,7 +240,7 @@ def precision_report(args):
     res = {}
     for prec in ("fp64", "fp32", "comp16", "comp16f", "bf16"):
         stream = torch.cuda.Stream()
-        s = pw.Solver(n, n, n, np_, h, dt, nt, precision=prec, cpml_r=1e-4, cpml_v0=4000.0, stream=stream.cuda_stream)
+        s = pw.Solver(n, n, n, np_, h, dt, nt, precision=prec, cpml_r=1e-4, cpml_v0=CPML_V0, stream=stream.cuda_stream)
         s.set_medium_layered(spec)
         s.set_source(n // 2, n // 2, n // 2, 25.0, (1.0, 1.0, 1.0, -1.0))
         s.set_receivers(fields, idx)
@@ -294,6 +299,36 @@ def reference_scheme_bytes(prec, n, np_):
     return 19 * s + mv * c + 16 * f * m, 20 * s + mt * c + 16 * f * m
 
 
+def survey_fracs(ref_b, cells, ms_v, ms_t):
+    """Each kernel's throughput priced at the reference scheme's bytes of that
+    kernel (velocity, stress), as a fraction of the 8 TB/s peak."""
+    return {"k_velocity": round(ref_b[0] * cells / (ms_v / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "k_stress": round(ref_b[1] * cells / (ms_t / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes": {"k_velocity": round(ref_b[0], 1), "k_stress": round(ref_b[1], 1)}}
+
+
+def serving_summary(shots, rate, seed):
+    """The default line's serving object: the first `shots` requests of the
+    shot-serving trace (Poisson arrivals, grids of 96^3-256^3, 500-1500 steps,
+    FP32 / comp16f) replayed against the server with its default configuration,
+    then as sequential FP32 shots (one at a time, first come first served) on the
+    same arrival times."""
+    n_vol, np_, h, dt = 512, 20, 5.0, 0.5e-3
+    spec = workload(n_vol, np_)
+    trace = serving_trace(shots, rate, seed, n_vol, np_)
+    srv, _ = run_serving(trace, n_vol, np_, h, dt, spec)
+    seq, _ = run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=True, max_active=1, round_steps=64,
+                         use_graphs=False, batched=False)
+    keys = ("shots_per_s", "gcell_steps_per_s", "p50_latency_s", "p99_latency_s", "mean_latency_s", "makespan_s",
+            "peak_active", "finite")
+    return {"trace": {"shots": shots, "rate_per_s": rate, "seed": seed, "grids": [96, 128, 192, 256],
+                      "steps": [500, 1500], "precisions": "fp32/comp16f 50/50"},
+            "server": {k: srv[k] for k in keys}, "sequential_fp32": {k: seq[k] for k in keys},
+            "throughput_ratio": round(srv["gcell_steps_per_s"] / seq["gcell_steps_per_s"], 3),
+            "p50_ratio": round(srv["p50_latency_s"] / seq["p50_latency_s"], 3),
+            "p99_ratio": round(srv["p99_latency_s"] / seq["p99_latency_s"], 3)}
+
+
 def rank_share_main(args):
     """One interior rank's share of the 512^3 workload split into z-slabs: the
     step kernels at that shape, n x n x planes owned planes plus one ghost
@@ -309,7 +344,7 @@ def rank_share_main(args):
     nz = planes + 2
     nt = args.warmup + args.steps + args.prof_steps
     stream = torch.cuda.Stream()
-    solver = pw.Solver(n, n, nz, np_, h, dt, nt, precision=args.precision, cpml_r=1e-4, cpml_v0=4000.0,
+    solver = pw.Solver(n, n, nz, np_, h, dt, nt, precision=args.precision, cpml_r=1e-4, cpml_v0=CPML_V0,
                        stream=stream.cuda_stream, cpml_open=("z_lo", "z_hi"))
     # the two-layer model with its interface mid-slab (z_top is a total-grid index)
     spec = workload(n, np_)
@@ -371,7 +406,7 @@ def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_
     untimed, on to total_steps for the accuracy comparison."""
     nt = max(warmup + steps + prof_steps, total_steps)
     stream = torch.cuda.Stream()
-    s = pw.Solver(n, n, n, np_, h, dt, nt, precision=precision, cpml_r=1e-4, cpml_v0=4000.0,
+    s = pw.Solver(n, n, n, np_, h, dt, nt, precision=precision, cpml_r=1e-4, cpml_v0=CPML_V0,
                   stream=stream.cuda_stream, space_order=order)
     s.set_medium_layered(spec)
     s.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
@@ -413,7 +448,7 @@ def side_run(pw, torch, precision, n, np_, h, dt, f0, spec, steps, warmup, prof_
 
 def fp64_outputs(pw, n, np_, h, dt, f0, spec, nt, order):
     """The same run in FP64 (untimed): the reference for every line's accuracy."""
-    s = pw.Solver(n, n, n, np_, h, dt, nt, precision="fp64", cpml_r=1e-4, cpml_v0=4000.0, space_order=order)
+    s = pw.Solver(n, n, n, np_, h, dt, nt, precision="fp64", cpml_r=1e-4, cpml_v0=CPML_V0, space_order=order)
     s.set_medium_layered(spec)
     s.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
     fields, idx = receivers(n, np_)
@@ -478,7 +513,7 @@ def host_job_main(args):
     n, np_, h, dt, f0 = args.n, args.np, 5.0, 0.5e-3, 25.0
     nt = args.job_steps
     spec = workload(n, np_)
-    s0 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
+    s0 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=CPML_V0)
     s0.set_medium_layered(spec)
     base = s0.base_fields()  # the caller's host buffers (not timed)
     s0.close()
@@ -491,7 +526,7 @@ def host_job_main(args):
     for prec in args.job_precisions.split(","):
         torch.cuda.synchronize()
         t = [time.perf_counter()]
-        s = pw.Solver(n, n, n, np_, h, dt, nt, precision=prec, cpml_r=1e-4, cpml_v0=4000.0)
+        s = pw.Solver(n, n, n, np_, h, dt, nt, precision=prec, cpml_r=1e-4, cpml_v0=CPML_V0)
         t.append(time.perf_counter())
         s.set_medium(base)
         t.append(time.perf_counter())
@@ -533,7 +568,7 @@ def medium_digest_main(args):
     rec_path = os.path.join(HERE, "tests", "golden", "medium_digests.json")
     recorded = json.load(open(rec_path)) if os.path.exists(rec_path) else {}
     for name, n, np_, h, dt, spec in todo:
-        s = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
+        s = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=CPML_V0)
         s.set_medium_layered(spec)
         base = s.base_fields()
         s.close()
@@ -564,11 +599,15 @@ def main():
                     help="every storage precision against FP64 on the same workload instead")
     ap.add_argument("--report-steps", type=int, default=2000)
     ap.add_argument("--shots", type=int, default=256)
+    ap.add_argument("--serving-shots", type=int, default=48,
+                    help="trace prefix replayed for the default line's serving object (0: none)")
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--round-steps", type=int, default=32)
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--fill-cells", type=int, default=None,
+                    help="server admission limit in grid cells (default: the library's)")
     ap.add_argument("--volume-every", type=int, default=0,
                     help="also take a full 3-D txx snapshot every K steps (asynchronous copy to the host)")
     ap.add_argument("--rank-share", action="store_true",
@@ -616,13 +655,13 @@ def main():
         n, np_, h, dt, f0 = args.n, args.np, 5.0, 0.5e-3, 25.0
         spec = workload(n, np_)
         name, medium = "biot_two_layer", "two-layer perturbed Biot medium generated on the GPU, seed 0"
-    # the default line also checks FP32 and comp16 against an FP64 run; those
-    # runs go on, untimed, to 1000 steps: the first arrival reaches the
-    # receiver line (226 cells from the source) near step 780
+    # the default line also checks FP32 and the compensated modes against an
+    # FP64 run; those runs go on, untimed, to 1000 steps: the first arrival
+    # reaches the receiver line (226 cells from the source) near step 780
     acc_steps = 1000 if (args.precision == "fp32" and not args.no_comp16 and not args.large) else 0
     nt = max(args.warmup + args.steps + args.prof_steps, acc_steps)
     stream = torch.cuda.Stream()
-    solver = pw.Solver(n, n, n, np_, h, dt, nt, precision=args.precision, cpml_r=1e-4, cpml_v0=4000.0,
+    solver = pw.Solver(n, n, n, np_, h, dt, nt, precision=args.precision, cpml_r=1e-4, cpml_v0=CPML_V0,
                        stream=stream.cuda_stream, space_order=args.space_order)
     solver.set_medium_layered(spec)
     solver.set_source(n // 2, n // 2, n // 2, f0, (1.0, 1.0, 1.0, -1.0))
@@ -672,8 +711,7 @@ def main():
     sec = ms_total / 1e3
     gcell = cells * args.steps / sec / 1e9
     step_bytes = info["bytes_per_cell_step"] * cells
-    # per step: the velocity (stress) kernel's time summed over its launches of
-    # that step (one launch per step, or one per z-slab with PW_SLAB)
+    # per step: each kernel's time over the profiled steps
     avg_v, avg_t = ms_v / launches, ms_t / launches
     # dominant kernel: the one with the larger total time
     if ms_v >= ms_t:
@@ -705,7 +743,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      # profiled for the 2nd-order step only
                      "traffic": measured_traffic(args.precision, kname, n) if args.space_order == 2 else None,
-                     "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 4)},
+                     "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 4),
+                     # each kernel priced at the reference scheme's per-kernel bytes
+                     "frac_survey": survey_fracs(ref_b, cells, avg_v, avg_t)},
         "step_hbm_gbs": round(step_bytes * args.steps / sec / 1e9, 1),
         "bytes_per_cell_step": round(info["bytes_per_cell_step"], 2),
         # the same throughput priced at the reference scheme's compulsory bytes
@@ -722,26 +762,38 @@ def main():
                                    "bytes_each": int(cells * 4)}
     solver.close()
     if args.precision == "fp32" and not args.no_comp16:
-        # the metric's comparison: compensated FP16 (the reference Half format) on the same workload
-        c16, c16_outputs = side_run(pw, torch, "comp16", n, np_, h, dt, f0, spec, args.steps, args.warmup,
-                                    args.prof_steps, order=args.space_order, total_steps=acc_steps)
-        c16["ratio_vs_fp32"] = round(c16["value"] / gcell, 4)
-        out["comp16"] = c16
-        if acc_steps and main_outputs[2] == nt and c16_outputs[2] == nt:
-            # both lines' accuracy against the FP64 run of the same steps
+        # the metric's comparison, compensated FP16 on the same workload: comp16f
+        # (the compensated state under an FP32 update; the accurate and fast mode)
+        # and comp16 (the reference Half format, bit-identical to the Half oracle)
+        side = {}
+        for prec in ("comp16f", "comp16"):
+            obj, outs = side_run(pw, torch, prec, n, np_, h, dt, f0, spec, args.steps, args.warmup, args.prof_steps,
+                                 order=args.space_order, total_steps=acc_steps)
+            obj["ratio_vs_fp32"] = round(obj["value"] / gcell, 4)
+            obj["roofline"]["frac_survey"] = survey_fracs(reference_scheme_bytes(prec, n, np_), cells,
+                                                          obj["kernel_ms"]["k_velocity"], obj["kernel_ms"]["k_stress"])
+            out[prec] = obj
+            side[prec] = outs
+        if acc_steps and main_outputs[2] == nt and all(o[2] == nt for o in side.values()):
+            # every line's accuracy against the FP64 run of the same steps
             ref = fp64_outputs(pw, n, np_, h, dt, f0, spec, nt, args.space_order)
             out["accuracy_vs_fp64"] = accuracy(main_outputs[:2], ref, fields)
-            c16["accuracy_vs_fp64"] = accuracy(c16_outputs[:2], ref, fields)
+            for prec, outs in side.items():
+                out[prec]["accuracy_vs_fp64"] = accuracy(outs[:2], ref, fields)
             del ref
+    if (args.precision == "fp32" and not args.large and args.space_order == 2 and args.n == 512
+            and args.serving_shots > 0):
+        # the metric's serving half, on the default line only
+        out["serving"] = serving_summary(args.serving_shots, args.rate, args.seed)
     if not args.no_cpu_baseline and not args.large:
         base = None
-        s2 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0,
+        s2 = pw.Solver(n, n, n, np_, h, dt, 2, precision="fp32", cpml_r=1e-4, cpml_v0=CPML_V0,
                        space_order=args.space_order)
         s2.set_medium_layered(spec)
         base = s2.base_fields()
         s2.close()
         out["cpu_baseline"] = cpu_baseline(n, np_, base, dt, h, args.cpu_steps, info["bytes_per_cell_step"],
-                                           order=args.space_order)
+                                           order=args.space_order, v0=info["vp_max"])
         del base
     print(json.dumps(out), flush=True)
     return 0

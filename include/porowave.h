@@ -339,6 +339,13 @@ typedef struct pw_server_config {
                                once as a hipGraph and replayed */
     int batched;            /* 1: one launch per step kernel for all active shots of a
                                precision; 0: each shot stepped by pw_run in turn */
+    long long fill_cells;   /* admission stops adding shots once the admitted shots hold
+                               this many grid cells (nx*ny*nz summed; a shot is always
+                               admitted when none is active).  Shots large enough to
+                               fill the GPU then run one or two at a time, first come
+                               first served, which keeps latency at the sequential
+                               level under load, while small shots still share the
+                               device.  0: no limit besides max_active and the budget */
 } pw_server_config;
 
 typedef struct pw_shot {
@@ -386,7 +393,8 @@ typedef struct pw_server_stats {
     int peak_active;           /* largest number of shots advanced together */
 } pw_server_stats;
 
-/* max_active 16, round_steps 32, graphs on, batched, budget 0 (automatic). */
+/* max_active 16, round_steps 32, graphs on, batched, budget 0 (automatic),
+ * fill_cells 2^25 (two 256^3 grids). */
 void pw_server_config_default(pw_server_config* cfg);
 pw_status pw_server_create(const pw_server_config* cfg, pw_server** out);
 /* Finishes the admitted shots, fails the queued ones, stops the worker. */

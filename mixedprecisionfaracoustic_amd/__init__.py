@@ -65,7 +65,8 @@ class Layered(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("stream", C.c_void_p), ("hbm_budget", C.c_longlong),
-                ("max_active", C.c_int), ("round_steps", C.c_int), ("use_graphs", C.c_int), ("batched", C.c_int)]
+                ("max_active", C.c_int), ("round_steps", C.c_int), ("use_graphs", C.c_int), ("batched", C.c_int),
+                ("fill_cells", C.c_longlong)]
 
 
 class ShotSpec(C.Structure):
@@ -463,13 +464,15 @@ class Server:
     with set_medium / set_medium_layered."""
 
     def __init__(self, device=0, max_active=16, round_steps=32, use_graphs=True, batched=True, hbm_budget=0,
-                 stream=None):
+                 stream=None, fill_cells=None):
         L = lib()
         cfg = ServerConfig()
         L.pw_server_config_default(C.byref(cfg))
         cfg.device, cfg.max_active, cfg.round_steps = int(device), int(max_active), int(round_steps)
         cfg.use_graphs, cfg.batched, cfg.hbm_budget = int(bool(use_graphs)), int(bool(batched)), int(hbm_budget)
         cfg.stream = stream
+        if fill_cells is not None:  # None keeps the library's default
+            cfg.fill_cells = int(fill_cells)
         h = C.c_void_p()
         _check(L.pw_server_create(C.byref(cfg), C.byref(h)))
         self._h = h

@@ -276,7 +276,7 @@ void plan_layout(const pw_config& c, Layout& L)
     // compensated fields: FP16 values, then the residual bytes at the same
     // offset (align256(2 * cells)) for every field
     for (int k = 0; k < PW_NFIELDS; ++k)
-        take(field_has_lo(prec, k) ? align256(cells * 2) + L.pad + cells : cells * (size_t)field_bytes(prec, k));
+        take(has_lo(prec) ? align256(cells * 2) + L.pad + cells : cells * sb);
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k < mem_slots(prec); ++k) take((size_t)g.slab_cells[ax] * mb);
     L.zero_bytes = L.total;
@@ -335,7 +335,7 @@ pw_status pw::solver_create(const pw_config* cfg, pw_solver** out)
     size_t i = 0;
     const size_t loff = align256((size_t)g.cells * 2) + L.pad;  // residual bytes after a field's FP16 values
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.f[k] = base + off[i++];
-    for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = field_has_lo(s->prec, k) ? (int8_t*)((char*)s->b.f[k] + loff) : nullptr;
+    for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = has_lo(s->prec) ? (int8_t*)((char*)s->b.f[k] + loff) : nullptr;
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k < kMemPerAxis; ++k) s->b.mem[ax][k] = k < mem_slots(s->prec) ? base + off[i++] : nullptr;
     for (int k = 0; k < kMaxCoef; ++k) s->b.coef[k] = coef_used(s->prec, k) ? base + off[i++] : nullptr;
@@ -453,10 +453,10 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
     m.nx = c.nx; m.ny = c.ny; m.nz = c.nz;
     m.dt = c.dt;
     double* d_res = nullptr;
-    PW_HIP_TRY(hipMalloc(&d_res, 4 * sizeof(double)));
-    double res[4] = {0, 0, 0, 0};
+    PW_HIP_TRY(hipMalloc(&d_res, kReduceSlots * sizeof(double)));
+    double res[kReduceSlots] = {0, 0, 0, 0};
     hipError_t e = medium_reduce(m, d_res, s->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(res, d_res, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(res, d_res, kReduceSlots * sizeof(double), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(d_res);
     if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_set_medium: reduction: ") + hipGetErrorString(e));
@@ -1000,16 +1000,10 @@ extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
     // physical form 4 (rho, rhof, C1, C2) + 5 (C, M, H, H-2mu, mu) cell-centred
     // arrays (comp16f: 4 + 4, no H)
     const int cv = is_scaled(s->prec) ? 15 : 4, ct = is_scaled(s->prec) ? 7 : (s->prec == PW_COMP16F ? 4 : 5);
-    double sv = 0.0, st2 = 0.0;
-    for (int k = 0; k < PW_NFIELDS; ++k) {
-        const bool vel = k <= PW_VWZ;  // velocities: updated by the velocity kernel, read by the stress kernel
-        sv += (vel ? 2.0 : 1.0) * field_bytes(s->prec, k);
-        st2 += (vel ? 1.0 : 2.0) * field_bytes(s->prec, k);
-    }
     // memory variables: each slot of a kernel read and written per damped axis
     const double mt = mem_slots(s->prec) * f * mem_bytes(s->prec);
-    out->bytes_velocity = sv + cv * coef_bytes(s->prec) + mt;
-    out->bytes_stress = st2 + ct * coef_bytes(s->prec) + mt;
+    out->bytes_velocity = 19.0 * state_bytes(s->prec) + cv * coef_bytes(s->prec) + mt;
+    out->bytes_stress = 20.0 * state_bytes(s->prec) + ct * coef_bytes(s->prec) + mt;
     out->bytes_per_cell_step = out->bytes_velocity + out->bytes_stress;
     out->vp_max = s->vp_max;
     out->courant = s->courant;

@@ -37,18 +37,11 @@ constexpr int kMaxCoef = 22;
 inline bool is_scaled(int prec) { return prec == PW_COMP16 || prec == PW_BF16; }
 // the compensated FP16 state: FP16 "hi" arrays plus int8 "lo" residual arrays
 inline bool has_lo(int prec) { return prec == PW_COMP16 || prec == PW_COMP16F; }
-// comp16f fields kept in FP32 (bit k = field k; a build-time mask, 0 by default)
-#ifndef PW_C16F_PLAIN
-#define PW_C16F_PLAIN 0
-#endif
-__host__ __device__ constexpr bool c16f_plain(int k) { return (PW_C16F_PLAIN >> k) & 1; }
-inline bool field_has_lo(int prec, int k) { return has_lo(prec) && !(prec == PW_COMP16F && c16f_plain(k)); }
 inline int ncoef(int prec) { return is_scaled(prec) ? (int)SC_N : (int)PC_N; }
 // comp16f forms H from H - 2mu and mu and holds no H array
 inline bool coef_used(int prec, int k) { return k < ncoef(prec) && !(prec == PW_COMP16F && k == PC_HH); }
 // Bytes of one stored wavefield value of field k / coefficient / memory variable.
 inline int state_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : has_lo(prec) ? 3 : 2; }
-inline int field_bytes(int prec, int k) { return has_lo(prec) && !field_has_lo(prec, k) ? 4 : state_bytes(prec); }
 inline int coef_bytes(int prec) { return prec == PW_FP64 ? 8 : (prec == PW_FP32 || prec == PW_COMP16F) ? 4 : 2; }
 // memory-variable slots in use per axis (the rest are null)
 inline int mem_slots(int prec) { return prec == PW_COMP16 ? 16 : 8; }
@@ -139,8 +132,11 @@ struct MediumArgs {
     double dt;
 };
 // res[0] = Kmax, res[1] = Rmax, res[2] = max fast-P velocity squared (Half/main.cu:745-791),
-// res[3] = max rhof / CC1 (the size of the relative fluid velocity against the solid's)
-hipError_t medium_reduce(const MediumArgs& m, double* d_res3, hipStream_t st);
+// res[3] = max rhof / CC1 (the size of the relative fluid velocity against the solid's).
+// d_res (kReduceSlots doubles) is cleared by a fill kernel first.
+constexpr int kReduceSlots = 4;
+hipError_t medium_reduce(const MediumArgs& m, double* d_res, hipStream_t st);
+
 // Writes the coefficient arrays of b (physical or scaled form).
 hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double scale_m, double inv_scale_m,
                                hipStream_t st);

@@ -243,27 +243,18 @@ struct PolC16 {
 // comp16f: the comp16 state (FP16 value + int8 residual, residual rounded to
 // nearest) under this build's FP32 update (kFormFast).  Stored values are
 // physical values times a power-of-two scale per field class; coefficients are
-// the cell-centred FP32 medium and memory variables FP32.  With
-// a bit of PW_C16F_PLAIN set, field k is stored as FP32 (a field whose lo is
-// null).  ld / st take and return physical values (side kernels); the
-// step kernels work in stored units through FA<PolC16F>.
+// the cell-centred FP32 medium and memory variables FP32.  ld / st take and
+// return physical values (side kernels); the step kernels work in stored units
+// through FA<PolC16F>.
 struct PolC16F {
     using R = float; using Coef = float; using Mem = float;
     static constexpr int form = kFormFast;
     static constexpr bool scaled = false;
     static constexpr bool pair_cpml = false;
     struct Field { f16* hi; int8_t* lo; unsigned z; float sc, isc; };
-    static __device__ __forceinline__ R ld(const Field& f, unsigned o)
-    {
-        if (!f.lo) return ((const float*)f.hi)[o] * f.isc;
-        return c16_decode(f.hi[o], f.lo[o]) * f.isc;
-    }
+    static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return c16_decode(f.hi[o], f.lo[o]) * f.isc; }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v)
     {
-        if (!f.lo) {
-            ((float*)f.hi)[o] = v * f.sc;
-            return;
-        }
         f16 h;
         int8_t l;
         c16_encode_rn(v * f.sc, h, l, f.z);
@@ -342,7 +333,7 @@ template <> __host__ PolB16::Field make_field<PolB16>(const DevBuffers& b, int k
 template <> __host__ PolC16F::Field make_field<PolC16F>(const DevBuffers& b, int k)
 {
     const float sc = k <= PW_VUZ ? b.sc_vel : (k <= PW_VWZ ? b.sc_vw : b.sc_stress);
-    return {(f16*)b.f[k], b.lo[k], 0u, sc, 1.0f / sc};  // lo null: an FP32 field
+    return {(f16*)b.f[k], b.lo[k], 0u, sc, 1.0f / sc};
 }
 
 template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
@@ -474,25 +465,17 @@ template <> struct FA<PolC16> {
     }
 };
 // comp16f, in stored units (the class scales are folded into the launch
-// constants); the fields of PW_C16F_PLAIN are FP32
+// constants): comp16's addressing with the round-to-nearest encode
 template <> struct FA<PolC16F> {
     using R = float;
     using P = PolC16F;
-    static __device__ __forceinline__ constexpr bool plain(int fid) { return c16f_plain(fid); }
     static __device__ __forceinline__ R ld(const StepArgs<P>& a, int fid, unsigned o)
     {
-        if (plain(fid)) return ((const float*)a.f[fid].hi)[o];
         const f16* h = a.f[fid].hi;
         return c16_decode(h[o], ((const int8_t*)h)[a.loff + o]);
     }
     template <int N> static __device__ __forceinline__ void ldv(const StepArgs<P>& a, int fid, unsigned o, R (&r)[N])
     {
-        if (plain(fid)) {
-            const Pack<float, N> t = ld_pack<N>((const float*)a.f[fid].hi, o);
-#pragma unroll
-            for (int j = 0; j < N; ++j) r[j] = t.v[j];
-            return;
-        }
         const f16* h = a.f[fid].hi;
         const Pack<f16, N> hv = ld_pack<N>(h, o);
         const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h, a.loff + o);
@@ -501,10 +484,6 @@ template <> struct FA<PolC16F> {
     }
     static __device__ __forceinline__ void st(const StepArgs<P>& a, int fid, unsigned o, R v)
     {
-        if (plain(fid)) {
-            ((float*)a.f[fid].hi)[o] = v;
-            return;
-        }
         f16* h = a.f[fid].hi;
         f16 hh;
         int8_t l;
@@ -515,13 +494,6 @@ template <> struct FA<PolC16F> {
     template <int N>
     static __device__ __forceinline__ void stv(const StepArgs<P>& a, int fid, unsigned o, const R (&r)[N])
     {
-        if (plain(fid)) {
-            Pack<float, N> t;
-#pragma unroll
-            for (int j = 0; j < N; ++j) t.v[j] = r[j];
-            st_pack<N>((float*)a.f[fid].hi, o, t);
-            return;
-        }
         f16* h = a.f[fid].hi;
         Pack<f16, N> hv;
         Pack<int8_t, N> lv;

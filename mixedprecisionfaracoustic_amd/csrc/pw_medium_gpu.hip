@@ -244,13 +244,20 @@ static BaseView make_view(const MediumArgs& m)
     return v;
 }
 
-hipError_t medium_reduce(const MediumArgs& m, double* d_res3, hipStream_t st)
+__global__ void k_fill_f64(double* p, int n, double v)
 {
-    hipError_t e = hipMemsetAsync(d_res3, 0, 4 * sizeof(double), st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_medium_reduce, dim3(2048), dim3(256), 0, st, make_view(m), m.nx, m.ny, m.nz, m.dt, d_res3);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t medium_reduce(const MediumArgs& m, double* d_res, hipStream_t st)
+{
+    // the maxima start at zero (a fill kernel, not a memset)
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, d_res, kReduceSlots, 0.0);
+    hipLaunchKernelGGL(k_medium_reduce, dim3(2048), dim3(256), 0, st, make_view(m), m.nx, m.ny, m.nz, m.dt, d_res);
     return hipGetLastError();
 }
+
 
 hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double scale_m, double inv_scale_m,
                                hipStream_t st)

@@ -318,6 +318,13 @@ void worker_main(pw_server* srv)
                 continue;
             }
             if (srv->used + q->bytes > srv->budget) break;
+            // grid cells already admitted: a shot that would take them past
+            // fill_cells waits for a slot (unless nothing is running)
+            if (srv->cfg.fill_cells > 0 && !srv->active.empty()) {
+                long long held = 0;
+                for (const Shot* a : srv->active) held += a->cells;
+                if (held + q->cells > srv->cfg.fill_cells) break;
+            }
             srv->pending.pop_front();
             q->state = PW_SHOT_RUNNING;
             q->t_admit = srv->clock();
@@ -466,14 +473,16 @@ extern "C" void pw_server_config_default(pw_server_config* c)
     c->round_steps = 32;
     c->use_graphs = 1;
     c->batched = 1;
+    c->fill_cells = 1LL << 25;
 }
 
 extern "C" pw_status pw_server_create(const pw_server_config* cfg, pw_server** out)
 {
     if (!cfg || !out) return fail(PW_ERR_INVALID, "pw_server_create: null argument");
     *out = nullptr;
-    if (cfg->max_active < 1 || cfg->round_steps < 1 || cfg->hbm_budget < 0)
-        return fail(PW_ERR_INVALID, "pw_server_create: max_active and round_steps must be >= 1, budget >= 0");
+    if (cfg->max_active < 1 || cfg->round_steps < 1 || cfg->hbm_budget < 0 || cfg->fill_cells < 0)
+        return fail(PW_ERR_INVALID,
+                    "pw_server_create: max_active and round_steps must be >= 1, budget and fill_cells >= 0");
     int ndev = 0;
     PW_HIP_TRY(hipGetDeviceCount(&ndev));
     if (cfg->device < 0 || cfg->device >= ndev) return fail(PW_ERR_INVALID, "pw_server_create: no such device");

@@ -46,7 +46,7 @@ def test_serving_trace_is_seeded_and_in_range():
         assert n in (96, 128, 192, 256)
         assert all(0 <= c and c + ni <= 512 - 40 for c in s["crop"])
         assert all(20 <= c < n - 20 for c in s["src"])
-        assert 500 <= s["nt"] <= 1500 and s["prec"] in ("fp32", "comp16")
+        assert 500 <= s["nt"] <= 1500 and s["prec"] in ("fp32", "comp16f")
         assert len(s["idx"]) == len(s["fields"]) == 128
         assert all(0 <= i < n ** 3 for i in s["idx"])
 
