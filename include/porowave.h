@@ -153,6 +153,21 @@ const char* pw_last_error(void);
  * wavefields to zero and the step counter to 0. */
 pw_status pw_set_medium(pw_solver* s, const double* const* base, int nxi, int nyi, int nzi);
 
+/* pw_set_medium for base fields already in device memory, allocating nothing.
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  d_base: PW_NBASE device pointers, on the handle's device,
+ * to the cell-centred interior fields (nxi*nyi*nzi doubles each, x fastest), as
+ * pw_set_medium takes them in host memory; the installed medium is
+ * bit-identical.  The medium's maxima (stability check, Half exponents) are
+ * reduced into a file-scope device array of 64 doubles instead of an
+ * allocation; the reduction needs 4 of them whatever the grid size.  Every call
+ * on a device shares that device's array and clears it first; calls in one
+ * process are serialised on it (calls that overlapped on one device would
+ * race).  Refuses a handle with cpml_v0 <= 0 (PW_ERR_UNSUPPORTED): its CPML
+ * tables are built from the medium on the host and uploaded. */
+pw_status pw_set_medium_device(pw_solver* s, const double* const* d_base, int nxi, int nyi, int nzi);
+
 /* Reference Ricker source (Float/main.cu:310-313, 321-322, 1059-1060): wavelet
  * -(1 - 2 pi^2 f0^2 tau^2) exp(-pi^2 f0^2 tau^2), tau = it*dt - t0, t0 = 1.2/f0,
  * added to txx, tyy, tzz and p at cell (sx, sy, sz) with the given weights

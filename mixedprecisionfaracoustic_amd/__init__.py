@@ -107,6 +107,7 @@ _SIGS = {
     "pw_destroy": (None, [C.c_void_p]),
     "pw_last_error": (C.c_char_p, []),
     "pw_set_medium": (C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.c_int, C.c_int, C.c_int]),
+    "pw_set_medium_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]),
     "pw_set_source": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]),
     "pw_set_receivers": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     "pw_set_receivers_reference": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
@@ -336,6 +337,18 @@ class Solver:
             raise ValueError(f"base must be [9, {nzi}, {nyi}, {nxi}], got {b.shape}")
         ptrs = (C.POINTER(C.c_double) * 9)(*[_dp(b[k]) for k in range(9)])
         _check(lib().pw_set_medium(self._h, ptrs, nxi, nyi, nzi))
+
+    def set_medium_device(self, ptrs):
+        """pw_set_medium_device: nine device addresses (ints) of the base fields on
+        the handle's device; allocates nothing.  The caller guarantees that each
+        address holds nzi*nyi*nxi FP64 values (this handle's interior, x fastest):
+        device buffers carry no size the library could check."""
+        nzi, nyi, nxi = self.interior
+        p = [int(x) for x in ptrs]
+        if len(p) != 9:
+            raise ValueError("set_medium_device takes 9 device addresses")
+        arr = (C.c_void_p * 9)(*p)
+        _check(lib().pw_set_medium_device(self._h, arr, nxi, nyi, nzi))
 
     def set_medium_layered(self, spec):
         _check(lib().pw_set_medium_layered(self._h, C.byref(spec)))

@@ -442,7 +442,7 @@ extern "C" void pw_destroy(pw_solver* s)
 }
 
 pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi, int nyi, int nzi,
-                             long long row_stride, long long plane_stride)
+                             long long row_stride, long long plane_stride, double* d_scratch)
 {
     const pw_config& c = s->cfg;
     MediumArgs m{};
@@ -452,13 +452,13 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
     m.nxi = nxi; m.nyi = nyi; m.nzi = nzi; m.np = c.np;
     m.nx = c.nx; m.ny = c.ny; m.nz = c.nz;
     m.dt = c.dt;
-    double* d_res = nullptr;
-    PW_HIP_TRY(hipMalloc(&d_res, kReduceSlots * sizeof(double)));
+    double* d_res = d_scratch;
+    if (!d_res) PW_HIP_TRY(hipMalloc(&d_res, kReduceSlots * sizeof(double)));
     double res[kReduceSlots] = {0, 0, 0, 0};
     hipError_t e = medium_reduce(m, d_res, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(res, d_res, kReduceSlots * sizeof(double), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    (void)hipFree(d_res);
+    if (!d_scratch) (void)hipFree(d_res);
     if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_set_medium: reduction: ") + hipGetErrorString(e));
     if (!(res[0] > 0) || !std::isfinite(res[0]) || !std::isfinite(res[1]) || !std::isfinite(res[2]))
         return fail(PW_ERR_INVALID, "pw_set_medium: medium has no positive finite modulus");
@@ -559,6 +559,30 @@ extern "C" pw_status pw_set_medium(pw_solver* s, const double* const* base, int 
     }
     st = install_medium(s, DenseBase(d_base, ni).p, nxi, nyi, nzi, 0, 0);
     (void)hipFree(d_base);
+    if (st == PW_OK) s->layered = false;
+    return st;
+}
+
+extern "C" pw_status pw_set_medium_device(pw_solver* s, const double* const* d_base, int nxi, int nyi, int nzi)
+{
+    pw_status st = check_handle(s, "pw_set_medium_device");
+    if (st != PW_OK) return st;
+    const pw_config& c = s->cfg;
+    if (!d_base) return fail(PW_ERR_INVALID, "pw_set_medium_device: null base");
+    if (nxi != c.nx - 2 * c.np || nyi != c.ny - 2 * c.np || nzi != c.nz - 2 * c.np)
+        return fail(PW_ERR_INVALID, "pw_set_medium_device: interior size must be (nx-2np, ny-2np, nz-2np)");
+    for (int k = 0; k < PW_NBASE; ++k)
+        if (!d_base[k]) return fail(PW_ERR_INVALID, "pw_set_medium_device: null base field");
+    if (!(c.cpml_v0 > 0))
+        return fail(PW_ERR_UNSUPPORTED,
+                    "pw_set_medium_device: cpml_v0 <= 0 builds the CPML tables from the medium on the host; "
+                    "give the handle a CPML velocity");
+    PW_HIP_TRY(hipSetDevice(c.device));
+    // the scratch array is shared by every call on this device
+    std::lock_guard<std::mutex> lk(scratch_mutex());
+    double* d_scratch = nullptr;
+    PW_HIP_TRY(scratch_for_device(c.device, &d_scratch));
+    st = install_medium(s, d_base, nxi, nyi, nzi, 0, 0, d_scratch);
     if (st == PW_OK) s->layered = false;
     return st;
 }

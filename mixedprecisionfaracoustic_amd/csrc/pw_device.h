@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "pw_internal.h"
 
@@ -136,7 +137,14 @@ struct MediumArgs {
 // d_res (kReduceSlots doubles) is cleared by a fill kernel first.
 constexpr int kReduceSlots = 4;
 hipError_t medium_reduce(const MediumArgs& m, double* d_res, hipStream_t st);
-
+// The scratch of the allocation-free medium installation (pw_set_medium_device):
+// a file-scope device array of kScratchDoubles doubles.  scratch_for_device
+// returns its address on device dev (taken once per device with that device
+// current, hipGetSymbolAddress); scratch_mutex serialises its users in this
+// process.
+constexpr int kScratchDoubles = 64;
+hipError_t scratch_for_device(int dev, double** out);
+std::mutex& scratch_mutex();
 // Writes the coefficient arrays of b (physical or scaled form).
 hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double scale_m, double inv_scale_m,
                                hipStream_t st);

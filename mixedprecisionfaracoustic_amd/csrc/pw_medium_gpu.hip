@@ -252,12 +252,41 @@ __global__ void k_fill_f64(double* p, int n, double v)
 
 hipError_t medium_reduce(const MediumArgs& m, double* d_res, hipStream_t st)
 {
-    // the maxima start at zero (a fill kernel, not a memset)
+    // the maxima start at zero (a fill kernel, not a memset: the same code serves
+    // the scratch array, which keeps the previous call's values)
     hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, d_res, kReduceSlots, 0.0);
     hipLaunchKernelGGL(k_medium_reduce, dim3(2048), dim3(256), 0, st, make_view(m), m.nx, m.ny, m.nz, m.dt, d_res);
     return hipGetLastError();
 }
 
+// Scratch of pw_set_medium_device: the medium reduction's maxima go here
+// instead of into an allocation.  Each device holds its own copy until the
+// process ends.  Host code never passes the name to a kernel: it takes the
+// array's device address once per device, with that device current.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static_assert(kReduceSlots <= kScratchDoubles, "the reduction must fit the scratch array");
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+std::mutex& scratch_mutex() { return g_scratch_mu; }
+
+// Caller holds scratch_mutex() and has made dev current.
+hipError_t scratch_for_device(int dev, double** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[dev]) {
+        double* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
 
 hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double scale_m, double inv_scale_m,
                                hipStream_t st)

@@ -70,8 +70,10 @@ pw_status solver_create(const pw_config* cfg, pw_solver** out);
 size_t solver_pool_bytes(const pw_config& c);
 // Installs a medium from device-resident FP64 base fields: field k of cell
 // (x, y, z) of the interior at d_base[k] + x + y*row_stride + z*plane_stride.
+// d_res: kReduceSlots doubles of device memory for the medium's maxima, or null
+// to allocate them for the call.
 pw_status install_medium(pw_solver* s, const double* const* d_base, int nxi, int nyi, int nzi, long long row_stride,
-                         long long plane_stride);
+                         long long plane_stride, double* d_res = nullptr);
 
 // What one step does besides the step kernels (Half/main.cu:1326-1377 for the
 // scaled forms): the dynamic rescale ratio applied first (0 = none), the source
