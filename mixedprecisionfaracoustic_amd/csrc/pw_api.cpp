@@ -233,7 +233,9 @@ struct Layout {
     std::vector<double> tab[3];
     int epml = 0;
     float isp = 1.0f;
-    std::vector<size_t> off;
+    std::vector<size_t> off;              // every array after the wavefields, in allocation order
+    size_t field_off[PW_NFIELDS] = {};    // a field's values (FP16 values for the compensated forms)
+    size_t field_lo[PW_NFIELDS] = {};     // its residual bytes (compensated forms)
     size_t zero_bytes = 0, total = 0;
     size_t pad = 0;   // bytes between consecutive arrays
 };
@@ -269,14 +271,33 @@ void plan_layout(const pw_config& c, Layout& L)
     // 512^3).  Measured on one GPU against no padding: 0.4-1.5 % faster at
     // 512^3 and 1024^3, and on a rank's share, in FP32 and comp16.
     L.pad = 8192;
-    auto take = [&](size_t bytes) {
-        L.off.push_back(L.total);
+    auto place = [&](size_t bytes) {
+        const size_t o = L.total;
         L.total += align256(bytes) + L.pad;
+        return o;
     };
+    auto take = [&](size_t bytes) { L.off.push_back(place(bytes)); };
     // compensated fields: FP16 values, then the residual bytes at the same
-    // offset (align256(2 * cells)) for every field
-    for (int k = 0; k < PW_NFIELDS; ++k)
-        take(has_lo(prec) ? align256(cells * 2) + L.pad + cells : cells * sb);
+    // offset (align256(2 * cells)) for every field; comp16f's pairs hold two
+    // fields' values and residuals interleaved, the residuals align256(4 * cells)
+    // after the values
+    for (int k = 0; k < PW_NFIELDS; ++k) {
+        if (prec == PW_COMP16F && c16f_pair(k) >= 0) {
+            if (c16f_slot(k) != 0) continue;  // placed with its slot-0 partner
+            const size_t o = place(align256(cells * 4) + L.pad + 2 * cells);
+            for (int q = 0; q < PW_NFIELDS; ++q)
+                if (c16f_pair(q) == c16f_pair(k)) {
+                    L.field_off[q] = o + 2 * c16f_slot(q);
+                    L.field_lo[q] = o + align256(cells * 4) + L.pad + c16f_slot(q);
+                }
+        } else if (has_lo(prec)) {
+            const size_t o = place(align256(cells * 2) + L.pad + cells);
+            L.field_off[k] = o;
+            L.field_lo[k] = o + align256(cells * 2) + L.pad;
+        } else {
+            L.field_off[k] = place(cells * sb);
+        }
+    }
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k < mem_slots(prec); ++k) take((size_t)g.slab_cells[ax] * mb);
     L.zero_bytes = L.total;
@@ -333,9 +354,8 @@ pw_status pw::solver_create(const pw_config* cfg, pw_solver** out)
     s->state_bytes_total = L.zero_bytes;
     char* base = (char*)s->pool;
     size_t i = 0;
-    const size_t loff = align256((size_t)g.cells * 2) + L.pad;  // residual bytes after a field's FP16 values
-    for (int k = 0; k < PW_NFIELDS; ++k) s->b.f[k] = base + off[i++];
-    for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = has_lo(s->prec) ? (int8_t*)((char*)s->b.f[k] + loff) : nullptr;
+    for (int k = 0; k < PW_NFIELDS; ++k) s->b.f[k] = base + L.field_off[k];
+    for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = has_lo(s->prec) ? (int8_t*)(base + L.field_lo[k]) : nullptr;
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k < kMemPerAxis; ++k) s->b.mem[ax][k] = k < mem_slots(s->prec) ? base + off[i++] : nullptr;
     for (int k = 0; k < kMaxCoef; ++k) s->b.coef[k] = coef_used(s->prec, k) ? base + off[i++] : nullptr;

@@ -41,6 +41,21 @@ inline bool has_lo(int prec) { return prec == PW_COMP16 || prec == PW_COMP16F; }
 inline int ncoef(int prec) { return is_scaled(prec) ? (int)SC_N : (int)PC_N; }
 // comp16f forms H from H - 2mu and mu and holds no H array
 inline bool coef_used(int prec, int k) { return k < ncoef(prec) && !(prec == PW_COMP16F && k == PC_HH); }
+// comp16f stores the fields that the step kernels read or write together at one
+// cell in pairs: a pair's FP16 values alternate in one array (slot 0, slot 1 of
+// each cell) and so do its residual bytes, so one 4-byte and one 2-byte access
+// per lane move both fields.  Pairs: (vux, vuy), (vuz, vwz), (vwx, vwy),
+// (txx, tyy), (txz, tyz), (tzz, p); txy is stored alone.
+__host__ __device__ constexpr int c16f_pair(int k)
+{
+    return k == PW_VUX || k == PW_VUY ? 0 : k == PW_VUZ || k == PW_VWZ ? 1 : k == PW_VWX || k == PW_VWY ? 2
+         : k == PW_TXX || k == PW_TYY ? 3 : k == PW_TXZ || k == PW_TYZ ? 4 : k == PW_TZZ || k == PW_P ? 5 : -1;
+}
+__host__ __device__ constexpr int c16f_slot(int k)
+{
+    return k == PW_VUY || k == PW_VWZ || k == PW_VWY || k == PW_TYY || k == PW_TYZ || k == PW_P ? 1 : 0;
+}
+
 // Bytes of one stored wavefield value of field k / coefficient / memory variable.
 inline int state_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : has_lo(prec) ? 3 : 2; }
 inline int coef_bytes(int prec) { return prec == PW_FP64 ? 8 : (prec == PW_FP32 || prec == PW_COMP16F) ? 4 : 2; }

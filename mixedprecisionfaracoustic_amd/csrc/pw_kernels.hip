@@ -251,15 +251,20 @@ struct PolC16F {
     static constexpr int form = kFormFast;
     static constexpr bool scaled = false;
     static constexpr bool pair_cpml = false;
-    struct Field { f16* hi; int8_t* lo; unsigned z; float sc, isc; };
-    static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return c16_decode(f.hi[o], f.lo[o]) * f.isc; }
+    // st: 2 for a field stored in a pair (its values and residuals alternate
+    // with the partner's), 1 for a field stored alone
+    struct Field { f16* hi; int8_t* lo; unsigned z; float sc, isc; unsigned st; };
+    static __device__ __forceinline__ R ld(const Field& f, unsigned o)
+    {
+        return c16_decode(f.hi[o * f.st], f.lo[o * f.st]) * f.isc;
+    }
     static __device__ __forceinline__ void st(const Field& f, unsigned o, R v)
     {
         f16 h;
         int8_t l;
         c16_encode_rn(v * f.sc, h, l, f.z);
-        f.hi[o] = h;
-        f.lo[o] = l;
+        f.hi[o * f.st] = h;
+        f.lo[o * f.st] = l;
     }
     static __device__ __forceinline__ R ldc(const Coef* c, unsigned o) { return c[o]; }
     template <int N> static __device__ __forceinline__ void ldcv(const Coef* c, unsigned o, R (&r)[N])
@@ -304,8 +309,9 @@ template <class P> struct StepArgs {
     int xcd_map;           // 1: XCD-aware block -> tile mapping
     typename P::R dt, h;
     // compensated fields: bytes from a field's FP16 values to its residual bytes
-    // (the same for every field)
-    unsigned loff;
+    // (the same for every field; comp16f: for the slot-0 field of every pair,
+    // and loff1 for txy, which is stored alone)
+    unsigned loff, loff1;
     // kFormFast launch constants (KF_*): dt/h, the factors of the staggered
     // means and the class storage scales folded together
     float kf[9];
@@ -333,7 +339,7 @@ template <> __host__ PolB16::Field make_field<PolB16>(const DevBuffers& b, int k
 template <> __host__ PolC16F::Field make_field<PolC16F>(const DevBuffers& b, int k)
 {
     const float sc = k <= PW_VUZ ? b.sc_vel : (k <= PW_VWZ ? b.sc_vw : b.sc_stress);
-    return {(f16*)b.f[k], b.lo[k], 0u, sc, 1.0f / sc};
+    return {(f16*)b.f[k], b.lo[k], 0u, sc, 1.0f / sc, c16f_pair(k) >= 0 ? 2u : 1u};
 }
 
 template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
@@ -367,6 +373,7 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     // the pool stores each compensated field's residual bytes right after its
     // FP16 values, at one offset for all fields
     a.loff = b.lo[0] ? (unsigned)((const char*)b.lo[0] - (const char*)b.f[0]) : 0u;
+    a.loff1 = b.lo[PW_TXY] ? (unsigned)((const char*)b.lo[PW_TXY] - (const char*)b.f[PW_TXY]) : 0u;
     const double g = b.dt / b.h, su = b.sc_vel, sw = b.sc_vw, stt = b.sc_stress;
     a.kf[KF_DT2] = (float)(0.5 * b.dt);
     a.kf[KF_WS] = (float)(4.0 * g * sw / stt);
@@ -465,22 +472,52 @@ template <> struct FA<PolC16> {
     }
 };
 // comp16f, in stored units (the class scales are folded into the launch
-// constants): comp16's addressing with the round-to-nearest encode
+// constants), with the round-to-nearest encode.  A paired field in slot q has
+// f[fid].hi = pair values + q (in FP16 units); the cell's values lie at 2o + q
+// and its residual byte a.loff - q + 2o bytes after f[fid].hi.  txy (alone)
+// has comp16's addressing with a.loff1.
 template <> struct FA<PolC16F> {
     using R = float;
     using P = PolC16F;
+    static __device__ __forceinline__ constexpr bool paired(int fid) { return c16f_pair(fid) >= 0; }
+    static __device__ __forceinline__ constexpr int slot(int fid) { return c16f_slot(fid); }
     static __device__ __forceinline__ R ld(const StepArgs<P>& a, int fid, unsigned o)
     {
         const f16* h = a.f[fid].hi;
-        return c16_decode(h[o], ((const int8_t*)h)[a.loff + o]);
+        if (paired(fid)) return c16_decode(h[2 * o], ((const int8_t*)h)[a.loff - slot(fid) + 2 * o]);
+        return c16_decode(h[o], ((const int8_t*)h)[a.loff1 + o]);
     }
     template <int N> static __device__ __forceinline__ void ldv(const StepArgs<P>& a, int fid, unsigned o, R (&r)[N])
     {
         const f16* h = a.f[fid].hi;
+        if (paired(fid)) {
+            // both fields' values of the N cells (one access each for values and
+            // residuals), this field's slot taken
+            const int q = slot(fid);
+            const Pack<f16, 2 * N> hv = ld_pack<2 * N>(h - q, 2 * o);
+            const Pack<int8_t, 2 * N> lv = ld_pack<2 * N>((const int8_t*)(h - q) + a.loff, 2 * o);
+#pragma unroll
+            for (int j = 0; j < N; ++j) r[j] = c16_decode(hv.v[2 * j + q], lv.v[2 * j + q]);
+            return;
+        }
         const Pack<f16, N> hv = ld_pack<N>(h, o);
-        const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h, a.loff + o);
+        const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h, a.loff1 + o);
 #pragma unroll
         for (int j = 0; j < N; ++j) r[j] = c16_decode(hv.v[j], lv.v[j]);
+    }
+    // fields fa (slot 0) and fb (slot 1) of one pair: one access each for values and residuals
+    template <int N>
+    static __device__ __forceinline__ void ld2v(const StepArgs<P>& a, int fa, int fb, unsigned o, R (&ra)[N], R (&rb)[N])
+    {
+        const f16* h = a.f[fa].hi;
+        const Pack<f16, 2 * N> hv = ld_pack<2 * N>(h, 2 * o);
+        const Pack<int8_t, 2 * N> lv = ld_pack<2 * N>((const int8_t*)h + a.loff, 2 * o);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            ra[j] = c16_decode(hv.v[2 * j], lv.v[2 * j]);
+            rb[j] = c16_decode(hv.v[2 * j + 1], lv.v[2 * j + 1]);
+        }
+        (void)fb;
     }
     static __device__ __forceinline__ void st(const StepArgs<P>& a, int fid, unsigned o, R v)
     {
@@ -488,21 +525,81 @@ template <> struct FA<PolC16F> {
         f16 hh;
         int8_t l;
         c16_encode_rn(v, hh, l, a.zero);
-        h[o] = hh;
-        ((int8_t*)h)[a.loff + o] = l;
+        if (paired(fid)) {
+            h[2 * o] = hh;
+            ((int8_t*)h)[a.loff - slot(fid) + 2 * o] = l;
+        } else {
+            h[o] = hh;
+            ((int8_t*)h)[a.loff1 + o] = l;
+        }
     }
     template <int N>
     static __device__ __forceinline__ void stv(const StepArgs<P>& a, int fid, unsigned o, const R (&r)[N])
     {
+        if (paired(fid)) {  // cell by cell: the partner's values lie in between
+#pragma unroll
+            for (int j = 0; j < N; ++j) st(a, fid, o + j, r[j]);
+            return;
+        }
         f16* h = a.f[fid].hi;
         Pack<f16, N> hv;
         Pack<int8_t, N> lv;
 #pragma unroll
         for (int j = 0; j < N; ++j) c16_encode_rn(r[j], hv.v[j], lv.v[j], a.zero);
         st_pack<N>(h, o, hv);
-        st_pack<N>((int8_t*)h, a.loff + o, lv);
+        st_pack<N>((int8_t*)h, a.loff1 + o, lv);
+    }
+    template <int N>
+    static __device__ __forceinline__ void st2v(const StepArgs<P>& a, int fa, int fb, unsigned o, const R (&ra)[N],
+                                                const R (&rb)[N])
+    {
+        f16* h = a.f[fa].hi;
+        Pack<f16, 2 * N> hv;
+        Pack<int8_t, 2 * N> lv;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            c16_encode_rn(ra[j], hv.v[2 * j], lv.v[2 * j], a.zero);
+            c16_encode_rn(rb[j], hv.v[2 * j + 1], lv.v[2 * j + 1], a.zero);
+        }
+        st_pack<2 * N>(h, 2 * o, hv);
+        st_pack<2 * N>((int8_t*)h + a.loff, 2 * o, lv);
+        (void)fb;
     }
 };
+
+// Two fields at the same cells: one paired access where the policy stores the
+// two as a pair (fa in slot 0, fb in slot 1), two accesses otherwise.
+template <class P> __device__ __forceinline__ constexpr bool is_pair(int fa, int fb)
+{
+    return std::is_same<P, PolC16F>::value && c16f_pair(fa) >= 0 && c16f_pair(fa) == c16f_pair(fb) &&
+           c16f_slot(fa) == 0 && c16f_slot(fb) == 1;
+}
+template <class P, int N>
+__device__ __forceinline__ void ld2v(const StepArgs<P>& a, int fa, int fb, unsigned o, typename P::R (&ra)[N],
+                                     typename P::R (&rb)[N])
+{
+    if constexpr (std::is_same<P, PolC16F>::value) {
+        if (is_pair<P>(fa, fb)) {
+            FA<P>::template ld2v<N>(a, fa, fb, o, ra, rb);
+            return;
+        }
+    }
+    FA<P>::template ldv<N>(a, fa, o, ra);
+    FA<P>::template ldv<N>(a, fb, o, rb);
+}
+template <class P, int N>
+__device__ __forceinline__ void st2v(const StepArgs<P>& a, int fa, int fb, unsigned o, const typename P::R (&ra)[N],
+                                     const typename P::R (&rb)[N])
+{
+    if constexpr (std::is_same<P, PolC16F>::value) {
+        if (is_pair<P>(fa, fb)) {
+            FA<P>::template st2v<N>(a, fa, fb, o, ra, rb);
+            return;
+        }
+    }
+    FA<P>::template stv<N>(a, fa, o, ra);
+    FA<P>::template stv<N>(a, fb, o, rb);
+}
 
 // Slab-local index along one axis, -1 outside the CPML slabs.
 __device__ __forceinline__ int slab_index(int i, int n, int lo, int hi)
@@ -809,6 +906,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
     // lanes past the row end (wide segments only) read the row's last cells; their
     // values are never used
     auto LD = [&](int fid, unsigned off, R (&r)[N]) { FA<P>::template ldv<N>(a, fid, off - L.sh, r); };
+    auto LD2 = [&](int fa, int fb, unsigned off, R (&ra)[N], R (&rb)[N]) { ld2v<P, N>(a, fa, fb, off - L.sh, ra, rb); };
     auto LDC = [&](int cid, unsigned off, R (&r)[N]) { P::template ldcv<N>(a.c[cid], off - L.sh, r); };
     auto ST = [&](int fid, unsigned off, const R (&r)[N]) {
         if (all) FA<P>::template stv<N>(a, fid, off, r);
@@ -816,6 +914,17 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
 #pragma unroll
             for (int j = 0; j < N; ++j)
                 if (upd[j]) FA<P>::st(a, fid, off + j, r[j]);
+        }
+    };
+    auto ST2 = [&](int fa, int fb, unsigned off, const R (&ra)[N], const R (&rb)[N]) {
+        if (all) st2v<P, N>(a, fa, fb, off, ra, rb);
+        else {
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                if (upd[j]) {
+                    FA<P>::st(a, fa, off + j, ra[j]);
+                    FA<P>::st(a, fb, off + j, rb[j]);
+                }
         }
     };
 
@@ -831,10 +940,8 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
     // z-2 and z; tzz, p at z-1 and z+1.
     R txz_m[N], tyz_m[N], tzz_c[N], p_c[N];
     R txz_mm[N], tyz_mm[N], txz_k[N], tyz_k[N], tzz_m[N], p_m[N], tzz_k[N], p_k[N];
-    LD(PW_TXZ, o - PL, txz_m);
-    LD(PW_TYZ, o - PL, tyz_m);
-    LD(PW_TZZ, o, tzz_c);
-    LD(PW_P, o, p_c);
+    LD2(PW_TXZ, PW_TYZ, o - PL, txz_m, tyz_m);
+    LD2(PW_TZZ, PW_P, o, tzz_c, p_c);
     if constexpr (ORD == 4) {
         LD(PW_TXZ, o - 2 * PL, txz_mm);
         LD(PW_TYZ, o - 2 * PL, tyz_mm);
@@ -862,13 +969,10 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         if constexpr (ORD == 2) {
             R txx_c[N], txx_r[N], tyy_c[N], tyy_q[N], txy_c[N], txy_l[N], txy_h[N], txz_l[N], tyz_h[N], p_r[N],
                 p_q[N];
-            LD(PW_TXZ, o, txz_c);
-            LD(PW_TYZ, o, tyz_c);
-            LD(PW_TZZ, o + PL, tzz_n);
-            LD(PW_P, o + PL, p_n);
-            LD(PW_TXX, o, txx_c);
+            LD2(PW_TXZ, PW_TYZ, o, txz_c, tyz_c);
+            LD2(PW_TZZ, PW_P, o + PL, tzz_n, p_n);
+            LD2(PW_TXX, PW_TYY, o, txx_c, tyy_c);
             seg_right(L, txx_c, txx_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_TXX, q); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
-            LD(PW_TYY, o, tyy_c);
             LD(PW_TYY, o + W, tyy_q);
             LD(PW_TXY, o, txy_c);
             seg_left(L, txy_c, txy_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXY, q); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
@@ -938,12 +1042,9 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         // per plane.  (Hoisting the coefficients as well costs a wave per SIMD in
         // registers and measured slower.)
         R wx0[N], wy0[N], wz0[N], ux0[N], uy0[N], uz0[N];
-        LD(PW_VWX, o, wx0);
-        LD(PW_VWY, o, wy0);
-        LD(PW_VWZ, o, wz0);
-        LD(PW_VUX, o, ux0);
-        LD(PW_VUY, o, uy0);
-        LD(PW_VUZ, o, uz0);
+        LD2(PW_VWX, PW_VWY, o, wx0, wy0);
+        LD2(PW_VUZ, PW_VWZ, o, uz0, wz0);
+        LD2(PW_VUX, PW_VUY, o, ux0, uy0);
         const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
         Tab4<R> tz = {0, 1, 0, 1};
         if (zs >= 0) tz = a.tab[2][iz];
@@ -1072,8 +1173,9 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
                 uz1[j] = uz0[j] + ri[j] * sz[j] - fr[j] * (wz1[j] - wz0[j]);
             }
         }
-        ST(PW_VWX, o, wx1); ST(PW_VWY, o, wy1); ST(PW_VWZ, o, wz1);
-        ST(PW_VUX, o, ux1); ST(PW_VUY, o, uy1); ST(PW_VUZ, o, uz1);
+        ST2(PW_VWX, PW_VWY, o, wx1, wy1);
+        ST2(PW_VUZ, PW_VWZ, o, uz1, wz1);
+        ST2(PW_VUX, PW_VUY, o, ux1, uy1);
 
         if constexpr (ORD == 2) {
 #pragma unroll
@@ -1119,6 +1221,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     // lanes past the row end (wide segments only) read the row's last cells; their
     // values are never used
     auto LD = [&](int fid, unsigned off, R (&r)[N]) { FA<P>::template ldv<N>(a, fid, off - L.sh, r); };
+    auto LD2 = [&](int fa, int fb, unsigned off, R (&ra)[N], R (&rb)[N]) { ld2v<P, N>(a, fa, fb, off - L.sh, ra, rb); };
     auto LDC = [&](int cid, unsigned off, R (&r)[N]) { P::template ldcv<N>(a.c[cid], off - L.sh, r); };
     auto ST = [&](int fid, unsigned off, const R (&r)[N]) {
         if (all) FA<P>::template stv<N>(a, fid, off, r);
@@ -1126,6 +1229,17 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
 #pragma unroll
             for (int j = 0; j < N; ++j)
                 if (upd[j]) FA<P>::st(a, fid, off + j, r[j]);
+        }
+    };
+    auto ST2 = [&](int fa, int fb, unsigned off, const R (&ra)[N], const R (&rb)[N]) {
+        if (all) st2v<P, N>(a, fa, fb, off, ra, rb);
+        else {
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                if (upd[j]) {
+                    FA<P>::st(a, fa, off + j, ra[j]);
+                    FA<P>::st(a, fb, off + j, rb[j]);
+                }
         }
     };
 
@@ -1141,10 +1255,8 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     // at z-2 and z; vux, vuy at z-1 and z+1.
     R vuz_m[N], vwz_m[N], vux_c[N], vuy_c[N];
     R vuz_mm[N], vwz_mm[N], vuz_k[N], vwz_k[N], vux_m[N], vuy_m[N], vux_k[N], vuy_k[N];
-    LD(PW_VUZ, o - PL, vuz_m);
-    LD(PW_VWZ, o - PL, vwz_m);
-    LD(PW_VUX, o, vux_c);
-    LD(PW_VUY, o, vuy_c);
+    LD2(PW_VUZ, PW_VWZ, o - PL, vuz_m, vwz_m);
+    LD2(PW_VUX, PW_VUY, o, vux_c, vuy_c);
     if constexpr (ORD == 4) {
         LD(PW_VUZ, o - 2 * PL, vuz_mm);
         LD(PW_VWZ, o - 2 * PL, vwz_mm);
@@ -1185,19 +1297,16 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         R vux_n[N], vuy_n[N], vuz_c[N], vwz_c[N];
         if constexpr (ORD == 2) {
             R vux_l[N], vux_q[N], vuy_h[N], vuy_r[N], vuz_r[N], vuz_q[N], vwx_c[N], vwx_l[N], vwy_c[N], vwy_h[N];
-            LD(PW_VUX, o + PL, vux_n);
-            LD(PW_VUY, o + PL, vuy_n);
-            LD(PW_VUZ, o, vuz_c);
-            LD(PW_VWZ, o, vwz_c);
+            LD2(PW_VUX, PW_VUY, o + PL, vux_n, vuy_n);
+            LD2(PW_VUZ, PW_VWZ, o, vuz_c, vwz_c);
             seg_left(L, vux_c, vux_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VUX, q); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
             LD(PW_VUX, o + W, vux_q);
             LD(PW_VUY, o - W, vuy_h);
             seg_right(L, vuy_c, vuy_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUY, q); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
             seg_right(L, vuz_c, vuz_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUZ, q); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
             LD(PW_VUZ, o + W, vuz_q);
-            LD(PW_VWX, o, vwx_c);
+            LD2(PW_VWX, PW_VWY, o, vwx_c, vwy_c);
             seg_left(L, vwx_c, vwx_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VWX, q); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
-            LD(PW_VWY, o, vwy_c);
             LD(PW_VWY, o - W, vwy_h);
 #pragma unroll
             for (int j = 0; j < N; ++j) {
@@ -1253,13 +1362,10 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         if (zs >= 0) tz = a.tab[2][iz];
 
         R p0[N], xx0[N], yy0[N], zz0[N], xy0[N], yz0[N], xz0[N];
-        LD(PW_P, o, p0);
-        LD(PW_TXX, o, xx0);
-        LD(PW_TYY, o, yy0);
-        LD(PW_TZZ, o, zz0);
+        LD2(PW_TZZ, PW_P, o, zz0, p0);
+        LD2(PW_TXX, PW_TYY, o, xx0, yy0);
         LD(PW_TXY, o, xy0);
-        LD(PW_TYZ, o, yz0);
-        LD(PW_TXZ, o, xz0);
+        LD2(PW_TXZ, PW_TYZ, o, xz0, yz0);
         R p1[N], xx1[N], yy1[N], zz1[N], xy1[N], yz1[N], xz1[N];
         R mu_n[N], mu_xn[N], mu_yn[N], mu_xy[N], mv_n[N];
         R C_[N], M_[N], HH_[N], H2u_[N], mxy[N], myz[N], mxz[N];
@@ -1356,9 +1462,10 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                 xz1[j] = xz0[j] + mxz[j] * (uxz + uzx);
             }
         }
-        ST(PW_P, o, p1);
-        ST(PW_TXX, o, xx1); ST(PW_TYY, o, yy1); ST(PW_TZZ, o, zz1);
-        ST(PW_TXY, o, xy1); ST(PW_TYZ, o, yz1); ST(PW_TXZ, o, xz1);
+        ST2(PW_TZZ, PW_P, o, zz1, p1);
+        ST2(PW_TXX, PW_TYY, o, xx1, yy1);
+        ST(PW_TXY, o, xy1);
+        ST2(PW_TXZ, PW_TYZ, o, xz1, yz1);
 
         if constexpr (ORD == 2) {
 #pragma unroll
