@@ -279,6 +279,21 @@ pw_status pw_get_info(pw_solver* s, pw_info* out);
 pw_status pw_set_profiling(pw_solver* s, int enable);
 pw_status pw_get_kernel_times(pw_solver* s, double* ms_velocity, double* ms_stress, long long* launches);
 
+/* The compensated FP16 codec of the comp16 and comp16f step kernels, run alone
+ * on the GPU so that it can be checked value by value.  No counterpart in the
+ * reference (its codec, Half/main.cu:49-100, is inline device code).  All
+ * pointers are device memory of n elements on the current device; stream is a
+ * hipStream_t (NULL: the null stream); both calls wait for the stream.
+ * pw_codec_decode: out[i] = hi[i] + lo[i] * u(hi[i]), hi FP16 bit patterns, u
+ *   the residual unit 2^(max(e,1) - 33) for hi's biased exponent e.
+ * pw_codec_encode: hi[i] = in[i] rounded to FP16 (nearest, ties to even), lo[i]
+ *   = (in[i] - hi[i]) / u(hi[i]) rounded as the precision's kernels round it and
+ *   clamped to [-127, 127]: PW_COMP16 half away from zero (the reference's
+ *   store_ch), PW_COMP16F to nearest, ties to even. */
+pw_status pw_codec_decode(const unsigned short* hi, const signed char* lo, float* out, long long n, void* stream);
+pw_status pw_codec_encode(const float* in, unsigned short* hi, signed char* lo, long long n, int precision,
+                          void* stream);
+
 /* ---------------------------------------------------------------------------
  * Medium front-ends (host, FP64).  Each writes PW_NBASE arrays of nxi*nyi*nzi
  * doubles into out (contiguous, field-major).

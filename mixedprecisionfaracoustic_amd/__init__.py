@@ -130,6 +130,8 @@ _SIGS = {
     "pw_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "pw_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                       C.POINTER(C.c_longlong)]),
+    "pw_codec_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p]),
+    "pw_codec_encode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_void_p]),
     "pw_write_bin": (C.c_int, [C.c_char_p, C.POINTER(BinHeader), C.POINTER(C.c_float)]),
     "pw_read_bin": (C.c_int, [C.c_char_p, C.POINTER(BinHeader), C.POINTER(C.c_float)]),
     "pw_write_outputs": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
@@ -257,6 +259,19 @@ def read_bin(path):
     head["magic"] = head["magic"].decode()
     head["label"] = head["label"].decode()
     return head, out
+
+
+def codec_decode(hi_ptr, lo_ptr, out_ptr, n, stream=None):
+    """pw_codec_decode on device addresses (ints): n FP16 bit patterns and int8
+    residuals to n floats, with the step kernels' decode."""
+    _check(lib().pw_codec_decode(hi_ptr, lo_ptr, out_ptr, int(n), stream))
+
+
+def codec_encode(in_ptr, hi_ptr, lo_ptr, n, precision, stream=None):
+    """pw_codec_encode on device addresses (ints): n floats to FP16 bit patterns
+    and int8 residuals, rounded as the precision's ("comp16" or "comp16f") step
+    kernels round them."""
+    _check(lib().pw_codec_encode(in_ptr, hi_ptr, lo_ptr, int(n), PRECISIONS[precision], stream))
 
 
 class Solver:

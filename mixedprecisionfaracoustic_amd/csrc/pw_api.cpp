@@ -1059,6 +1059,30 @@ extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
     return PW_OK;
 }
 
+extern "C" pw_status pw_codec_decode(const unsigned short* hi, const signed char* lo, float* out, long long n,
+                                     void* stream)
+{
+    if (n < 0) return fail(PW_ERR_INVALID, "pw_codec_decode: n < 0");
+    if (n > 0 && (!hi || !lo || !out)) return fail(PW_ERR_INVALID, "pw_codec_decode: null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    PW_HIP_TRY(launch_codec_decode(hi, (const int8_t*)lo, out, (unsigned long long)n, st));
+    PW_HIP_TRY(hipStreamSynchronize(st));
+    return PW_OK;
+}
+
+extern "C" pw_status pw_codec_encode(const float* in, unsigned short* hi, signed char* lo, long long n, int precision,
+                                     void* stream)
+{
+    if (n < 0) return fail(PW_ERR_INVALID, "pw_codec_encode: n < 0");
+    if (precision != PW_COMP16 && precision != PW_COMP16F)
+        return fail(PW_ERR_INVALID, "pw_codec_encode: precision must be PW_COMP16 or PW_COMP16F");
+    if (n > 0 && (!in || !hi || !lo)) return fail(PW_ERR_INVALID, "pw_codec_encode: null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    PW_HIP_TRY(launch_codec_encode(in, hi, (int8_t*)lo, (unsigned long long)n, precision == PW_COMP16F, st));
+    PW_HIP_TRY(hipStreamSynchronize(st));
+    return PW_OK;
+}
+
 extern "C" pw_status pw_set_profiling(pw_solver* s, int enable)
 {
     pw_status st = check_handle(s, "pw_set_profiling");

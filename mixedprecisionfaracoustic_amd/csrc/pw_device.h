@@ -95,6 +95,10 @@ hipError_t launch_snapshot(const DevBuffers& b, hipStream_t st, int field, int y
 hipError_t launch_decode(const DevBuffers& b, hipStream_t st, int field, float* dst, float scale);
 hipError_t launch_rescale(const DevBuffers& b, hipStream_t st, float ratio);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);
+hipError_t launch_codec_decode(const unsigned short* hi, const int8_t* lo, float* out, unsigned long long n,
+                               hipStream_t st);
+hipError_t launch_codec_encode(const float* in, unsigned short* hi, int8_t* lo, unsigned long long n, bool rne,
+                               hipStream_t st);
 
 // ---- multi-shot step (shot server) ----
 // One member of a batch: the solver buffers of one shot plus its per-step

@@ -782,11 +782,13 @@ __device__ __forceinline__ void seg_right(const Lanes& L, const R (&c)[N], R (&r
             row(r);
         }
     } else {
-        R s = __shfl_down(c[0], 1, 64);
-        if (L.lane == 63) s = L.edge_r ? edge(on) : (R)0;
+        // lane 63's neighbour comes from one wave-uniform load (or is 0 past the row end)
+        const bool er = __builtin_amdgcn_readlane((int)L.edge_r, 63) != 0;
+        const R e = er ? edge(__builtin_amdgcn_readlane(on, 63)) : (R)0;
+        const R s = lane_move<kDppWaveShl1>(c[0]);
 #pragma unroll
         for (int j = 0; j < N - 1; ++j) r[j] = c[j + 1];
-        r[N - 1] = s;
+        r[N - 1] = L.lane == 63 ? e : s;
     }
 }
 // r[j] = value at x-1 of element j; on = the lane's offset of its first
@@ -804,9 +806,10 @@ __device__ __forceinline__ void seg_left(const Lanes& L, const R (&c)[N], R (&r)
             row(r);
         }
     } else {
-        R s = __shfl_up(c[N - 1], 1, 64);
-        if (L.lane == 0) s = L.edge_l ? edge(on) : (R)0;
-        r[0] = s;
+        const bool el = __builtin_amdgcn_readlane((int)L.edge_l, 0) != 0;
+        const R e = el ? edge(__builtin_amdgcn_readlane(on, 0)) : (R)0;
+        const R s = lane_move<kDppWaveShr1>(c[N - 1]);
+        r[0] = L.lane == 0 ? e : s;
 #pragma unroll
         for (int j = 1; j < N; ++j) r[j] = c[j - 1];
     }
@@ -1884,6 +1887,48 @@ hipError_t launch_rescale(const DevBuffers& b, hipStream_t st, float ratio)
             (unsigned long long)b.g.slab_cells[2], ratio);
         return hipGetLastError();
     });
+}
+
+// The codec alone, element by element (grid-stride), with the device
+// functions the step kernels inline.
+__global__ void k_codec_decode(const unsigned short* __restrict__ hi, const int8_t* __restrict__ lo,
+                               float* __restrict__ out, unsigned long long n)
+{
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = c16_decode(__builtin_bit_cast(f16, hi[i]), lo[i]);
+}
+__global__ void k_codec_encode(const float* __restrict__ in, unsigned short* __restrict__ hi, int8_t* __restrict__ lo,
+                               unsigned long long n, int rne)
+{
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        f16 h;
+        int8_t l;
+        if (rne) c16_encode_rn(in[i], h, l, 0u);
+        else c16_encode(in[i], h, l, 0u);
+        hi[i] = __builtin_bit_cast(unsigned short, h);
+        lo[i] = l;
+    }
+}
+static unsigned codec_blocks(unsigned long long n)
+{
+    const unsigned long long b = (n + 255) / 256;
+    return (unsigned)(b > 8192 ? 8192 : b);
+}
+hipError_t launch_codec_decode(const unsigned short* hi, const int8_t* lo, float* out, unsigned long long n,
+                               hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_codec_decode, dim3(codec_blocks(n)), dim3(256), 0, st, hi, lo, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_codec_encode(const float* in, unsigned short* hi, int8_t* lo, unsigned long long n, bool rne,
+                               hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_codec_encode, dim3(codec_blocks(n)), dim3(256), 0, st, in, hi, lo, n, rne ? 1 : 0);
+    return hipGetLastError();
 }
 
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st)

@@ -546,6 +546,20 @@ float or_load_ch(unsigned short hi, signed char lo)
     return hf + (float)lo * or_residual_unit(hf);
 }
 
+/* or_load_ch / or_store_ch over arrays (codec tests) */
+void or_load_ch_n(const unsigned short* hi, const signed char* lo, long n, float* out)
+{
+    long i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; i++) out[i] = or_load_ch(hi[i], lo[i]);
+}
+void or_store_ch_n(const float* v, long n, unsigned short* hi, signed char* lo)
+{
+    long i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; i++) or_store_ch(v[i], hi + i, lo + i);
+}
+
 static inline float ldch(const unsigned short* hi, const signed char* lo, long o)
 {
     const float hf = H2F(hi[o]);

@@ -122,6 +122,9 @@ float or_f16_to_f32(unsigned short h);
 float or_residual_unit(float hi_float);
 void or_store_ch(float v, unsigned short* hi, signed char* lo);
 float or_load_ch(unsigned short hi, signed char lo);
+/* the two over arrays of n elements */
+void or_load_ch_n(const unsigned short* hi, const signed char* lo, long n, float* out);
+void or_store_ch_n(const float* v, long n, unsigned short* hi, signed char* lo);
 
 /* Half exponent + coefficient precompute (Half/main.cu:745-1045) for a
  * homogeneous-by-cell medium: returns em, epml and writes the 22 FP16 coefficient

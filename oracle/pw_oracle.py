@@ -78,6 +78,10 @@ def lib():
         L.or_store_ch.restype = None
         L.or_load_ch.argtypes = [C.c_ushort, C.c_byte]
         L.or_load_ch.restype = C.c_float
+        L.or_load_ch_n.argtypes = [C.POINTER(C.c_ushort), C.POINTER(C.c_byte), C.c_long, fp]
+        L.or_load_ch_n.restype = None
+        L.or_store_ch_n.argtypes = [fp, C.c_long, C.POINTER(C.c_ushort), C.POINTER(C.c_byte)]
+        L.or_store_ch_n.restype = None
         L.or_half_precompute.argtypes = [C.POINTER(OrArgs), fp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(C.c_ushort)]
         L.or_half_precompute.restype = C.c_int
@@ -87,6 +91,24 @@ def lib():
 
 def _ptr(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
+
+
+def load_ch(hi, lo):
+    """load_cr_value (Half/main.cu:69-73) over arrays: FP16 bit patterns and int8 residuals to float32."""
+    hi = np.ascontiguousarray(hi, np.uint16)
+    lo = np.ascontiguousarray(lo, np.int8)
+    out = np.empty(hi.shape, np.float32)
+    lib().or_load_ch_n(_ptr(hi, C.c_ushort), _ptr(lo, C.c_byte), hi.size, _ptr(out, C.c_float))
+    return out
+
+
+def store_ch(v):
+    """store_ch (Half/main.cu:85-100) over an array: float32 values to (FP16 bit patterns, int8 residuals)."""
+    v = np.ascontiguousarray(v, np.float32)
+    hi = np.empty(v.shape, np.uint16)
+    lo = np.empty(v.shape, np.int8)
+    lib().or_store_ch_n(_ptr(v, C.c_float), v.size, _ptr(hi, C.c_ushort), _ptr(lo, C.c_byte))
+    return hi, lo
 
 
 def borehole_model(nzi, nyi, nxi):

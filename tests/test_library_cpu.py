@@ -185,3 +185,13 @@ def test_cli_usage_and_argument_errors():
     if not torch.cuda.is_available():
         r = subprocess.run([cli, "--nt", "2", "--quiet"], capture_output=True, text=True, timeout=120)
         assert r.returncode == 1 and "pw_create" in r.stderr
+
+
+def test_codec_entry_points_refuse_bad_arguments():
+    """pw_codec_decode / pw_codec_encode check their arguments before any device call."""
+    for call in (lambda: pw.codec_decode(0, 0, 0, -1), lambda: pw.codec_decode(0, 0, 0, 8),
+                 lambda: pw.codec_encode(0, 0, 0, -1, "comp16"), lambda: pw.codec_encode(0, 0, 0, 8, "comp16f"),
+                 lambda: pw.codec_encode(0, 0, 0, 8, "bf16")):
+        with pytest.raises(pw.PorowaveError) as ei:
+            call()
+        assert ei.value.status == 1

@@ -67,3 +67,23 @@ def test_compensated_roundtrip_bound():
             assert abs(dec - float(v)) <= 0.5 * unit * (1 + 1e-6)
         # the pair is ~8 bits finer than FP16 alone
         assert abs(dec - float(v)) <= abs(hf - float(v)) + 1e-30
+
+
+def test_array_forms_match_the_scalar_codec():
+    """O.load_ch / O.store_ch (the array forms the GPU codec test compares
+    against) give the scalar functions' results element by element."""
+    L = O.lib()
+    rng = np.random.default_rng(5)
+    hi = rng.integers(0, 0x7c00, 3000).astype(np.uint16)
+    hi[::2] |= 0x8000
+    lo = rng.integers(-128, 128, 3000).astype(np.int8)
+    got = O.load_ch(hi, lo)
+    ref = np.array([L.or_load_ch(int(h), int(l)) for h, l in zip(hi, lo)], np.float32)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    v = (rng.standard_normal(3000) * 10.0 ** rng.uniform(-9, 4, 3000)).astype(np.float32)
+    h2, l2 = O.store_ch(v)
+    hb = (O.C.c_ushort * 1)()
+    lb = (O.C.c_byte * 1)()
+    for k in range(v.size):
+        L.or_store_ch(float(v[k]), hb, lb)
+        assert (h2[k], l2[k]) == (hb[0], lb[0])
