@@ -223,6 +223,10 @@ pw_status check_config(const pw_config& c)
     // its FP16 values: 2 bytes per cell of values, then 1 per cell
     if (has_lo(c.precision) && 3ull * pitch * c.ny * c.nz >= (1ull << 32) - (1ull << 20))
         return fail(PW_ERR_UNSUPPORTED, "pw_create: comp16 grid exceeds 2^32 / 3 padded cells (32-bit offsets)");
+    // comp16f: a pair's residuals lie 4 bytes per cell past its values, and a
+    // coefficient record's values are 4 FP16 elements per cell
+    if (c.precision == PW_COMP16F && 4ull * pitch * c.ny * c.nz >= (1ull << 32) - (1ull << 20))
+        return fail(PW_ERR_UNSUPPORTED, "pw_create: comp16f grid exceeds 2^30 padded cells (32-bit offsets)");
     return PW_OK;
 }
 
@@ -298,11 +302,24 @@ void plan_layout(const pw_config& c, Layout& L)
             L.field_off[k] = place(cells * sb);
         }
     }
-    for (int ax = 0; ax < 3; ++ax)
-        for (int k = 0; k < mem_slots(prec); ++k) take((size_t)g.slab_cells[ax] * mb);
+    // memory variables: one array per slot, or (mem_records) one array of
+    // 4-variable records for each kernel's four terms of the axis
+    for (int ax = 0; ax < 3; ++ax) {
+        if (mem_records(prec)) {
+            take((size_t)g.slab_cells[ax] * 4 * mb);
+            take((size_t)g.slab_cells[ax] * 4 * mb);
+        } else {
+            for (int k = 0; k < mem_slots(prec); ++k) take((size_t)g.slab_cells[ax] * mb);
+        }
+    }
     L.zero_bytes = L.total;
     for (int k = 0; k < kMaxCoef; ++k)
         if (coef_used(prec, k)) take(cells * cb);
+    if (has_records(prec))
+        for (int r = 0; r < kRecords; ++r) {
+            take(cells * 2 * rec_width(r));  // the FP16 values
+            take(cells * rec_width(r));      // the residual bytes
+        }
     for (int ax = 0; ax < 3; ++ax) take((size_t)n3[ax] * 4 * rb);
 }
 
@@ -357,8 +374,16 @@ pw_status pw::solver_create(const pw_config* cfg, pw_solver** out)
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.f[k] = base + L.field_off[k];
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = has_lo(s->prec) ? (int8_t*)(base + L.field_lo[k]) : nullptr;
     for (int ax = 0; ax < 3; ++ax)
-        for (int k = 0; k < kMemPerAxis; ++k) s->b.mem[ax][k] = k < mem_slots(s->prec) ? base + off[i++] : nullptr;
+        for (int k = 0; k < kMemPerAxis; ++k) {
+            const bool used = mem_records(s->prec) ? (k == 0 || k == 4) : k < mem_slots(s->prec);
+            s->b.mem[ax][k] = used ? base + off[i++] : nullptr;
+        }
     for (int k = 0; k < kMaxCoef; ++k) s->b.coef[k] = coef_used(s->prec, k) ? base + off[i++] : nullptr;
+    for (int r = 0; r < kRecords; ++r) {
+        s->b.rec_hi[r] = has_records(s->prec) ? base + off[i++] : nullptr;
+        s->b.rec_lo[r] = has_records(s->prec) ? base + off[i++] : nullptr;
+        for (int q = 0; q < 4; ++q) s->b.rec_scale[r][q] = 1.0f;
+    }
     for (int ax = 0; ax < 3; ++ax) s->b.tab[ax] = base + off[i++];
     s->b.prec = s->prec;
     s->b.dt = c.dt;
@@ -474,7 +499,7 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
     m.dt = c.dt;
     double* d_res = d_scratch;
     if (!d_res) PW_HIP_TRY(hipMalloc(&d_res, kReduceSlots * sizeof(double)));
-    double res[kReduceSlots] = {0, 0, 0, 0};
+    double res[kReduceSlots] = {};
     hipError_t e = medium_reduce(m, d_res, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(res, d_res, kReduceSlots * sizeof(double), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -533,6 +558,15 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
             if (res[3] > 0.0 && std::isfinite(res[3])) kw = std::max(0, std::min(40, (int)std::lrint(-std::log2(res[3]))));
             s->b.sc_vw = std::ldexp(s->b.sc_vel, kw);
             sm = ism = 1.0;
+            // coefficient records: each coefficient times 2^(15 - ex), its largest
+            // magnitude m in [2^(ex-1), 2^ex) then stored in [2^14, 2^15)
+            for (int r = 0; r < kRecords; ++r)
+                for (int q = 0; q < rec_width(r); ++q) {
+                    const double mx = res[4 + rec_base(r, q)];
+                    int ex = 15;
+                    if (mx > 0.0 && std::isfinite(mx)) std::frexp(mx, &ex);
+                    s->b.rec_scale[r][q] = std::ldexp(1.0f, ex - 15);
+                }
         }
     }
     e = medium_coefficients(m, s->b, sm, ism, s->stream);
@@ -1042,7 +1076,7 @@ extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
     // stress: reads vu x3 + vw x3, read-modify-writes tau x6 + p (20 touches).
     // coefficient arrays: scaled form 15 (velocity) + 7 (stress) FP16 arrays;
     // physical form 4 (rho, rhof, C1, C2) + 5 (C, M, H, H-2mu, mu) cell-centred
-    // arrays (comp16f: 4 + 4, no H)
+    // arrays (comp16f: two records of 4 compensated values, no H)
     const int cv = is_scaled(s->prec) ? 15 : 4, ct = is_scaled(s->prec) ? 7 : (s->prec == PW_COMP16F ? 4 : 5);
     // memory variables: each slot of a kernel read and written per damped axis
     const double mt = mem_slots(s->prec) * f * mem_bytes(s->prec);

@@ -21,6 +21,59 @@ namespace pw {
 // different kernels.
 __device__ __forceinline__ float round32(float v, unsigned z) { return __uint_as_float(__float_as_uint(v) ^ z); }
 
+// --------------------------------------------------------------------------
+// Compensated FP16 codec (Half/main.cu:59-100): a value is an FP16 "hi" plus an
+// int8 residual in units of 2^(e-19), |hi| = f * 2^e with f in [1/2, 1), and
+// 2^-32 below 2^-14.  From the FP16 bit pattern: biased FP16 exponent e5 (0 for
+// zero/subnormal) gives 2^(max(e5, 1) - 33), i.e. float bits
+// (max(e5,1)+94) << 23 with the 127 bias: e5 = 1 -> 2^-32, the subnormal value.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float c16_unit_bits(unsigned short hb)
+{
+    const unsigned e5 = (hb >> 10) & 31u;
+    return __uint_as_float(((e5 > 1u ? e5 : 1u) + 94u) << 23);
+}
+__device__ __forceinline__ float c16_inv_unit_bits(unsigned short hb)
+{
+    const unsigned e5 = (hb >> 10) & 31u;
+    return __uint_as_float((160u - (e5 > 1u ? e5 : 1u)) << 23);
+}
+__device__ __forceinline__ float c16_decode(_Float16 hi, int8_t lo)
+{
+    return (float)hi + (float)lo * c16_unit_bits(__builtin_bit_cast(unsigned short, hi));
+}
+__device__ __forceinline__ void c16_encode(float v_in, _Float16& hi, int8_t& lo, unsigned z)
+{
+    const float v = round32(v_in, z);
+    const _Float16 h = (_Float16)v;
+    const float hf = (float)h;
+    // exact: the unit is a power of two
+    const float scaled = (v - hf) * c16_inv_unit_bits(__builtin_bit_cast(unsigned short, h));
+    int q = (int)(scaled + (scaled >= 0.0f ? 0.5f : -0.5f));
+    q = q > 127 ? 127 : (q < -127 ? -127 : q);
+    hi = h;
+    lo = (int8_t)q;
+}
+
+// comp16f's encode: the same format, with the residual rounded to nearest
+// (ties to even) instead of half away from zero.  The inverse unit comes from
+// the FP32 exponent field E of the FP16 value, 2^(145 - max(E, 113)); the
+// clamped residual plus 1.5 * 2^23 is an FP32 whose low byte holds the rounded
+// residual in two's complement.
+__device__ __forceinline__ void c16_encode_rn(float v_in, _Float16& hi, int8_t& lo, unsigned z)
+{
+    const float v = round32(v_in, z);
+    const _Float16 h = (_Float16)v;
+    const float hf = (float)h;
+    const unsigned e = __float_as_uint(hf) & 0x7f800000u;
+    const unsigned eb = e > (113u << 23) ? e : (113u << 23);
+    const float iu = __uint_as_float((272u << 23) - eb);
+    const float q = __builtin_amdgcn_fmed3f((v - hf) * iu, -127.0f, 127.0f);
+    hi = h;
+    lo = (int8_t)(__float_as_uint(q + 12582912.0f) & 0xffu);
+}
+
+
 // Coefficient slots.  "physical" form (FP32 / FP64 storage): the nine
 // cell-centred medium fields on the padded grid (the reference's *_ext arrays
 // after extmodel); the kernels form the staggered averages and the
@@ -38,27 +91,57 @@ constexpr int kMaxCoef = 22;
 inline bool is_scaled(int prec) { return prec == PW_COMP16 || prec == PW_BF16; }
 // the compensated FP16 state: FP16 "hi" arrays plus int8 "lo" residual arrays
 inline bool has_lo(int prec) { return prec == PW_COMP16 || prec == PW_COMP16F; }
+// CPML memory variables as 4-variable records (one per damped cell and axis and
+// kernel: mem[ax][0] the velocity kernel's, mem[ax][4] the stress kernel's) for
+// the precisions with one variable per term; the scaled forms keep one array
+// per slot (their rescale walks the slots, comp16 pairs psi with prev).
+inline bool mem_records(int prec) { return !is_scaled(prec); }
 inline int ncoef(int prec) { return is_scaled(prec) ? (int)SC_N : (int)PC_N; }
-// comp16f forms H from H - 2mu and mu and holds no H array
-inline bool coef_used(int prec, int k) { return k < ncoef(prec) && !(prec == PW_COMP16F && k == PC_HH); }
-// comp16f stores the fields that the step kernels read or write together at one
-// cell in pairs: a pair's FP16 values alternate in one array (slot 0, slot 1 of
-// each cell) and so do its residual bytes, so one 4-byte and one 2-byte access
-// per lane move both fields.  Pairs: (vux, vuy), (vuz, vwz), (vwx, vwy),
-// (txx, tyy), (txz, tyz), (tzz, p); txy is stored alone.
+// the coefficient arrays of a precision (comp16f holds records instead, below)
+inline bool coef_used(int prec, int k) { return k < ncoef(prec) && prec != PW_COMP16F; }
+// comp16f keeps its coefficients as compensated records instead: per cell the
+// FP16 values of a record's coefficients side by side, and their residual
+// bytes side by side in a second array.  Record 0 holds the velocity kernel's
+// {rho, C1, C2, rhof}, records 1 and 2 the stress kernel's {C, M} and
+// {H - 2 mu, mu} (mu is also read on the neighbouring row, where a 2-wide
+// record touches half the cache lines of a 4-wide one).  Each coefficient is
+// the cell-centred value times a power of two that brings its largest
+// magnitude into [2^14, 2^15) (rec_scale holds the inverse).
+constexpr int kRecords = 3;
+enum { RV_RHO = 0, RV_C1, RV_C2, RV_RHOF };
+enum { RS_C = 0, RS_M };
+enum { RS_H2U = 0, RS_MU };
+__host__ __device__ constexpr int rec_width(int r) { return r == 0 ? 4 : 2; }
+__host__ __device__ constexpr int rec_base(int r, int q)
+{
+    return r == 0 ? (q == 0 ? PW_RHO : q == 1 ? PW_C1 : q == 2 ? PW_C2 : PW_RHOF)
+                  : r == 1 ? (q == 0 ? PW_CC : PW_MM) : (q == 0 ? PW_H2U : PW_MU);
+}
+inline bool has_records(int prec) { return prec == PW_COMP16F; }
+// comp16f stores fields in pairs: a pair's FP16 values alternate in one array
+// (slot 0, slot 1 of each cell) and so do its residual bytes, so one 4-byte and
+// one 2-byte access per lane move both fields.  The pairs are the fields the
+// step kernels read together on a neighbouring row, where a pair costs the
+// cache lines of one field and a half: the velocity kernel reads (tyy, p) on
+// row y+1 and (txy, tyz) on row y-1, the stress kernel (vux, vuz) on y+1 and
+// (vuy, vwy) on y-1.  The rest pair up as (txx, tzz) and (vwx, vwz); txz is
+// stored alone.
 __host__ __device__ constexpr int c16f_pair(int k)
 {
-    return k == PW_VUX || k == PW_VUY ? 0 : k == PW_VUZ || k == PW_VWZ ? 1 : k == PW_VWX || k == PW_VWY ? 2
-         : k == PW_TXX || k == PW_TYY ? 3 : k == PW_TXZ || k == PW_TYZ ? 4 : k == PW_TZZ || k == PW_P ? 5 : -1;
+    return k == PW_VUX || k == PW_VUZ ? 0 : k == PW_VUY || k == PW_VWY ? 1 : k == PW_VWX || k == PW_VWZ ? 2
+         : k == PW_TYY || k == PW_P ? 3 : k == PW_TXY || k == PW_TYZ ? 4 : k == PW_TXX || k == PW_TZZ ? 5 : -1;
 }
 __host__ __device__ constexpr int c16f_slot(int k)
 {
-    return k == PW_VUY || k == PW_VWZ || k == PW_VWY || k == PW_TYY || k == PW_TYZ || k == PW_P ? 1 : 0;
+    return k == PW_VUZ || k == PW_VWY || k == PW_VWZ || k == PW_P || k == PW_TYZ || k == PW_TZZ ? 1 : 0;
 }
+// the field comp16f stores alone
+constexpr int kC16FSingle = PW_TXZ;
 
 // Bytes of one stored wavefield value of field k / coefficient / memory variable.
 inline int state_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : has_lo(prec) ? 3 : 2; }
-inline int coef_bytes(int prec) { return prec == PW_FP64 ? 8 : (prec == PW_FP32 || prec == PW_COMP16F) ? 4 : 2; }
+// (comp16f: per coefficient of a record, 2 B of FP16 value + 1 B of residual)
+inline int coef_bytes(int prec) { return prec == PW_FP64 ? 8 : prec == PW_FP32 ? 4 : prec == PW_COMP16F ? 3 : 2; }
 // memory-variable slots in use per axis (the rest are null)
 inline int mem_slots(int prec) { return prec == PW_COMP16 ? 16 : 8; }
 inline int mem_bytes(int prec) { return prec == PW_FP64 ? 8 : (prec == PW_FP32 || prec == PW_COMP16F) ? 4 : 2; }
@@ -71,6 +154,9 @@ struct DevBuffers {
     void* f[PW_NFIELDS];
     int8_t* lo[PW_NFIELDS];
     void* coef[kMaxCoef];
+    void* rec_hi[kRecords];        // comp16f coefficient records (see rec_base), null otherwise
+    void* rec_lo[kRecords];
+    float rec_scale[kRecords][4];  // physical coefficient = decoded record value * rec_scale
     void* mem[3][kMemPerAxis];
     void* tab[3];    // per axis: n entries of {c_full, e_full, c_half, e_half} (float or double)
     double dt, h;
@@ -152,9 +238,10 @@ struct MediumArgs {
     double dt;
 };
 // res[0] = Kmax, res[1] = Rmax, res[2] = max fast-P velocity squared (Half/main.cu:745-791),
-// res[3] = max rhof / CC1 (the size of the relative fluid velocity against the solid's).
+// res[3] = max rhof / CC1 (the size of the relative fluid velocity against the solid's),
+// res[4 + f] = max |base field f| (f < PW_NBASE).
 // d_res (kReduceSlots doubles) is cleared by a fill kernel first.
-constexpr int kReduceSlots = 4;
+constexpr int kReduceSlots = 4 + PW_NBASE;
 hipError_t medium_reduce(const MediumArgs& m, double* d_res, hipStream_t st);
 // The scratch of the allocation-free medium installation (pw_set_medium_device):
 // a file-scope device array of kScratchDoubles doubles.  scratch_for_device
@@ -164,7 +251,8 @@ hipError_t medium_reduce(const MediumArgs& m, double* d_res, hipStream_t st);
 constexpr int kScratchDoubles = 64;
 hipError_t scratch_for_device(int dev, double** out);
 std::mutex& scratch_mutex();
-// Writes the coefficient arrays of b (physical or scaled form).
+// Writes the coefficient arrays of b (physical or scaled form), or comp16f's
+// records with each coefficient stored times 1 / b.rec_scale.
 hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double scale_m, double inv_scale_m,
                                hipStream_t st);
 

@@ -50,66 +50,6 @@ typedef _Float16 f16;
 typedef __bf16 bf16;
 
 // --------------------------------------------------------------------------
-// Compensated FP16 codec (Half/main.cu:59-100).
-// unit = 2^(e-19) where |hi| = f * 2^e, f in [1/2, 1): for a normal float with
-// biased exponent E that is 2^(E-145); below 2^-14 the unit is 2^-32.
-// --------------------------------------------------------------------------
-__device__ __forceinline__ float c16_unit(float hf)
-{
-    const float a = __builtin_fabsf(hf);
-    if (a < 6.103515625e-5f) return 2.3283064365386963e-10f;
-    const unsigned E = (__float_as_uint(a) >> 23) & 0xffu;
-    return __uint_as_float((E - 18u) << 23);
-}
-// The same unit from the FP16 bit pattern: biased FP16 exponent e5 (0 for
-// zero/subnormal) gives 2^(max(e5, 1) - 33), i.e. float bits (max(e5,1)+94) << 23
-// with the 127 bias: e5 = 1 -> 2^-32, the subnormal value.
-__device__ __forceinline__ float c16_unit_bits(unsigned short hb)
-{
-    const unsigned e5 = (hb >> 10) & 31u;
-    return __uint_as_float(((e5 > 1u ? e5 : 1u) + 94u) << 23);
-}
-__device__ __forceinline__ float c16_inv_unit_bits(unsigned short hb)
-{
-    const unsigned e5 = (hb >> 10) & 31u;
-    return __uint_as_float((160u - (e5 > 1u ? e5 : 1u)) << 23);
-}
-__device__ __forceinline__ float c16_decode(f16 hi, int8_t lo)
-{
-    return (float)hi + (float)lo * c16_unit_bits(__builtin_bit_cast(unsigned short, hi));
-}
-__device__ __forceinline__ void c16_encode(float v_in, f16& hi, int8_t& lo, unsigned z)
-{
-    const float v = round32(v_in, z);
-    const f16 h = (f16)v;
-    const float hf = (float)h;
-    // exact: the unit is a power of two
-    const float scaled = (v - hf) * c16_inv_unit_bits(__builtin_bit_cast(unsigned short, h));
-    int q = (int)(scaled + (scaled >= 0.0f ? 0.5f : -0.5f));
-    q = q > 127 ? 127 : (q < -127 ? -127 : q);
-    hi = h;
-    lo = (int8_t)q;
-}
-
-// comp16f's encode: the same format, with the residual rounded to nearest
-// (ties to even) instead of half away from zero.  The inverse unit comes from
-// the FP32 exponent field E of the FP16 value, 2^(145 - max(E, 113)); the
-// clamped residual plus 1.5 * 2^23 is an FP32 whose low byte holds the rounded
-// residual in two's complement.
-__device__ __forceinline__ void c16_encode_rn(float v_in, f16& hi, int8_t& lo, unsigned z)
-{
-    const float v = round32(v_in, z);
-    const f16 h = (f16)v;
-    const float hf = (float)h;
-    const unsigned e = __float_as_uint(hf) & 0x7f800000u;
-    const unsigned eb = e > (113u << 23) ? e : (113u << 23);
-    const float iu = __uint_as_float((272u << 23) - eb);
-    const float q = __builtin_amdgcn_fmed3f((v - hf) * iu, -127.0f, 127.0f);
-    hi = h;
-    lo = (int8_t)(__float_as_uint(q + 12582912.0f) & 0xffu);
-}
-
-// --------------------------------------------------------------------------
 // Storage policies
 //   ld / st        one value
 //   ldv / stv      N consecutive values (offset a multiple of N) as one wide access
@@ -310,11 +250,16 @@ template <class P> struct StepArgs {
     typename P::R dt, h;
     // compensated fields: bytes from a field's FP16 values to its residual bytes
     // (the same for every field; comp16f: for the slot-0 field of every pair,
-    // and loff1 for txy, which is stored alone)
+    // and loff1 for the field stored alone)
     unsigned loff, loff1;
     // kFormFast launch constants (KF_*): dt/h, the factors of the staggered
     // means and the class storage scales folded together
     float kf[9];
+    // comp16f coefficient records (rec_base order): FP16 values, residuals, and
+    // the factor from a decoded value to the physical coefficient
+    const f16* rh[kRecords];
+    const int8_t* rl[kRecords];
+    float rs[kRecords][4];
 };
 
 // kFormFast constants; g = dt/h, s_u, s_w, s_t = storage scales of the solid
@@ -373,7 +318,7 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     // the pool stores each compensated field's residual bytes right after its
     // FP16 values, at one offset for all fields
     a.loff = b.lo[0] ? (unsigned)((const char*)b.lo[0] - (const char*)b.f[0]) : 0u;
-    a.loff1 = b.lo[PW_TXY] ? (unsigned)((const char*)b.lo[PW_TXY] - (const char*)b.f[PW_TXY]) : 0u;
+    a.loff1 = b.lo[kC16FSingle] ? (unsigned)((const char*)b.lo[kC16FSingle] - (const char*)b.f[kC16FSingle]) : 0u;
     const double g = b.dt / b.h, su = b.sc_vel, sw = b.sc_vw, stt = b.sc_stress;
     a.kf[KF_DT2] = (float)(0.5 * b.dt);
     a.kf[KF_WS] = (float)(4.0 * g * sw / stt);
@@ -384,6 +329,11 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     a.kf[KF_G2] = (float)(2.0 * g * stt / su);
     a.kf[KF_G4] = (float)(4.0 * g * stt / su);
     a.kf[KF_W] = (float)(su / sw);
+    for (int r = 0; r < kRecords; ++r) {
+        a.rh[r] = (const f16*)b.rec_hi[r];
+        a.rl[r] = (const int8_t*)b.rec_lo[r];
+        for (int q = 0; q < 4; ++q) a.rs[r][q] = b.rec_scale[r][q];
+    }
     return a;
 }
 
@@ -474,8 +424,8 @@ template <> struct FA<PolC16> {
 // comp16f, in stored units (the class scales are folded into the launch
 // constants), with the round-to-nearest encode.  A paired field in slot q has
 // f[fid].hi = pair values + q (in FP16 units); the cell's values lie at 2o + q
-// and its residual byte a.loff - q + 2o bytes after f[fid].hi.  txy (alone)
-// has comp16's addressing with a.loff1.
+// and its residual byte a.loff - q + 2o bytes after f[fid].hi.  The field
+// stored alone has comp16's addressing with a.loff1.
 template <> struct FA<PolC16F> {
     using R = float;
     using P = PolC16F;
@@ -484,8 +434,8 @@ template <> struct FA<PolC16F> {
     static __device__ __forceinline__ R ld(const StepArgs<P>& a, int fid, unsigned o)
     {
         const f16* h = a.f[fid].hi;
-        if (paired(fid)) return c16_decode(h[2 * o], ((const int8_t*)h)[a.loff - slot(fid) + 2 * o]);
-        return c16_decode(h[o], ((const int8_t*)h)[a.loff1 + o]);
+        if (paired(fid)) return c16_decode(h[2 * o], ((const int8_t*)h + a.loff - slot(fid))[2 * o]);
+        return c16_decode(h[o], ((const int8_t*)h + a.loff1)[o]);
     }
     template <int N> static __device__ __forceinline__ void ldv(const StepArgs<P>& a, int fid, unsigned o, R (&r)[N])
     {
@@ -501,7 +451,7 @@ template <> struct FA<PolC16F> {
             return;
         }
         const Pack<f16, N> hv = ld_pack<N>(h, o);
-        const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h, a.loff1 + o);
+        const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h + a.loff1, o);
 #pragma unroll
         for (int j = 0; j < N; ++j) r[j] = c16_decode(hv.v[j], lv.v[j]);
     }
@@ -527,10 +477,10 @@ template <> struct FA<PolC16F> {
         c16_encode_rn(v, hh, l, a.zero);
         if (paired(fid)) {
             h[2 * o] = hh;
-            ((int8_t*)h)[a.loff - slot(fid) + 2 * o] = l;
+            ((int8_t*)h + a.loff - slot(fid))[2 * o] = l;
         } else {
             h[o] = hh;
-            ((int8_t*)h)[a.loff1 + o] = l;
+            ((int8_t*)h + a.loff1)[o] = l;
         }
     }
     template <int N>
@@ -547,7 +497,7 @@ template <> struct FA<PolC16F> {
 #pragma unroll
         for (int j = 0; j < N; ++j) c16_encode_rn(r[j], hv.v[j], lv.v[j], a.zero);
         st_pack<N>(h, o, hv);
-        st_pack<N>((int8_t*)h, a.loff1 + o, lv);
+        st_pack<N>((int8_t*)h + a.loff1, o, lv);
     }
     template <int N>
     static __device__ __forceinline__ void st2v(const StepArgs<P>& a, int fa, int fb, unsigned o, const R (&ra)[N],
@@ -566,6 +516,35 @@ template <> struct FA<PolC16F> {
         (void)fb;
     }
 };
+
+// comp16f coefficient record r (rec_base order, K = rec_width(r) coefficients)
+// of cells o .. o+N-1: one access for the N x K FP16 values, one for the
+// residual bytes; c[q][j] = coefficient q of cell o + j.
+template <int K, int N, class P, class R>
+__device__ __forceinline__ void ld_rec(const StepArgs<P>& a, int r, unsigned o, R (&c)[K][N])
+{
+    const Pack<f16, K * N> h = ld_pack<K * N>(a.rh[r] + K * (size_t)o, 0u);
+    const Pack<int8_t, K * N> l = ld_pack<K * N>(a.rl[r] + K * (size_t)o, 0u);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int q = 0; q < K; ++q) c[q][j] = c16_decode(h.v[K * j + q], l.v[K * j + q]) * a.rs[r][q];
+}
+// coefficient q of the K-wide record r at one cell
+template <int K, class P> __device__ __forceinline__ float ld_rec1(const StepArgs<P>& a, int r, int q, unsigned o)
+{
+    const size_t k = K * (size_t)o + q;
+    return c16_decode(a.rh[r][k], a.rl[r][k]) * a.rs[r][q];
+}
+// coefficient q of the K-wide record r at cells o .. o+N-1 (the whole records)
+template <int K, int N, class P, class R>
+__device__ __forceinline__ void ld_rec_slot(const StepArgs<P>& a, int r, int q, unsigned o, R (&c)[N])
+{
+    R t[K][N];
+    ld_rec<K, N>(a, r, o, t);
+#pragma unroll
+    for (int j = 0; j < N; ++j) c[j] = t[q][j];
+}
 
 // Two fields at the same cells: one paired access where the policy stores the
 // two as a pair (fa in slot 0, fb in slot 1), two accesses otherwise.
@@ -675,6 +654,20 @@ __device__ __forceinline__ void cpml4(typename P::Mem* const* phi, unsigned k, c
             P::stm(phi[q + 8], k, d, z);
             *der[q] = d + nw;
         }
+    } else if constexpr (!P::scaled) {
+        // the four variables of (cell, axis) k as one record at phi[0] + 4k
+        using M = typename P::Mem;
+        M* rec = phi[0] + 4 * (size_t)k;
+        const Pack<M, 4> t = ld_pack<4>(rec, 0u);
+        Pack<M, 4> u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const R d = *der[q];
+            const R psi = (R)t.v[q] - c[q] * d;
+            u.v[q] = (M)(e[q] * (psi - c[q] * d));
+            *der[q] = d + psi;
+        }
+        st_pack<4>(rec, 0u, u);
     } else {
         R m[4];
 #pragma unroll
@@ -943,8 +936,20 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
     // z-2 and z; tzz, p at z-1 and z+1.
     R txz_m[N], tyz_m[N], tzz_c[N], p_c[N];
     R txz_mm[N], tyz_mm[N], txz_k[N], tyz_k[N], tzz_m[N], p_m[N], tzz_k[N], p_k[N];
-    LD2(PW_TXZ, PW_TYZ, o - PL, txz_m, tyz_m);
-    LD2(PW_TZZ, PW_P, o, tzz_c, p_c);
+    // comp16f, order 2: its pairs (tyy, p) and (txx, tzz) are read a plane ahead,
+    // so txx and tyy are carried too
+    constexpr bool PAIRS = std::is_same<P, PolC16F>::value && ORD == 2;
+    R txx_p[N], tyy_p[N];
+    if constexpr (PAIRS) {
+        R txy_m[N];
+        LD(PW_TXZ, o - PL, txz_m);
+        LD2(PW_TXY, PW_TYZ, o - PL, txy_m, tyz_m);
+        LD2(PW_TYY, PW_P, o, tyy_p, p_c);
+        LD2(PW_TXX, PW_TZZ, o, txx_p, tzz_c);
+    } else {
+        LD2(PW_TXZ, PW_TYZ, o - PL, txz_m, tyz_m);
+        LD2(PW_TZZ, PW_P, o, tzz_c, p_c);
+    }
     if constexpr (ORD == 4) {
         LD(PW_TXZ, o - 2 * PL, txz_mm);
         LD(PW_TYZ, o - 2 * PL, tyz_mm);
@@ -956,8 +961,17 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         LD(PW_P, o + PL, p_k);
     }
     // physical form: cell-centred rho, C1, C2 of this plane, carried up the column
-    R rho_c[N], c1_c[N], c2_c[N];
-    if constexpr (!P::scaled) {
+    // (comp16f: its whole coefficient record, rhof too)
+    constexpr bool REC = std::is_same<P, PolC16F>::value;
+    R rho_c[N], c1_c[N], c2_c[N], rf_c[N];
+    if constexpr (REC) {
+        R t[4][N];
+        ld_rec<4, N>(a, 0, o - L.sh, t);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            rho_c[j] = t[RV_RHO][j]; c1_c[j] = t[RV_C1][j]; c2_c[j] = t[RV_C2][j]; rf_c[j] = t[RV_RHOF][j];
+        }
+    } else if constexpr (!P::scaled) {
         LDC(PC_RHO, o, rho_c);
         LDC(PC_C1, o, c1_c);
         LDC(PC_C2, o, c2_c);
@@ -972,18 +986,33 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         if constexpr (ORD == 2) {
             R txx_c[N], txx_r[N], tyy_c[N], tyy_q[N], txy_c[N], txy_l[N], txy_h[N], txz_l[N], tyz_h[N], p_r[N],
                 p_q[N];
-            LD2(PW_TXZ, PW_TYZ, o, txz_c, tyz_c);
-            LD2(PW_TZZ, PW_P, o + PL, tzz_n, p_n);
-            LD2(PW_TXX, PW_TYY, o, txx_c, tyy_c);
+            if constexpr (PAIRS) {
+                R txx_n[N], tyy_n[N];
+                LD(PW_TXZ, o, txz_c);
+                LD2(PW_TXY, PW_TYZ, o, txy_c, tyz_c);
+                LD2(PW_TYY, PW_P, o + PL, tyy_n, p_n);
+                LD2(PW_TXX, PW_TZZ, o + PL, txx_n, tzz_n);
+                LD2(PW_TYY, PW_P, o + W, tyy_q, p_q);
+                LD2(PW_TXY, PW_TYZ, o - W, txy_h, tyz_h);
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    txx_c[j] = txx_p[j]; txx_p[j] = txx_n[j];
+                    tyy_c[j] = tyy_p[j]; tyy_p[j] = tyy_n[j];
+                }
+            } else {
+                LD2(PW_TXZ, PW_TYZ, o, txz_c, tyz_c);
+                LD2(PW_TZZ, PW_P, o + PL, tzz_n, p_n);
+                LD2(PW_TXX, PW_TYY, o, txx_c, tyy_c);
+                LD(PW_TYY, o + W, tyy_q);
+                LD(PW_TXY, o, txy_c);
+                LD(PW_TXY, o - W, txy_h);
+                LD(PW_TYZ, o - W, tyz_h);
+                LD(PW_P, o + W, p_q);
+            }
             seg_right(L, txx_c, txx_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_TXX, q); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
-            LD(PW_TYY, o + W, tyy_q);
-            LD(PW_TXY, o, txy_c);
             seg_left(L, txy_c, txy_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXY, q); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
-            LD(PW_TXY, o - W, txy_h);
             seg_left(L, txz_c, txz_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXZ, q); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
-            LD(PW_TYZ, o - W, tyz_h);
             seg_right(L, p_c, p_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_P, q); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
-            LD(PW_P, o + W, p_q);
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 dx1[j] = txx_r[j] - txx_c[j]; dx2[j] = txy_c[j] - txy_h[j]; dx3[j] = txz_c[j] - txz_m[j];
@@ -1045,9 +1074,9 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         // per plane.  (Hoisting the coefficients as well costs a wave per SIMD in
         // registers and measured slower.)
         R wx0[N], wy0[N], wz0[N], ux0[N], uy0[N], uz0[N];
-        LD2(PW_VWX, PW_VWY, o, wx0, wy0);
-        LD2(PW_VUZ, PW_VWZ, o, uz0, wz0);
-        LD2(PW_VUX, PW_VUY, o, ux0, uy0);
+        LD2(PW_VWX, PW_VWZ, o, wx0, wz0);
+        LD2(PW_VUY, PW_VWY, o, uy0, wy0);
+        LD2(PW_VUX, PW_VUZ, o, ux0, uz0);
         const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
         Tab4<R> tz = {0, 1, 0, 1};
         if (zs >= 0) tz = a.tab[2][iz];
@@ -1089,6 +1118,23 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         R wx1[N], wy1[N], wz1[N], ux1[N], uy1[N], uz1[N];
         if constexpr (!P::scaled) {
             R rho_n[N], rho_x[N], rho_y[N], c1_n[N], c1_x[N], c1_y[N], c2_n[N], c2_x[N], c2_y[N], rf[N];
+            if constexpr (REC) {
+                // the next plane's records (carried), the x+1 neighbours across
+                // lanes, the y+1 row's records; rhof is this plane's
+                R tn[4][N], ty[4][N];
+                ld_rec<4, N>(a, 0, o + PL - L.sh, tn);
+                seg_right(L, rho_c, rho_x, o + N, [&](unsigned q) { return ld_rec1<4>(a, 0, RV_RHO, q); }, [&](R (&r)[N]) { ld_rec_slot<4, N>(a, 0, RV_RHO, o + 1 - L.sh, r); });
+                seg_right(L, c1_c, c1_x, o + N, [&](unsigned q) { return ld_rec1<4>(a, 0, RV_C1, q); }, [&](R (&r)[N]) { ld_rec_slot<4, N>(a, 0, RV_C1, o + 1 - L.sh, r); });
+                seg_right(L, c2_c, c2_x, o + N, [&](unsigned q) { return ld_rec1<4>(a, 0, RV_C2, q); }, [&](R (&r)[N]) { ld_rec_slot<4, N>(a, 0, RV_C2, o + 1 - L.sh, r); });
+                ld_rec<4, N>(a, 0, o + W - L.sh, ty);
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    rho_n[j] = tn[RV_RHO][j]; c1_n[j] = tn[RV_C1][j]; c2_n[j] = tn[RV_C2][j];
+                    rho_y[j] = ty[RV_RHO][j]; c1_y[j] = ty[RV_C1][j]; c2_y[j] = ty[RV_C2][j];
+                    rf[j] = rf_c[j];
+                    rf_c[j] = tn[RV_RHOF][j];
+                }
+            } else {
             LDC(PC_RHO, o + PL, rho_n);
             seg_right(L, rho_c, rho_x, o + N, [&](unsigned q) { return P::ldc(a.c[PC_RHO], q); }, [&](R (&r)[N]) { LDC(PC_RHO, o + 1, r); });
             LDC(PC_RHO, o + W, rho_y);
@@ -1099,6 +1145,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             seg_right(L, c2_c, c2_x, o + N, [&](unsigned q) { return P::ldc(a.c[PC_C2], q); }, [&](R (&r)[N]) { LDC(PC_C2, o + 1, r); });
             LDC(PC_C2, o + W, c2_y);
             LDC(PC_RHOF, o, rf);
+            }
             if constexpr (P::form == kFormFast) {
                 // Float/main.cu:106-113, 160-189 in stored units, with the
                 // staggered means folded: ir = 1/(rho + rho') (rho^-1 = 2 ir),
@@ -1176,9 +1223,9 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
                 uz1[j] = uz0[j] + ri[j] * sz[j] - fr[j] * (wz1[j] - wz0[j]);
             }
         }
-        ST2(PW_VWX, PW_VWY, o, wx1, wy1);
-        ST2(PW_VUZ, PW_VWZ, o, uz1, wz1);
-        ST2(PW_VUX, PW_VUY, o, ux1, uy1);
+        ST2(PW_VWX, PW_VWZ, o, wx1, wz1);
+        ST2(PW_VUY, PW_VWY, o, uy1, wy1);
+        ST2(PW_VUX, PW_VUZ, o, ux1, uz1);
 
         if constexpr (ORD == 2) {
 #pragma unroll
@@ -1258,8 +1305,20 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     // at z-2 and z; vux, vuy at z-1 and z+1.
     R vuz_m[N], vwz_m[N], vux_c[N], vuy_c[N];
     R vuz_mm[N], vwz_mm[N], vuz_k[N], vwz_k[N], vux_m[N], vuy_m[N], vux_k[N], vuy_k[N];
-    LD2(PW_VUZ, PW_VWZ, o - PL, vuz_m, vwz_m);
-    LD2(PW_VUX, PW_VUY, o, vux_c, vuy_c);
+    // comp16f, order 2: its pairs (vux, vuz) and (vuy, vwy) are read a plane
+    // ahead, so vuz and vwy are carried too
+    constexpr bool PAIRS = std::is_same<P, PolC16F>::value && ORD == 2;
+    R vuz_p[N], vwy_p[N];
+    if constexpr (PAIRS) {
+        R vux_m1[N], vwx_m1[N];
+        LD2(PW_VUX, PW_VUZ, o - PL, vux_m1, vuz_m);
+        LD2(PW_VWX, PW_VWZ, o - PL, vwx_m1, vwz_m);
+        LD2(PW_VUX, PW_VUZ, o, vux_c, vuz_p);
+        LD2(PW_VUY, PW_VWY, o, vuy_c, vwy_p);
+    } else {
+        LD2(PW_VUZ, PW_VWZ, o - PL, vuz_m, vwz_m);
+        LD2(PW_VUX, PW_VUY, o, vux_c, vuy_c);
+    }
     if constexpr (ORD == 4) {
         LD(PW_VUZ, o - 2 * PL, vuz_mm);
         LD(PW_VWZ, o - 2 * PL, vwz_mm);
@@ -1284,7 +1343,27 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     };
     auto EMU = [&](unsigned q) { return MU(P::ldc(a.c[PC_MU], q)); };
     R mu_c[N], mu_x[N], mu_y[N], mv_c[N];
-    if constexpr (!P::scaled) {
+    // comp16f: this plane's H - 2mu comes with mu in its record, carried
+    constexpr bool REC = std::is_same<P, PolC16F>::value;
+    R lam_c[N];
+    auto EMUR = [&](unsigned q) { return MU(ld_rec1<2>(a, 2, RS_MU, q)); };
+    auto LDMUR = [&](unsigned off, R (&r)[N]) {
+        ld_rec_slot<2, N>(a, 2, RS_MU, off - L.sh, r);
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = MU(r[j]);
+    };
+    if constexpr (REC) {
+        R t[2][N];
+        ld_rec<2, N>(a, 2, o - L.sh, t);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            lam_c[j] = t[RS_H2U][j];
+            mv_c[j] = t[RS_MU][j];
+            mu_c[j] = MU(mv_c[j]);
+        }
+        seg_right(L, mu_c, mu_x, o + N, EMUR, [&](R (&r)[N]) { LDMUR(o + 1, r); });
+        LDMUR(o + W, mu_y);
+    } else if constexpr (!P::scaled) {
         LDC(PC_MU, o, mv_c);
 #pragma unroll
         for (int j = 0; j < N; ++j) mu_c[j] = MU(mv_c[j]);
@@ -1300,17 +1379,31 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         R vux_n[N], vuy_n[N], vuz_c[N], vwz_c[N];
         if constexpr (ORD == 2) {
             R vux_l[N], vux_q[N], vuy_h[N], vuy_r[N], vuz_r[N], vuz_q[N], vwx_c[N], vwx_l[N], vwy_c[N], vwy_h[N];
-            LD2(PW_VUX, PW_VUY, o + PL, vux_n, vuy_n);
-            LD2(PW_VUZ, PW_VWZ, o, vuz_c, vwz_c);
+            if constexpr (PAIRS) {
+                R vuz_n[N], vwy_n[N];
+                LD2(PW_VUX, PW_VUZ, o + PL, vux_n, vuz_n);
+                LD2(PW_VUY, PW_VWY, o + PL, vuy_n, vwy_n);
+                LD2(PW_VWX, PW_VWZ, o, vwx_c, vwz_c);
+                LD2(PW_VUX, PW_VUZ, o + W, vux_q, vuz_q);
+                LD2(PW_VUY, PW_VWY, o - W, vuy_h, vwy_h);
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    vuz_c[j] = vuz_p[j]; vuz_p[j] = vuz_n[j];
+                    vwy_c[j] = vwy_p[j]; vwy_p[j] = vwy_n[j];
+                }
+            } else {
+                LD2(PW_VUX, PW_VUY, o + PL, vux_n, vuy_n);
+                LD2(PW_VUZ, PW_VWZ, o, vuz_c, vwz_c);
+                LD(PW_VUX, o + W, vux_q);
+                LD(PW_VUY, o - W, vuy_h);
+                LD(PW_VUZ, o + W, vuz_q);
+                LD2(PW_VWX, PW_VWY, o, vwx_c, vwy_c);
+                LD(PW_VWY, o - W, vwy_h);
+            }
             seg_left(L, vux_c, vux_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VUX, q); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
-            LD(PW_VUX, o + W, vux_q);
-            LD(PW_VUY, o - W, vuy_h);
             seg_right(L, vuy_c, vuy_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUY, q); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
             seg_right(L, vuz_c, vuz_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUZ, q); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
-            LD(PW_VUZ, o + W, vuz_q);
-            LD2(PW_VWX, PW_VWY, o, vwx_c, vwy_c);
             seg_left(L, vwx_c, vwx_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VWX, q); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
-            LD(PW_VWY, o - W, vwy_h);
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 dxx[j] = vux_c[j] - vux_l[j]; dyy[j] = vuy_c[j] - vuy_h[j]; dzz[j] = vuz_c[j] - vuz_m[j];
@@ -1365,14 +1458,31 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         if (zs >= 0) tz = a.tab[2][iz];
 
         R p0[N], xx0[N], yy0[N], zz0[N], xy0[N], yz0[N], xz0[N];
-        LD2(PW_TZZ, PW_P, o, zz0, p0);
-        LD2(PW_TXX, PW_TYY, o, xx0, yy0);
-        LD(PW_TXY, o, xy0);
-        LD2(PW_TXZ, PW_TYZ, o, xz0, yz0);
+        LD2(PW_TYY, PW_P, o, yy0, p0);
+        LD2(PW_TXY, PW_TYZ, o, xy0, yz0);
+        LD2(PW_TXX, PW_TZZ, o, xx0, zz0);
+        LD(PW_TXZ, o, xz0);
         R p1[N], xx1[N], yy1[N], zz1[N], xy1[N], yz1[N], xz1[N];
         R mu_n[N], mu_xn[N], mu_yn[N], mu_xy[N], mv_n[N];
         R C_[N], M_[N], HH_[N], H2u_[N], mxy[N], myz[N], mxz[N];
-        if constexpr (!P::scaled) {
+        if constexpr (REC) {
+            R tn[2][N], tc[2][N];
+            ld_rec<2, N>(a, 2, o + PL - L.sh, tn);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                mv_n[j] = tn[RS_MU][j];
+                mu_n[j] = MU(mv_n[j]);
+            }
+            seg_right(L, mu_n, mu_xn, o + PL + N, EMUR, [&](R (&r)[N]) { LDMUR(o + PL + 1, r); });
+            LDMUR(o + PL + W, mu_yn);
+            seg_right(L, mu_y, mu_xy, o + W + N, EMUR, [&](R (&r)[N]) { LDMUR(o + W + 1, r); });
+            ld_rec<2, N>(a, 1, o - L.sh, tc);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                C_[j] = tc[RS_C][j]; M_[j] = tc[RS_M][j]; H2u_[j] = lam_c[j];
+                lam_c[j] = tn[RS_H2U][j];
+            }
+        } else if constexpr (!P::scaled) {
             LDC(PC_MU, o + PL, mv_n);
 #pragma unroll
             for (int j = 0; j < N; ++j) mu_n[j] = MU(mv_n[j]);
@@ -1465,10 +1575,10 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                 xz1[j] = xz0[j] + mxz[j] * (uxz + uzx);
             }
         }
-        ST2(PW_TZZ, PW_P, o, zz1, p1);
-        ST2(PW_TXX, PW_TYY, o, xx1, yy1);
-        ST(PW_TXY, o, xy1);
-        ST2(PW_TXZ, PW_TYZ, o, xz1, yz1);
+        ST2(PW_TYY, PW_P, o, yy1, p1);
+        ST2(PW_TXY, PW_TYZ, o, xy1, yz1);
+        ST2(PW_TXX, PW_TZZ, o, xx1, zz1);
+        ST(PW_TXZ, o, xz1);
 
         if constexpr (ORD == 2) {
 #pragma unroll

@@ -91,9 +91,10 @@ __device__ __forceinline__ void atomic_max_pos(double* addr, double v)
 
 __global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double dt, double* res)
 {
-    __shared__ double sk[256], sr[256], sv[256], sw[256];
+    __shared__ double sh[kReduceSlots][256];
     const long long n = (long long)nx * ny * nz;
     double kmax = 0.0, rmax = 0.0, vmax = 0.0, wmax = 0.0;
+    double fmx[PW_NBASE] = {};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const int ix = (int)(i % nx), iy = (int)((i / nx) % ny), iz = (int)(i / ((long long)nx * ny));
@@ -120,24 +121,18 @@ __global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double dt, d
         }
         vmax = fmax(vmax, vp2_cell(v.at(PW_RHO, ix, iy, iz), rhof, v.at(PW_C1, ix, iy, iz), v.at(PW_C2, ix, iy, iz),
                                    HH, C, M));
+        // largest magnitude of every base field (comp16f's coefficient scales)
+        for (int f = 0; f < PW_NBASE; ++f) fmx[f] = fmax(fmx[f], fabs(v.at(f, ix, iy, iz)));
     }
-    sk[threadIdx.x] = kmax; sr[threadIdx.x] = rmax; sv[threadIdx.x] = vmax; sw[threadIdx.x] = wmax;
+    sh[0][threadIdx.x] = kmax; sh[1][threadIdx.x] = rmax; sh[2][threadIdx.x] = vmax; sh[3][threadIdx.x] = wmax;
+    for (int f = 0; f < PW_NBASE; ++f) sh[4 + f][threadIdx.x] = fmx[f];
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            sk[threadIdx.x] = fmax(sk[threadIdx.x], sk[threadIdx.x + w]);
-            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + w]);
-            sv[threadIdx.x] = fmax(sv[threadIdx.x], sv[threadIdx.x + w]);
-            sw[threadIdx.x] = fmax(sw[threadIdx.x], sw[threadIdx.x + w]);
-        }
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < kReduceSlots; ++k) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        atomic_max_pos(res + 0, sk[0]);
-        atomic_max_pos(res + 1, sr[0]);
-        atomic_max_pos(res + 2, sv[0]);
-        atomic_max_pos(res + 3, sw[0]);
-    }
+    if (threadIdx.x < kReduceSlots) atomic_max_pos(res + threadIdx.x, sh[threadIdx.x][0]);
 }
 
 template <class T> __device__ __forceinline__ T to_store(double v) { return (T)v; }
@@ -167,6 +162,33 @@ __global__ void k_coef_physical(BaseView v, int nx, int ny, int nz, double dt, C
     c[PC_H2U][o] = to_store<T>(v.at(PW_H2U, ix, iy, iz));
     c[PC_MU][o] = to_store<T>(v.at(PW_MU, ix, iy, iz));
     (void)ny; (void)nz; (void)dt;
+}
+
+// comp16f: the two coefficient records of each cell (rec_base order), each
+// coefficient rounded to FP32, times its power-of-two scale (exact), then
+// encoded as FP16 value + residual with comp16f's rounding.
+struct RecOut {
+    f16* hi[kRecords];
+    int8_t* lo[kRecords];
+    float enc[kRecords][4];
+    int pitch;
+    long long plane;
+};
+__global__ void k_coef_records(BaseView v, int nx, RecOut out)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= nx) return;
+    const size_t o = (size_t)ix + (size_t)iy * out.pitch + (size_t)iz * out.plane;
+    for (int r = 0; r < kRecords; ++r) {
+        const int w = rec_width(r);
+        for (int q = 0; q < w; ++q) {
+            f16 h;
+            int8_t l;
+            c16_encode_rn((float)v.at(rec_base(r, q), ix, iy, iz) * out.enc[r][q], h, l, 0u);
+            out.hi[r][w * o + q] = h;
+            out.lo[r][w * o + q] = l;
+        }
+    }
 }
 
 // Scaled form (comp16 / bf16), Half/main.cu:967-1044: FP16 arrays prescaled by
@@ -300,7 +322,18 @@ hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double 
     switch (b.prec) {
     case PW_FP32: hipLaunchKernelGGL(k_coef_physical<float>, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, out); break;
     case PW_FP64: hipLaunchKernelGGL(k_coef_physical<double>, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, out); break;
-    case PW_COMP16F: hipLaunchKernelGGL(k_coef_physical<float>, grid, block, 0, st, v, m.nx, m.ny, m.nz, m.dt, out); break;
+    case PW_COMP16F: {
+        RecOut ro;
+        for (int r = 0; r < kRecords; ++r) {
+            ro.hi[r] = (f16*)b.rec_hi[r];
+            ro.lo[r] = (int8_t*)b.rec_lo[r];
+            for (int q = 0; q < 4; ++q) ro.enc[r][q] = 1.0f / b.rec_scale[r][q];
+        }
+        ro.pitch = b.g.pitch;
+        ro.plane = b.g.plane;
+        hipLaunchKernelGGL(k_coef_records, grid, block, 0, st, v, m.nx, ro);
+        break;
+    }
     case PW_COMP16:
     case PW_BF16:
         hipLaunchKernelGGL(k_coef_scaled, grid, block, 0, st, v, m.nx, m.ny, m.nz, (float)m.dt, (float)scale_m,
