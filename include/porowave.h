@@ -424,7 +424,8 @@ typedef struct pw_server_stats {
 } pw_server_stats;
 
 /* max_active 16, round_steps 32, graphs on, batched, budget 0 (automatic),
- * fill_cells 2^25 (two 256^3 grids). */
+ * fill_cells 2^23 (half a 256^3 grid: a 256^3 shot runs alone, smaller shots
+ * share the device). */
 void pw_server_config_default(pw_server_config* cfg);
 pw_status pw_server_create(const pw_server_config* cfg, pw_server** out);
 /* Finishes the admitted shots, fails the queued ones, stops the worker. */

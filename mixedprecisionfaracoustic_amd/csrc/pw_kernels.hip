@@ -999,20 +999,24 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
                     txx_c[j] = txx_p[j]; txx_p[j] = txx_n[j];
                     tyy_c[j] = tyy_p[j]; tyy_p[j] = tyy_n[j];
                 }
+                seg_right(L, txx_c, txx_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_TXX, q); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
+                seg_left(L, txy_c, txy_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXY, q); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
+                seg_left(L, txz_c, txz_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXZ, q); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
+                seg_right(L, p_c, p_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_P, q); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
             } else {
                 LD2(PW_TXZ, PW_TYZ, o, txz_c, tyz_c);
                 LD2(PW_TZZ, PW_P, o + PL, tzz_n, p_n);
                 LD2(PW_TXX, PW_TYY, o, txx_c, tyy_c);
+                seg_right(L, txx_c, txx_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_TXX, q); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
                 LD(PW_TYY, o + W, tyy_q);
                 LD(PW_TXY, o, txy_c);
+                seg_left(L, txy_c, txy_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXY, q); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
                 LD(PW_TXY, o - W, txy_h);
+                seg_left(L, txz_c, txz_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXZ, q); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
                 LD(PW_TYZ, o - W, tyz_h);
+                seg_right(L, p_c, p_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_P, q); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
                 LD(PW_P, o + W, p_q);
             }
-            seg_right(L, txx_c, txx_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_TXX, q); }, [&](R (&r)[N]) { LD(PW_TXX, o + 1, r); });
-            seg_left(L, txy_c, txy_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXY, q); }, [&](R (&r)[N]) { LD(PW_TXY, o - 1, r); });
-            seg_left(L, txz_c, txz_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_TXZ, q); }, [&](R (&r)[N]) { LD(PW_TXZ, o - 1, r); });
-            seg_right(L, p_c, p_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_P, q); }, [&](R (&r)[N]) { LD(PW_P, o + 1, r); });
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 dx1[j] = txx_r[j] - txx_c[j]; dx2[j] = txy_c[j] - txy_h[j]; dx3[j] = txz_c[j] - txz_m[j];
@@ -1074,9 +1078,15 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         // per plane.  (Hoisting the coefficients as well costs a wave per SIMD in
         // registers and measured slower.)
         R wx0[N], wy0[N], wz0[N], ux0[N], uy0[N], uz0[N];
-        LD2(PW_VWX, PW_VWZ, o, wx0, wz0);
-        LD2(PW_VUY, PW_VWY, o, uy0, wy0);
-        LD2(PW_VUX, PW_VUZ, o, ux0, uz0);
+        if constexpr (PAIRS) {
+            LD2(PW_VWX, PW_VWZ, o, wx0, wz0);
+            LD2(PW_VUY, PW_VWY, o, uy0, wy0);
+            LD2(PW_VUX, PW_VUZ, o, ux0, uz0);
+        } else {
+            LD2(PW_VWX, PW_VWY, o, wx0, wy0);
+            LD2(PW_VUZ, PW_VWZ, o, uz0, wz0);
+            LD2(PW_VUX, PW_VUY, o, ux0, uy0);
+        }
         const int zs = slab_index(iz, a.nz, a.lo[2], a.hi[2]);
         Tab4<R> tz = {0, 1, 0, 1};
         if (zs >= 0) tz = a.tab[2][iz];
@@ -1223,9 +1233,15 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
                 uz1[j] = uz0[j] + ri[j] * sz[j] - fr[j] * (wz1[j] - wz0[j]);
             }
         }
-        ST2(PW_VWX, PW_VWZ, o, wx1, wz1);
-        ST2(PW_VUY, PW_VWY, o, uy1, wy1);
-        ST2(PW_VUX, PW_VUZ, o, ux1, uz1);
+        if constexpr (PAIRS) {
+            ST2(PW_VWX, PW_VWZ, o, wx1, wz1);
+            ST2(PW_VUY, PW_VWY, o, uy1, wy1);
+            ST2(PW_VUX, PW_VUZ, o, ux1, uz1);
+        } else {
+            ST2(PW_VWX, PW_VWY, o, wx1, wy1);
+            ST2(PW_VUZ, PW_VWZ, o, uz1, wz1);
+            ST2(PW_VUX, PW_VUY, o, ux1, uy1);
+        }
 
         if constexpr (ORD == 2) {
 #pragma unroll
@@ -1391,19 +1407,23 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                     vuz_c[j] = vuz_p[j]; vuz_p[j] = vuz_n[j];
                     vwy_c[j] = vwy_p[j]; vwy_p[j] = vwy_n[j];
                 }
+                seg_left(L, vux_c, vux_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VUX, q); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
+                seg_right(L, vuy_c, vuy_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUY, q); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
+                seg_right(L, vuz_c, vuz_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUZ, q); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
+                seg_left(L, vwx_c, vwx_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VWX, q); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
             } else {
                 LD2(PW_VUX, PW_VUY, o + PL, vux_n, vuy_n);
                 LD2(PW_VUZ, PW_VWZ, o, vuz_c, vwz_c);
+                seg_left(L, vux_c, vux_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VUX, q); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
                 LD(PW_VUX, o + W, vux_q);
                 LD(PW_VUY, o - W, vuy_h);
+                seg_right(L, vuy_c, vuy_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUY, q); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
+                seg_right(L, vuz_c, vuz_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUZ, q); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
                 LD(PW_VUZ, o + W, vuz_q);
                 LD2(PW_VWX, PW_VWY, o, vwx_c, vwy_c);
+                seg_left(L, vwx_c, vwx_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VWX, q); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
                 LD(PW_VWY, o - W, vwy_h);
             }
-            seg_left(L, vux_c, vux_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VUX, q); }, [&](R (&r)[N]) { LD(PW_VUX, o - 1, r); });
-            seg_right(L, vuy_c, vuy_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUY, q); }, [&](R (&r)[N]) { LD(PW_VUY, o + 1, r); });
-            seg_right(L, vuz_c, vuz_r, o + N, [&](unsigned q) { return FA<P>::ld(a, PW_VUZ, q); }, [&](R (&r)[N]) { LD(PW_VUZ, o + 1, r); });
-            seg_left(L, vwx_c, vwx_l, o - 1, [&](unsigned q) { return FA<P>::ld(a, PW_VWX, q); }, [&](R (&r)[N]) { LD(PW_VWX, o - 1, r); });
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 dxx[j] = vux_c[j] - vux_l[j]; dyy[j] = vuy_c[j] - vuy_h[j]; dzz[j] = vuz_c[j] - vuz_m[j];
@@ -1458,10 +1478,17 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         if (zs >= 0) tz = a.tab[2][iz];
 
         R p0[N], xx0[N], yy0[N], zz0[N], xy0[N], yz0[N], xz0[N];
-        LD2(PW_TYY, PW_P, o, yy0, p0);
-        LD2(PW_TXY, PW_TYZ, o, xy0, yz0);
-        LD2(PW_TXX, PW_TZZ, o, xx0, zz0);
-        LD(PW_TXZ, o, xz0);
+        if constexpr (PAIRS) {
+            LD2(PW_TYY, PW_P, o, yy0, p0);
+            LD2(PW_TXY, PW_TYZ, o, xy0, yz0);
+            LD2(PW_TXX, PW_TZZ, o, xx0, zz0);
+            LD(PW_TXZ, o, xz0);
+        } else {
+            LD2(PW_TZZ, PW_P, o, zz0, p0);
+            LD2(PW_TXX, PW_TYY, o, xx0, yy0);
+            LD(PW_TXY, o, xy0);
+            LD2(PW_TXZ, PW_TYZ, o, xz0, yz0);
+        }
         R p1[N], xx1[N], yy1[N], zz1[N], xy1[N], yz1[N], xz1[N];
         R mu_n[N], mu_xn[N], mu_yn[N], mu_xy[N], mv_n[N];
         R C_[N], M_[N], HH_[N], H2u_[N], mxy[N], myz[N], mxz[N];
@@ -1575,10 +1602,17 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                 xz1[j] = xz0[j] + mxz[j] * (uxz + uzx);
             }
         }
-        ST2(PW_TYY, PW_P, o, yy1, p1);
-        ST2(PW_TXY, PW_TYZ, o, xy1, yz1);
-        ST2(PW_TXX, PW_TZZ, o, xx1, zz1);
-        ST(PW_TXZ, o, xz1);
+        if constexpr (PAIRS) {
+            ST2(PW_TYY, PW_P, o, yy1, p1);
+            ST2(PW_TXY, PW_TYZ, o, xy1, yz1);
+            ST2(PW_TXX, PW_TZZ, o, xx1, zz1);
+            ST(PW_TXZ, o, xz1);
+        } else {
+            ST2(PW_TZZ, PW_P, o, zz1, p1);
+            ST2(PW_TXX, PW_TYY, o, xx1, yy1);
+            ST(PW_TXY, o, xy1);
+            ST2(PW_TXZ, PW_TYZ, o, xz1, yz1);
+        }
 
         if constexpr (ORD == 2) {
 #pragma unroll

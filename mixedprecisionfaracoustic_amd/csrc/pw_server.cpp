@@ -473,7 +473,7 @@ extern "C" void pw_server_config_default(pw_server_config* c)
     c->round_steps = 32;
     c->use_graphs = 1;
     c->batched = 1;
-    c->fill_cells = 1LL << 25;
+    c->fill_cells = 1LL << 23;
 }
 
 extern "C" pw_status pw_server_create(const pw_server_config* cfg, pw_server** out)

@@ -156,6 +156,22 @@ def test_memory_budget_limits_admission(volume, alone):
     assert stats["peak_active"] == 1
 
 
+def test_cell_limit_limits_admission(volume, alone):
+    """fill_cells: shots are admitted while the admitted shots hold at most that
+    many grid cells (one is always admitted when none runs).  A limit of the
+    two largest shots' cells lets two run together, never three; results are
+    unchanged."""
+    shots, ref = alone
+    cells = sorted(q["nx"] * q["ny"] * q["nz"] for q in shots)
+    res, st, stats = run_server(volume, shots, max_active=8, round_steps=8, fill_cells=cells[-1] + cells[-2])
+    assert stats["peak_active"] == 2
+    for a, b in zip(res, ref):
+        assert np.array_equal(a, b)
+    res, st, stats = run_server(volume, shots, max_active=8, round_steps=8, fill_cells=1)
+    assert stats["peak_active"] == 1
+    assert [s["t_admit"] for s in st] == sorted(s["t_admit"] for s in st)
+
+
 def test_failing_shots_do_not_disturb_others(volume, alone):
     shots, ref = alone
     with pw.Server(max_active=8, hbm_budget=1 << 30) as srv:
@@ -234,7 +250,8 @@ def test_server_at_trace_grid_sizes_matches_solver():
         idx = [int(x) + (n // 2) * n + (np_ + 10) * n * n for x in xs] * 2
         shots.append(dict(n=n, nt=nt, crop=crop, src=src, prec="fp32" if k % 2 == 0 else "comp16",
                           fields=["txx"] * 32 + ["vux"] * 32, idx=idx))
-    with pw.Server(max_active=4, round_steps=32) as srv:
+    # no cell limit: all four sizes advance in the same batched launches
+    with pw.Server(max_active=4, round_steps=32, fill_cells=0) as srv:
         srv.set_medium(vol)
         ids = [srv.submit(q["n"], q["n"], q["n"], np_, H, DT, q["nt"], q["prec"], q["crop"], q["src"], 25.0,
                           q["fields"], q["idx"], weights=(1.0, 1.0, 1.0, -1.0), cpml_r=1e-4, cpml_v0=4000.0)
