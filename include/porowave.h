@@ -56,8 +56,9 @@ typedef enum {
     PW_COMP16F = 4 /* the comp16 state (FP16 value + int8 residual, residual rounded to
                       nearest; stresses stored times 2^(es+6), velocities 2^(es+6-em) with
                       em of Half/main.cu:793-807) under an FP32 update of the Float
-                      program's equations: FP32 cell-centred coefficients, FP32 memory
-                      variables, hardware reciprocals */
+                      program's equations: cell-centred coefficients held as
+                      compensated values (FP16 + int8, a power-of-two scale each),
+                      FP32 memory variables, hardware reciprocals */
 } pw_precision;
 
 /* Wavefield ids for receivers, snapshots and pw_get_field. */
@@ -159,9 +160,10 @@ pw_status pw_set_medium(pw_solver* s, const double* const* base, int nxi, int ny
  * forbid hipMalloc.  d_base: PW_NBASE device pointers, on the handle's device,
  * to the cell-centred interior fields (nxi*nyi*nzi doubles each, x fastest), as
  * pw_set_medium takes them in host memory; the installed medium is
- * bit-identical.  The medium's maxima (stability check, Half exponents) are
- * reduced into a file-scope device array of 64 doubles instead of an
- * allocation; the reduction needs 4 of them whatever the grid size.  Every call
+ * bit-identical.  The medium's maxima (stability check, Half exponents, comp16f
+ * coefficient scales) are reduced into a file-scope device array of 64 doubles
+ * instead of an allocation; the reduction needs 13 of them whatever the grid
+ * size.  Every call
  * on a device shares that device's array and clears it first; calls in one
  * process are serialised on it (calls that overlapped on one device would
  * race).  Refuses a handle with cpml_v0 <= 0 (PW_ERR_UNSUPPORTED): its CPML

@@ -223,10 +223,6 @@ pw_status check_config(const pw_config& c)
     // its FP16 values: 2 bytes per cell of values, then 1 per cell
     if (has_lo(c.precision) && 3ull * pitch * c.ny * c.nz >= (1ull << 32) - (1ull << 20))
         return fail(PW_ERR_UNSUPPORTED, "pw_create: comp16 grid exceeds 2^32 / 3 padded cells (32-bit offsets)");
-    // comp16f: a pair's residuals lie 4 bytes per cell past its values, and a
-    // coefficient record's values are 4 FP16 elements per cell
-    if (c.precision == PW_COMP16F && 4ull * pitch * c.ny * c.nz >= (1ull << 32) - (1ull << 20))
-        return fail(PW_ERR_UNSUPPORTED, "pw_create: comp16f grid exceeds 2^30 padded cells (32-bit offsets)");
     return PW_OK;
 }
 

@@ -250,8 +250,9 @@ template <class P> struct StepArgs {
     typename P::R dt, h;
     // compensated fields: bytes from a field's FP16 values to its residual bytes
     // (the same for every field; comp16f: for the slot-0 field of every pair,
-    // and loff1 for the field stored alone)
-    unsigned loff, loff1;
+    // and loff1 for the field stored alone; 64-bit: comp16f's pairs put their
+    // residuals 4 bytes per cell past the values)
+    unsigned long long loff, loff1;
     // kFormFast launch constants (KF_*): dt/h, the factors of the staggered
     // means and the class storage scales folded together
     float kf[9];
@@ -317,8 +318,9 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     a.h = (R)b.h;
     // the pool stores each compensated field's residual bytes right after its
     // FP16 values, at one offset for all fields
-    a.loff = b.lo[0] ? (unsigned)((const char*)b.lo[0] - (const char*)b.f[0]) : 0u;
-    a.loff1 = b.lo[kC16FSingle] ? (unsigned)((const char*)b.lo[kC16FSingle] - (const char*)b.f[kC16FSingle]) : 0u;
+    a.loff = b.lo[0] ? (unsigned long long)((const char*)b.lo[0] - (const char*)b.f[0]) : 0ull;
+    a.loff1 = b.lo[kC16FSingle] ? (unsigned long long)((const char*)b.lo[kC16FSingle] - (const char*)b.f[kC16FSingle])
+                                : 0ull;
     const double g = b.dt / b.h, su = b.sc_vel, sw = b.sc_vw, stt = b.sc_stress;
     a.kf[KF_DT2] = (float)(0.5 * b.dt);
     a.kf[KF_WS] = (float)(4.0 * g * sw / stt);
@@ -390,13 +392,13 @@ template <> struct FA<PolC16> {
     static __device__ __forceinline__ R ld(const StepArgs<P>& a, int fid, unsigned o)
     {
         const f16* h = a.f[fid].hi;
-        return c16_decode(h[o], ((const int8_t*)h)[a.loff + o]);
+        return c16_decode(h[o], ((const int8_t*)h)[(unsigned)a.loff + o]);
     }
     template <int N> static __device__ __forceinline__ void ldv(const StepArgs<P>& a, int fid, unsigned o, R (&r)[N])
     {
         const f16* h = a.f[fid].hi;
         const Pack<f16, N> hv = ld_pack<N>(h, o);
-        const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h, a.loff + o);
+        const Pack<int8_t, N> lv = ld_pack<N>((const int8_t*)h, (unsigned)a.loff + o);
 #pragma unroll
         for (int j = 0; j < N; ++j) r[j] = c16_decode(hv.v[j], lv.v[j]);
     }
@@ -407,7 +409,7 @@ template <> struct FA<PolC16> {
         int8_t l;
         c16_encode(v, hh, l, a.zero);
         h[o] = hh;
-        ((int8_t*)h)[a.loff + o] = l;
+        ((int8_t*)h)[(unsigned)a.loff + o] = l;
     }
     template <int N>
     static __device__ __forceinline__ void stv(const StepArgs<P>& a, int fid, unsigned o, const R (&r)[N])
@@ -418,7 +420,7 @@ template <> struct FA<PolC16> {
 #pragma unroll
         for (int j = 0; j < N; ++j) c16_encode(r[j], hv.v[j], lv.v[j], a.zero);
         st_pack<N>(h, o, hv);
-        st_pack<N>((int8_t*)h, a.loff + o, lv);
+        st_pack<N>((int8_t*)h, (unsigned)a.loff + o, lv);
     }
 };
 // comp16f, in stored units (the class scales are folded into the launch

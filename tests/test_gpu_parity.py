@@ -182,7 +182,8 @@ def test_comp16f_matches_fp64_within_1e3(case, borehole, layered):
     """comp16f: the compensated state (FP16 + int8 residual, stresses stored
     times 2^6, velocities on the Half exponent, the relative fluid velocity on
     its own scale) under this build's FP32 form of the Float update (undivided
-    differences, hardware reciprocals, FP32 cell-centred coefficients).  It has
+    differences, hardware reciprocals, the cell-centred coefficients held as
+    compensated values, FP32 memory variables).  It has
     no oracle of its own, so it is held to the FP64 oracle: every seismogram
     channel <= 1e-3 (or 2x the Float path's own distance to FP64 where larger).
     Full fields: <= max(1e-3, 64 x that distance): the relative fluid velocity vw
