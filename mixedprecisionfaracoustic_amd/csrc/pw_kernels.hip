@@ -815,7 +815,8 @@ __device__ __forceinline__ void seg_left(const Lanes& L, const R (&c)[N], R (&r)
 // them: with moves, FP64 (two moves per value) measured 3 % slower, and the
 // FP16- and BF16-stored fields, whose neighbours must first be decoded in every
 // lane, 12 % (comp16) and 3 % (BF16) slower.
-template <class P> constexpr bool kX4Dpp = PW_X_DPP && std::is_same<P, PolF32>::value;
+template <class P>
+constexpr bool kX4Dpp = PW_X_DPP && (std::is_same<P, PolF32>::value || std::is_same<P, PolC16F>::value);
 
 // 4th order, one cell per lane: a field's values one and two cells along x
 // from the lane's centre value c (o = the lane's offset).  The cell one out is
@@ -940,7 +941,8 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
     R txz_mm[N], tyz_mm[N], txz_k[N], tyz_k[N], tzz_m[N], p_m[N], tzz_k[N], p_k[N];
     // comp16f, order 2: its pairs (tyy, p) and (txx, tzz) are read a plane ahead,
     // so txx and tyy are carried too
-    constexpr bool PAIRS = std::is_same<P, PolC16F>::value && ORD == 2;
+    constexpr bool C16F = std::is_same<P, PolC16F>::value;
+    constexpr bool PAIRS = C16F && ORD == 2;
     R txx_p[N], tyy_p[N];
     if constexpr (PAIRS) {
         R txy_m[N];
@@ -1057,17 +1059,39 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
                 txz_l1 = F(PW_TXZ, -X); txz_l2 = F(PW_TXZ, -2 * X); txz_r1 = F(PW_TXZ, X);
             }
             dx1[0] = C4A * (txx_r1 - txx0) + C4B * (txx_r2 - txx_l1);
-            dx2[0] = C4A * (txy0 - F(PW_TXY, -Y)) + C4B * (F(PW_TXY, Y) - F(PW_TXY, -2 * Y));
             dx3[0] = C4A * (txz_k[0] - txz_m[0]) + C4B * (txz_p - txz_mm[0]);
             ds1[0] = C4A * (p_r1 - p_c[0]) + C4B * (p_r2 - p_l1);
-            dy1[0] = C4A * (F(PW_TYY, Y) - tyy0) + C4B * (F(PW_TYY, 2 * Y) - F(PW_TYY, -Y));
             dy2[0] = C4A * (txy0 - txy_l1) + C4B * (txy_r1 - txy_l2);
             dy3[0] = C4A * (tyz_k[0] - tyz_m[0]) + C4B * (tyz_p - tyz_mm[0]);
-            ds2[0] = C4A * (F(PW_P, Y) - p_c[0]) + C4B * (F(PW_P, 2 * Y) - F(PW_P, -Y));
             dz1[0] = C4A * (tzz_k[0] - tzz_c[0]) + C4B * (tzz_nn - tzz_m[0]);
             dz2[0] = C4A * (txz_k[0] - txz_l1) + C4B * (txz_r1 - txz_l2);
-            dz3[0] = C4A * (tyz_k[0] - F(PW_TYZ, -Y)) + C4B * (F(PW_TYZ, Y) - F(PW_TYZ, -2 * Y));
             ds3[0] = C4A * (p_k[0] - p_c[0]) + C4B * (p_nn - p_m[0]);
+            if constexpr (std::is_same<P, PolC16F>::value) {
+                // the y neighbours in comp16f's pairs: (txy, tyz) and (tyy, p) at
+                // the same rows, one paired access each
+                auto F2 = [&](int fa, int fb, long d, R& ra, R& rb) {
+                    R x[1], y[1];
+                    ld2v<P, 1>(a, fa, fb, (unsigned)((long)o + d), x, y);
+                    ra = x[0];
+                    rb = y[0];
+                };
+                R txy_ym, txy_yp, txy_ym2, tyz_ym, tyz_yp, tyz_ym2, tyy_yp, tyy_yp2, tyy_ym, p_yp, p_yp2, p_ym;
+                F2(PW_TXY, PW_TYZ, -Y, txy_ym, tyz_ym);
+                F2(PW_TXY, PW_TYZ, Y, txy_yp, tyz_yp);
+                F2(PW_TXY, PW_TYZ, -2 * Y, txy_ym2, tyz_ym2);
+                F2(PW_TYY, PW_P, Y, tyy_yp, p_yp);
+                F2(PW_TYY, PW_P, 2 * Y, tyy_yp2, p_yp2);
+                F2(PW_TYY, PW_P, -Y, tyy_ym, p_ym);
+                dx2[0] = C4A * (txy0 - txy_ym) + C4B * (txy_yp - txy_ym2);
+                dy1[0] = C4A * (tyy_yp - tyy0) + C4B * (tyy_yp2 - tyy_ym);
+                ds2[0] = C4A * (p_yp - p_c[0]) + C4B * (p_yp2 - p_ym);
+                dz3[0] = C4A * (tyz_k[0] - tyz_ym) + C4B * (tyz_yp - tyz_ym2);
+            } else {
+                dx2[0] = C4A * (txy0 - F(PW_TXY, -Y)) + C4B * (F(PW_TXY, Y) - F(PW_TXY, -2 * Y));
+                dy1[0] = C4A * (F(PW_TYY, Y) - tyy0) + C4B * (F(PW_TYY, 2 * Y) - F(PW_TYY, -Y));
+                ds2[0] = C4A * (F(PW_P, Y) - p_c[0]) + C4B * (F(PW_P, 2 * Y) - F(PW_P, -Y));
+                dz3[0] = C4A * (tyz_k[0] - F(PW_TYZ, -Y)) + C4B * (F(PW_TYZ, Y) - F(PW_TYZ, -2 * Y));
+            }
             // shift the z carries to the next plane
             txz_mm[0] = txz_m[0]; txz_m[0] = txz_k[0]; txz_k[0] = txz_p;
             tyz_mm[0] = tyz_m[0]; tyz_m[0] = tyz_k[0]; tyz_k[0] = tyz_p;
@@ -1080,7 +1104,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
         // per plane.  (Hoisting the coefficients as well costs a wave per SIMD in
         // registers and measured slower.)
         R wx0[N], wy0[N], wz0[N], ux0[N], uy0[N], uz0[N];
-        if constexpr (PAIRS) {
+        if constexpr (C16F) {
             LD2(PW_VWX, PW_VWZ, o, wx0, wz0);
             LD2(PW_VUY, PW_VWY, o, uy0, wy0);
             LD2(PW_VUX, PW_VUZ, o, ux0, uz0);
@@ -1235,7 +1259,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
                 uz1[j] = uz0[j] + ri[j] * sz[j] - fr[j] * (wz1[j] - wz0[j]);
             }
         }
-        if constexpr (PAIRS) {
+        if constexpr (C16F) {
             ST2(PW_VWX, PW_VWZ, o, wx1, wz1);
             ST2(PW_VUY, PW_VWY, o, uy1, wy1);
             ST2(PW_VUX, PW_VUZ, o, ux1, uz1);
@@ -1325,7 +1349,8 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
     R vuz_mm[N], vwz_mm[N], vuz_k[N], vwz_k[N], vux_m[N], vuy_m[N], vux_k[N], vuy_k[N];
     // comp16f, order 2: its pairs (vux, vuz) and (vuy, vwy) are read a plane
     // ahead, so vuz and vwy are carried too
-    constexpr bool PAIRS = std::is_same<P, PolC16F>::value && ORD == 2;
+    constexpr bool C16F = std::is_same<P, PolC16F>::value;
+    constexpr bool PAIRS = C16F && ORD == 2;
     R vuz_p[N], vwy_p[N];
     if constexpr (PAIRS) {
         R vux_m1[N], vwx_m1[N];
@@ -1459,17 +1484,39 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                 vuz_r1 = F(PW_VUZ, X); vuz_r2 = F(PW_VUZ, 2 * X); vuz_l1 = F(PW_VUZ, -X);
             }
             dxx[0] = C4A * (vux_c[0] - vux_l1) + C4B * (vux_r1 - vux_l2);
-            dyy[0] = C4A * (vuy_c[0] - F(PW_VUY, -Y)) + C4B * (F(PW_VUY, Y) - F(PW_VUY, -2 * Y));
             dzz[0] = C4A * (vuz_k[0] - vuz_m[0]) + C4B * (vuz_p - vuz_mm[0]);
             ewx[0] = C4A * (vwx0 - vwx_l1) + C4B * (vwx_r1 - vwx_l2);
-            ewy[0] = C4A * (vwy0 - F(PW_VWY, -Y)) + C4B * (F(PW_VWY, Y) - F(PW_VWY, -2 * Y));
             ewz[0] = C4A * (vwz_k[0] - vwz_m[0]) + C4B * (vwz_p - vwz_mm[0]);
-            dxy[0] = C4A * (F(PW_VUX, Y) - vux_c[0]) + C4B * (F(PW_VUX, 2 * Y) - F(PW_VUX, -Y));
             dyx[0] = C4A * (vuy_r1 - vuy_c[0]) + C4B * (vuy_r2 - vuy_l1);
             dxz[0] = C4A * (vux_k[0] - vux_c[0]) + C4B * (vux_nn - vux_m[0]);
             dzx[0] = C4A * (vuz_r1 - vuz_k[0]) + C4B * (vuz_r2 - vuz_l1);
             dyz[0] = C4A * (vuy_k[0] - vuy_c[0]) + C4B * (vuy_nn - vuy_m[0]);
-            dzy[0] = C4A * (F(PW_VUZ, Y) - vuz_k[0]) + C4B * (F(PW_VUZ, 2 * Y) - F(PW_VUZ, -Y));
+            if constexpr (std::is_same<P, PolC16F>::value) {
+                // the y neighbours in comp16f's pairs: (vuy, vwy) and (vux, vuz) at
+                // the same rows, one paired access each
+                auto F2 = [&](int fa, int fb, long d, R& ra, R& rb) {
+                    R x[1], y[1];
+                    ld2v<P, 1>(a, fa, fb, (unsigned)((long)o + d), x, y);
+                    ra = x[0];
+                    rb = y[0];
+                };
+                R vuy_ym, vuy_yp, vuy_ym2, vwy_ym, vwy_yp, vwy_ym2, vux_yp, vux_yp2, vux_ym, vuz_yp, vuz_yp2, vuz_ym;
+                F2(PW_VUY, PW_VWY, -Y, vuy_ym, vwy_ym);
+                F2(PW_VUY, PW_VWY, Y, vuy_yp, vwy_yp);
+                F2(PW_VUY, PW_VWY, -2 * Y, vuy_ym2, vwy_ym2);
+                F2(PW_VUX, PW_VUZ, Y, vux_yp, vuz_yp);
+                F2(PW_VUX, PW_VUZ, 2 * Y, vux_yp2, vuz_yp2);
+                F2(PW_VUX, PW_VUZ, -Y, vux_ym, vuz_ym);
+                dyy[0] = C4A * (vuy_c[0] - vuy_ym) + C4B * (vuy_yp - vuy_ym2);
+                ewy[0] = C4A * (vwy0 - vwy_ym) + C4B * (vwy_yp - vwy_ym2);
+                dxy[0] = C4A * (vux_yp - vux_c[0]) + C4B * (vux_yp2 - vux_ym);
+                dzy[0] = C4A * (vuz_yp - vuz_k[0]) + C4B * (vuz_yp2 - vuz_ym);
+            } else {
+                dyy[0] = C4A * (vuy_c[0] - F(PW_VUY, -Y)) + C4B * (F(PW_VUY, Y) - F(PW_VUY, -2 * Y));
+                ewy[0] = C4A * (vwy0 - F(PW_VWY, -Y)) + C4B * (F(PW_VWY, Y) - F(PW_VWY, -2 * Y));
+                dxy[0] = C4A * (F(PW_VUX, Y) - vux_c[0]) + C4B * (F(PW_VUX, 2 * Y) - F(PW_VUX, -Y));
+                dzy[0] = C4A * (F(PW_VUZ, Y) - vuz_k[0]) + C4B * (F(PW_VUZ, 2 * Y) - F(PW_VUZ, -Y));
+            }
             vuz_mm[0] = vuz_m[0]; vuz_m[0] = vuz_k[0]; vuz_k[0] = vuz_p;
             vwz_mm[0] = vwz_m[0]; vwz_m[0] = vwz_k[0]; vwz_k[0] = vwz_p;
             vux_m[0] = vux_c[0]; vux_c[0] = vux_k[0]; vux_k[0] = vux_nn;
@@ -1480,7 +1527,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
         if (zs >= 0) tz = a.tab[2][iz];
 
         R p0[N], xx0[N], yy0[N], zz0[N], xy0[N], yz0[N], xz0[N];
-        if constexpr (PAIRS) {
+        if constexpr (C16F) {
             LD2(PW_TYY, PW_P, o, yy0, p0);
             LD2(PW_TXY, PW_TYZ, o, xy0, yz0);
             LD2(PW_TXX, PW_TZZ, o, xx0, zz0);
@@ -1604,7 +1651,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                 xz1[j] = xz0[j] + mxz[j] * (uxz + uzx);
             }
         }
-        if constexpr (PAIRS) {
+        if constexpr (C16F) {
             ST2(PW_TYY, PW_P, o, yy1, p1);
             ST2(PW_TXY, PW_TYZ, o, xy1, yz1);
             ST2(PW_TXX, PW_TZZ, o, xx1, zz1);

@@ -119,7 +119,7 @@ def test_ragged_tilings_are_bit_identical(shape, np_, src):
 
 
 @pytest.mark.parametrize("prec,order", [("fp32", 2), ("fp32", 4), ("fp64", 4), ("comp16", 2), ("comp16", 4),
-                                        ("comp16f", 2), ("bf16", 2)])
+                                        ("comp16f", 2), ("comp16f", 4), ("bf16", 2)])
 def test_mirror_symmetry(prec, order):
     """Homogeneous sandstone on a 131 x 99 x 99 grid, explosive source in the
     middle cell (65, 49, 49): every field is even or odd about the middle plane

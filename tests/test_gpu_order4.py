@@ -82,7 +82,7 @@ def test_order4_reduced_precisions(case4):
     grid, base, nt, recs, ref32, ref64 = case4
     g64 = run_gpu4(grid, nt, "fp64", recs, base)
     ref = {"seis": g64["seis"]}
-    for prec, tol in (("comp16", 5e-3), ("bf16", 5e-2)):
+    for prec, tol in (("comp16", 5e-3), ("comp16f", 1e-3), ("bf16", 5e-2)):
         g = run_gpu4(grid, nt, prec, recs, base)
         e = errs(g, ref, recs, final=False)
         print(f"order4 {prec} vs FP64:", " ".join(f"{k}={v:.2e}" for k, v in e.items()))
@@ -95,7 +95,7 @@ def test_order4_tilings_are_bit_identical(case4):
     puts two waves side by side by default): one or two waves per tile row, and
     64 x 2 or 64 x 8 tiles, give the same bits."""
     grid, base, nt, recs, _, _ = case4
-    for prec in ("bf16", "fp32", "comp16"):
+    for prec in ("bf16", "fp32", "comp16", "comp16f"):
         outs = []
         for t in ({}, {"waves_x": 1}, {"waves_x": 2}, {"rows": 8}, {"rows": 2, "waves_x": 2}):
             outs.append((t, run_gpu4(dict(grid, tiling=t), nt, prec, recs, base)))
