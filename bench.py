@@ -198,7 +198,8 @@ def serving_main(args):
     trace = serving_trace(args.shots, args.rate, args.seed, n_vol, np_)
     print(f"serving: {len(trace)} shots over {trace[-1]['t']:.1f} s of arrivals", file=sys.stderr, flush=True)
     batched, seis_b = run_serving(trace, n_vol, np_, h, dt, spec, max_active=args.slots, round_steps=args.round_steps,
-                                  use_graphs=not args.no_graphs, batched=True, fill_cells=args.fill_cells)
+                                  use_graphs=not args.no_graphs, batched=True, fill_cells=args.fill_cells,
+                                  persistent=args.persistent)
     print("  batched:", json.dumps(batched), file=sys.stderr, flush=True)
     seq, seis_s = run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=True, max_active=1, round_steps=64,
                               use_graphs=False, batched=False)
@@ -215,7 +216,8 @@ def serving_main(args):
            "config": {"workload": "shot_serving_trace", "shots": args.shots, "rate_per_s": args.rate,
                       "seed": args.seed, "grids": [96, 128, 192, 256], "steps": [500, 1500],
                       "precisions": "fp32/comp16f 50/50", "slots": args.slots, "round_steps": args.round_steps,
-                      "graphs": not args.no_graphs, "fill_cells": args.fill_cells},
+                      "graphs": not args.no_graphs, "fill_cells": args.fill_cells,
+                      "persistent": args.persistent},
            "serving": batched, "sequential_fp32": seq,
            "throughput_ratio": round(batched["gcell_steps_per_s"] / seq["gcell_steps_per_s"], 3),
            "p50_ratio": round(batched["p50_latency_s"] / seq["p50_latency_s"], 3),
@@ -606,6 +608,8 @@ def main():
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--round-steps", type=int, default=32)
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--persistent", action="store_true",
+                    help="--serving: the server runs each round as one persistent cooperative launch")
     ap.add_argument("--fill-cells", type=int, default=None,
                     help="server admission limit in grid cells (default: the library's)")
     ap.add_argument("--volume-every", type=int, default=0,

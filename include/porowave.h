@@ -353,7 +353,8 @@ pw_status pw_get_base_fields(pw_solver* s, double* out);
  * and the slot count allow; it keeps every admitted shot resident; and it
  * advances all of them together, in rounds of steps:
  *  - one launch per step kernel covers every active shot of a precision;
- *  - a round that repeats is captured once as a hipGraph and replayed.
+ *  - a round that repeats is captured once as a hipGraph and replayed;
+ *  - or (persistent) one cooperative launch per precision runs the whole round.
  * Shots join and leave between rounds.  A shot's seismogram is bit-identical
  * to the same shot run alone through pw_run, because the per-cell code is the
  * same.  All calls are thread-safe; one worker thread owns the device.
@@ -378,6 +379,10 @@ typedef struct pw_server_config {
                                first served, which keeps latency at the sequential
                                level under load, while small shots still share the
                                device.  0: no limit besides max_active and the budget */
+    int persistent;         /* 1: each round of a batch runs as ONE persistent cooperative
+                               launch (as many blocks as the device holds at once, grid
+                               barriers between the phases of a step) instead of one launch
+                               per step kernel; results are bit-identical.  Needs batched */
 } pw_server_config;
 
 typedef struct pw_shot {
@@ -427,7 +432,7 @@ typedef struct pw_server_stats {
 
 /* max_active 16, round_steps 32, graphs on, batched, budget 0 (automatic),
  * fill_cells 2^23 (half a 256^3 grid: a 256^3 shot runs alone, smaller shots
- * share the device). */
+ * share the device), persistent off. */
 void pw_server_config_default(pw_server_config* cfg);
 pw_status pw_server_create(const pw_server_config* cfg, pw_server** out);
 /* Finishes the admitted shots, fails the queued ones, stops the worker. */

@@ -228,6 +228,10 @@ void batch_free(BatchGroup& g);
 // round offset k, and the end-of-round advance it[s] += K.
 hipError_t batch_step(const BatchGroup& g, hipStream_t st, int k);
 hipError_t batch_advance(const BatchGroup& g, hipStream_t st, int K);
+// K steps of every shot of g in one persistent cooperative launch (grid barriers
+// between the phases of a step); results bit-identical to K batch_step calls.
+// The caller still calls batch_advance.
+hipError_t batch_round(const BatchGroup& g, hipStream_t st, int K);
 
 // ---- medium precompute (pw_medium_gpu.hip), all FP64 on the device ----
 struct MediumArgs {

@@ -35,6 +35,7 @@
 // (FP16 hi + int8 residual, Half/main.cu:49-100) and BF16 use the reference
 // Half formulas with FP16 prescaled coefficients.  All arithmetic is FP32
 // except the FP64 policy.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1882,15 +1883,12 @@ template <class P, int N> __global__ __launch_bounds__(512) void k_stress_batch(
     stress_tile<P, N, 2>(a, tile_of_index(t, a.gx, a.gy));
 }
 
-// one thread per (shot, source channel); the arithmetic of k_source
-template <class P> __global__ void k_source_batch(const BatchArgs c, int k)
+// Source channel ch (0..3: txx, tyy, tzz, p) of one shot at step it; the
+// arithmetic of k_source
+// (Dsc: ShotDesc<P>, or the same read through the constant address space)
+template <class P, class Dsc> __device__ __forceinline__ void source_channel(const Dsc& d, int it, int ch)
 {
     using R = typename P::R;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = g >> 2, ch = g & 3;
-    if (s >= c.n) return;
-    const ShotDesc<P>& d = ((const ShotDesc<P>*)c.descs)[s];
-    const int it = c.it[s] + k;
     if (it >= d.nt || !d.has_src || d.w[ch] == (R)0) return;
     const int fid[4] = {PW_TXX, PW_TYY, PW_TZZ, PW_P};
     const typename P::Field f = d.a.f[fid[ch]];
@@ -1898,12 +1896,31 @@ template <class P> __global__ void k_source_batch(const BatchArgs c, int k)
     P::st(f, d.src_off, P::ld(f, d.src_off) + d.w[ch] * amp);
 }
 
+// Receiver q of one shot at step it; the arithmetic of k_gather
+template <class P, class Dsc> __device__ __forceinline__ void gather_one(const Dsc& d, int it, unsigned q)
+{
+    if (it >= d.nt) return;
+    const int fid = d.rfield[q];
+    const float v = (float)ld_any<P>(d.a, fid, d.roff[q]);
+    const float s_stress = d.oscale[2 * it], s_vel = d.oscale[2 * it + 1];
+    d.seis[(size_t)q * d.nt + it] = (fid <= PW_VWZ ? s_vel : s_stress) * v;
+}
+
+// one thread per (shot, source channel)
+template <class P> __global__ void k_source_batch(const BatchArgs c, int k)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = g >> 2, ch = g & 3;
+    if (s >= c.n) return;
+    source_channel<P>(((const ShotDesc<P>*)c.descs)[s], c.it[s] + k, ch);
+}
+
 // Multiplies array arr (0..12 wavefields, then the memory variables) of one
 // shot by ratio over the indices i0, i0 + stride, ...  (k_rescale's arithmetic)
-template <class P>
-__device__ __forceinline__ void rescale_array(const StepArgs<P>& a, int arr, unsigned long long i0,
-                                              unsigned long long stride, unsigned long long ncell,
-                                              const unsigned long long nmem[3], float ratio)
+// (A: StepArgs<P>, or the same read through the constant address space)
+template <class P, class A>
+__device__ __forceinline__ void rescale_array(const A& a, int arr, unsigned long long i0, unsigned long long stride,
+                                              unsigned long long ncell, const unsigned long long nmem[3], float ratio)
 {
     if (arr < PW_NFIELDS) {
         const typename P::Field f = a.f[arr];
@@ -1932,26 +1949,157 @@ template <class P> __global__ void k_rescale_batch(const BatchArgs c, int k)
     for (int arr = 0; arr < PW_NFIELDS + 3 * kMemPerAxis; ++arr) rescale_array<P>(a, arr, i0, stride, d.ncell, d.nmem, ratio);
 }
 
-// one thread per receiver of the batch; the arithmetic of k_gather
+// one thread per receiver of the batch
 template <class P> __global__ void k_gather_batch(const BatchArgs c, int k)
 {
     const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= c.nrec) return;
     const int s = find_shot(c.rstart, c.n, r);
-    const ShotDesc<P>& d = ((const ShotDesc<P>*)c.descs)[s];
-    const int it = c.it[s] + k;
-    if (it >= d.nt) return;
-    const unsigned q = r - c.rstart[s];
-    const int fid = d.rfield[q];
-    const float v = (float)ld_any<P>(d.a, fid, d.roff[q]);
-    const float s_stress = d.oscale[2 * it], s_vel = d.oscale[2 * it + 1];
-    d.seis[(size_t)q * d.nt + it] = (fid <= PW_VWZ ? s_vel : s_stress) * v;
+    gather_one<P>(((const ShotDesc<P>*)c.descs)[s], c.it[s] + k, r - c.rstart[s]);
 }
 
 __global__ void k_advance(int* it, const int* __restrict__ nt, int n, int K)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n) it[s] = min(it[s] + K, nt[s]);
+}
+
+// --------------------------------------------------------------------------
+// Persistent round: ONE cooperative launch advances every shot of a batch by K
+// steps.  The grid is as many blocks as fit on the device at once; between the
+// phases of a step the whole grid meets at a grid barrier (cooperative groups),
+// which orders the phases as the separate launches of batch_step do:
+//   [receivers of step k-1, then the source of step k]  (per shot, one block)
+//   velocity tiles of step k  |  stress tiles of step k
+// A step at which some shot rescales (comp16, BF16) takes the receivers, the
+// rescale and the source as three phases instead.  Each block walks the tiles
+// of its XCD's share in turn (tiles j0, j0 + blocks per XCD, ...), so the
+// blocks of one XCD sweep a contiguous window of its tiles as the batched
+// launches do.  Every block passes the same barriers: the loop bounds and the
+// rescale test depend only on kernel arguments and on tables nobody writes
+// during the launch.  The per-cell code is that of the other step kernels, so
+// results are bit-identical to them.
+// --------------------------------------------------------------------------
+// Tables nobody writes during a launch, read through the constant address
+// space: the compiler then loads them into scalar registers.  Read as ordinary
+// memory after the stores of a tile loop they came through vector loads, and
+// the descriptor's constants took ~50 VGPRs of the tile code's.
+// (The host pass, which only parses the kernels, sees a plain pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using cptr = const __attribute__((address_space(4))) T*;
+#else
+template <class T> using cptr = const T*;
+#endif
+
+template <class T> __device__ __forceinline__ int find_shot_c(cptr<T> start, int n, unsigned t)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class P> __device__ __forceinline__ bool batch_rescales(const BatchArgs& c, int k)
+{
+    if constexpr (!P::scaled) {
+        return false;
+    } else {
+        const cptr<ShotDesc<P>> D = (cptr<ShotDesc<P>>)c.descs;
+        const cptr<int> its = (cptr<int>)c.it;
+        for (int s = 0; s < c.n; ++s) {
+            const int it = its[s] + k;
+            if (it < D[s].nt && D[s].ratio && D[s].ratio[it] != 0.0f) return true;
+        }
+        return false;
+    }
+}
+
+// One step tile of one shot (VEL: velocity, else stress).
+template <class P, int N, bool VEL> __device__ __forceinline__ void round_tile(cptr<ShotDesc<P>> d, unsigned local)
+{
+    const StepArgs<P> a = d->a;
+    const Tile tl = tile_of_index(local, a.gx, a.gy);
+    if constexpr (VEL) velocity_tile<P, N, 2>(a, tl);
+    else stress_tile<P, N, 2>(a, tl);
+}
+
+// The step tiles of one phase that this block owns.  One call site for the
+// tile body: two inlined copies (one per map) measured 124 VGPRs instead of 78.
+template <class P, int N, bool VEL> __device__ __forceinline__ void round_tiles(const BatchArgs& c, int k)
+{
+    const cptr<ShotDesc<P>> D = (cptr<ShotDesc<P>>)c.descs;
+    const cptr<int> its = (cptr<int>)c.it;
+    // without the XCD map block b takes tiles b, b + grid, ... of the batch's
+    // tile prefix; with it the grid is a multiple of 8 blocks, XCD x runs
+    // blocks x, x + 8, ... and they take turns over XCD x's share
+    const unsigned x = c.xcd_map ? blockIdx.x % 8u : 0u;
+    const cptr<unsigned> xs = (cptr<unsigned>)(c.xcd_map ? c.xstart + x * (unsigned)(c.n + 1) : c.bstart);
+    const unsigned first = c.xcd_map ? blockIdx.x / 8u : blockIdx.x;
+    const unsigned step = c.xcd_map ? gridDim.x / 8u : gridDim.x;
+    for (unsigned j = first; j < xs[c.n]; j += step) {
+        const int s = find_shot_c(xs, c.n, j);
+        if (its[s] + k >= D[s].nt) continue;
+        unsigned base = 0;
+        if (c.xcd_map) {
+            const unsigned nb = (unsigned)D[s].a.nblocks, q = nb / 8u, r = nb % 8u;
+            base = x * q + (x < r ? x : r);
+        }
+        round_tile<P, N, VEL>(D + s, base + (j - xs[s]));
+    }
+}
+
+template <class P, int N> __global__ __launch_bounds__(512) void k_round(const BatchArgs c, int K)
+{
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    const cptr<ShotDesc<P>> D = (cptr<ShotDesc<P>>)c.descs;
+    const cptr<int> its = (cptr<int>)c.it;
+    const cptr<unsigned> rstart = (cptr<unsigned>)c.rstart;
+    const unsigned nthr = blockDim.x * blockDim.y, tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int k = 0; k < K; ++k) {
+        if (batch_rescales<P>(c, k)) {
+            // receivers of step k-1, every shot's rescale, then the sources
+            if (k > 0)
+                for (unsigned r = blockIdx.x * nthr + tid; r < c.nrec; r += gridDim.x * nthr) {
+                    const int s = find_shot_c(rstart, c.n, r);
+                    gather_one<P>(D[s], its[s] + k - 1, r - rstart[s]);
+                }
+            grid.sync();
+            const unsigned long long stride = (unsigned long long)gridDim.x * nthr;
+            const unsigned long long i0 = (unsigned long long)blockIdx.x * nthr + tid;
+            for (int s = 0; s < c.n; ++s) {
+                const int it = its[s] + k;
+                if (it >= D[s].nt || !D[s].ratio) continue;
+                const float ratio = D[s].ratio[it];
+                if (ratio == 0.0f) continue;
+                const unsigned long long nmem[3] = {D[s].nmem[0], D[s].nmem[1], D[s].nmem[2]};
+                for (int arr = 0; arr < PW_NFIELDS + 3 * kMemPerAxis; ++arr)
+                    rescale_array<P>(D[s].a, arr, i0, stride, D[s].ncell, nmem, ratio);
+            }
+            grid.sync();
+            for (int s = blockIdx.x; s < c.n; s += gridDim.x)
+                if (tid < 4) source_channel<P>(D[s], its[s] + k, (int)tid);
+        } else {
+            // one block per shot: its receivers of step k-1 are read before its
+            // source of step k is added (a receiver may sit at the source cell)
+            for (int s = blockIdx.x; s < c.n; s += gridDim.x) {
+                if (k > 0)
+                    for (unsigned q = tid; q < (unsigned)D[s].nrec; q += nthr) gather_one<P>(D[s], its[s] + k - 1, q);
+                __syncthreads();
+                if (tid < 4) source_channel<P>(D[s], its[s] + k, (int)tid);
+            }
+        }
+        grid.sync();
+        round_tiles<P, N, true>(c, k);
+        grid.sync();
+        round_tiles<P, N, false>(c, k);
+        grid.sync();
+    }
+    for (int s = blockIdx.x; s < c.n; s += gridDim.x)
+        for (unsigned q = tid; q < (unsigned)D[s].nrec; q += nthr) gather_one<P>(D[s], its[s] + K - 1, q);
 }
 
 // --------------------------------------------------------------------------
@@ -2256,6 +2404,45 @@ hipError_t batch_step(const BatchGroup& g, hipStream_t st, int k)
         if (e != hipSuccess) return e;
         if (g.nrec > 0) k_gather_batch<P><<<(g.nrec + 255) / 256, 256, 0, st>>>(c, k);
         return hipGetLastError();
+    });
+}
+
+// Blocks of the persistent round kernel: as many as fit on the device at once
+// (a cooperative launch needs every block resident), a multiple of 8, and no
+// more than the batch has tiles.
+template <class K> static hipError_t round_blocks(K kern, const BatchGroup& g, unsigned& blocks)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * g.ty, 0);
+    if (e != hipSuccess) return e;
+    unsigned b = (unsigned)per_cu * (unsigned)cus;
+    const unsigned want = (g.nblocks + 7u) & ~7u;
+    if (b > want) b = want;
+    b &= ~7u;
+    if (b == 0) return hipErrorCooperativeLaunchTooLarge;
+    blocks = b;
+    return hipSuccess;
+}
+
+hipError_t batch_round(const BatchGroup& g, hipStream_t st, int K)
+{
+    if (g.n <= 0 || K <= 0) return hipSuccess;
+    BatchArgs c = batch_args(g);
+    return dispatch(g.prec, [&](auto pol) -> hipError_t {
+        using P = decltype(pol);
+        return dispatch_vx(g.vx, [&](auto n) -> hipError_t {
+            auto kern = k_round<P, decltype(n)::value>;
+            unsigned blocks = 0;
+            hipError_t e = round_blocks(kern, g, blocks);
+            if (e != hipSuccess) return e;
+            int Kc = K;
+            void* args[] = {(void*)&c, (void*)&Kc};
+            e = hipLaunchCooperativeKernel((const void*)kern, dim3(blocks), dim3(64, g.ty), args, 0, st);
+            if (e != hipSuccess) return e;
+            return hipGetLastError();
+        });
     });
 }
 

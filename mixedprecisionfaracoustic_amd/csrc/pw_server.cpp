@@ -8,7 +8,9 @@
 //      steps any active shot has left), so shots finish on round boundaries:
 //      per precision, one launch per step kernel covers all its shots (the
 //      descriptor tables of pw_kernels.hip), and the K steps are captured once
-//      as a hipGraph and replayed while the set of shots stays the same;
+//      as a hipGraph and replayed while the set of shots stays the same; or,
+//      with cfg.persistent, one persistent cooperative launch per precision
+//      runs the whole round;
 //   3. waits for the round, copies the seismograms of finished shots to the
 //      host and frees their device memory.
 // Submitting, waiting and fetching results are thread-safe host operations.
@@ -248,6 +250,16 @@ struct Engine {
             pw_status st = rebuild();
             if (st != PW_OK) return st;
         }
+        if (c.persistent) {
+            // one cooperative launch per precision for the whole round
+            for (const BatchGroup& g : groups) {
+                if (g.n <= 0) continue;
+                PW_HIP_TRY(batch_round(g, srv->stream, K));
+                PW_HIP_TRY(batch_advance(g, srv->stream, K));
+                launches += 2;
+            }
+            return PW_OK;
+        }
         // A graph pays off only when it is replayed: capture a round when the
         // previous round ran the same shots for the same number of steps.
         const bool repeat = last_K == K;
@@ -474,6 +486,7 @@ extern "C" void pw_server_config_default(pw_server_config* c)
     c->use_graphs = 1;
     c->batched = 1;
     c->fill_cells = 1LL << 23;
+    c->persistent = 0;
 }
 
 extern "C" pw_status pw_server_create(const pw_server_config* cfg, pw_server** out)
@@ -483,10 +496,19 @@ extern "C" pw_status pw_server_create(const pw_server_config* cfg, pw_server** o
     if (cfg->max_active < 1 || cfg->round_steps < 1 || cfg->hbm_budget < 0 || cfg->fill_cells < 0)
         return fail(PW_ERR_INVALID,
                     "pw_server_create: max_active and round_steps must be >= 1, budget and fill_cells >= 0");
+    if (cfg->persistent != 0 && cfg->persistent != 1)
+        return fail(PW_ERR_INVALID, "pw_server_create: persistent must be 0 or 1");
+    if (cfg->persistent && !cfg->batched)
+        return fail(PW_ERR_INVALID, "pw_server_create: persistent rounds need batched rounds");
     int ndev = 0;
     PW_HIP_TRY(hipGetDeviceCount(&ndev));
     if (cfg->device < 0 || cfg->device >= ndev) return fail(PW_ERR_INVALID, "pw_server_create: no such device");
     PW_HIP_TRY(hipSetDevice(cfg->device));
+    if (cfg->persistent) {
+        int coop = 0;
+        PW_HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, cfg->device));
+        if (!coop) return fail(PW_ERR_UNSUPPORTED, "pw_server_create: the device has no cooperative launch");
+    }
     std::unique_ptr<pw_server> srv(new pw_server());
     srv->cfg = *cfg;
     srv->t0 = std::chrono::steady_clock::now();

@@ -108,7 +108,8 @@ def test_each_shot_alone_through_the_server_is_identical(volume, alone):
             srv.release(i)
 
 
-@pytest.mark.parametrize("kw", [dict(use_graphs=False), dict(batched=False), dict(round_steps=1)])
+@pytest.mark.parametrize("kw", [dict(use_graphs=False), dict(batched=False), dict(round_steps=1),
+                                dict(persistent=True), dict(persistent=True, round_steps=7)])
 def test_round_modes_are_identical(volume, alone, kw):
     shots, ref = alone
     res, _, _ = run_server(volume, shots, max_active=4, **kw)
@@ -227,11 +228,13 @@ def test_server_shots_match_the_oracle(volume):
         assert e <= tol, (q["prec"], e)
 
 
-def test_server_at_trace_grid_sizes_matches_solver():
+@pytest.mark.parametrize("persistent", [False, True])
+def test_server_at_trace_grid_sizes_matches_solver(persistent):
     """One shot of each grid size of the serving trace (96^3, 128^3, 192^3 and
     256^3 total, 20-cell CPML, 500-800 steps, FP32 and comp16 alternating),
     cropped from a two-layer perturbed volume: every seismogram bit-identical to
-    the same shot run alone through the Solver."""
+    the same shot run alone through the Solver, with the batched launches and
+    with the persistent round kernel."""
     np_ = 20
     n_vol = 256
     spec = pw.layered([0, n_vol // 2], [cases.sandstone(), cases.lower_layer()], perturb=0.05, corr_cells=8.0, seed=0)
@@ -251,7 +254,7 @@ def test_server_at_trace_grid_sizes_matches_solver():
         shots.append(dict(n=n, nt=nt, crop=crop, src=src, prec="fp32" if k % 2 == 0 else "comp16",
                           fields=["txx"] * 32 + ["vux"] * 32, idx=idx))
     # no cell limit: all four sizes advance in the same batched launches
-    with pw.Server(max_active=4, round_steps=32, fill_cells=0) as srv:
+    with pw.Server(max_active=4, round_steps=32, fill_cells=0, persistent=persistent) as srv:
         srv.set_medium(vol)
         ids = [srv.submit(q["n"], q["n"], q["n"], np_, H, DT, q["nt"], q["prec"], q["crop"], q["src"], 25.0,
                           q["fields"], q["idx"], weights=(1.0, 1.0, 1.0, -1.0), cpml_r=1e-4, cpml_v0=4000.0)

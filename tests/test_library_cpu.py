@@ -143,6 +143,10 @@ def test_config_validation_errors():
     assert ei.value.status == 1 and "tile_rows" in str(ei.value)
     with pytest.raises(ValueError):
         pw.Solver(40, 40, 40, 6, 5.0, 5e-4, 10, tiling={"zc": 4})
+    # the persistent round kernel is a form of the batched rounds
+    with pytest.raises(pw.PorowaveError) as ei:
+        pw.Server(persistent=True, batched=False)
+    assert ei.value.status == 1 and "persistent" in str(ei.value)
 
 
 def test_binary_files_round_trip(tmp_path):
