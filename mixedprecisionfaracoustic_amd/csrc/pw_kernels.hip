@@ -1837,6 +1837,28 @@ __device__ __forceinline__ int find_shot(const unsigned* __restrict__ start, int
     return lo;
 }
 
+// Tables nobody writes during a launch, read through the constant address
+// space: the compiler then loads them into scalar registers.  Read as ordinary
+// memory after the stores of a tile loop they came through vector loads, and
+// the descriptor's constants took ~50 VGPRs of the tile code's.
+// (The host pass, which only parses the kernels, sees a plain pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using cptr = const __attribute__((address_space(4))) T*;
+#else
+template <class T> using cptr = const T*;
+#endif
+
+template <class T> __device__ __forceinline__ int find_shot_c(cptr<T> start, int n, unsigned t)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Block -> (shot, tile index within the shot).  With the XCD map, XCD x (the
 // blocks L with L % 8 == x) gets the x-th eighth of EVERY shot's tiles, each
 // eighth contiguous in the shot's x-y-z tile order: the XCDs share the work
@@ -1845,17 +1867,17 @@ __device__ __forceinline__ int find_shot(const unsigned* __restrict__ start, int
 // Returns false for the spare blocks of the XCDs with fewer tiles.
 template <class P> __device__ __forceinline__ bool batch_block(const BatchArgs& c, int& s, unsigned& local)
 {
-    const ShotDesc<P>* D = (const ShotDesc<P>*)c.descs;
+    const cptr<ShotDesc<P>> D = (cptr<ShotDesc<P>>)c.descs;
     if (!c.xcd_map) {
         const unsigned t = blockIdx.x;
-        s = find_shot(c.bstart, c.n, t);
-        local = t - c.bstart[s];
+        s = find_shot_c((cptr<unsigned>)c.bstart, c.n, t);
+        local = t - ((cptr<unsigned>)c.bstart)[s];
         return true;
     }
     const unsigned x = blockIdx.x % 8u, j = blockIdx.x / 8u;
-    const unsigned* xs = c.xstart + x * (unsigned)(c.n + 1);
+    const cptr<unsigned> xs = (cptr<unsigned>)(c.xstart + x * (unsigned)(c.n + 1));
     if (j >= xs[c.n]) return false;
-    s = find_shot(xs, c.n, j);
+    s = find_shot_c(xs, c.n, j);
     const unsigned nb = (unsigned)D[s].a.nblocks, q = nb / 8u, r = nb % 8u;
     local = x * q + (x < r ? x : r) + (j - xs[s]);
     return true;
@@ -1866,8 +1888,8 @@ template <class P, int N> __global__ __launch_bounds__(512) void k_velocity_batc
     int s;
     unsigned t;
     if (!batch_block<P>(c, s, t)) return;
-    const ShotDesc<P>* d = (const ShotDesc<P>*)c.descs + s;
-    if (c.it[s] + k >= d->nt) return;
+    const cptr<ShotDesc<P>> d = (cptr<ShotDesc<P>>)c.descs + s;
+    if (((cptr<int>)c.it)[s] + k >= d->nt) return;
     const StepArgs<P> a = d->a;
     velocity_tile<P, N, 2>(a, tile_of_index(t, a.gx, a.gy));
 }
@@ -1877,8 +1899,8 @@ template <class P, int N> __global__ __launch_bounds__(512) void k_stress_batch(
     int s;
     unsigned t;
     if (!batch_block<P>(c, s, t)) return;
-    const ShotDesc<P>* d = (const ShotDesc<P>*)c.descs + s;
-    if (c.it[s] + k >= d->nt) return;
+    const cptr<ShotDesc<P>> d = (cptr<ShotDesc<P>>)c.descs + s;
+    if (((cptr<int>)c.it)[s] + k >= d->nt) return;
     const StepArgs<P> a = d->a;
     stress_tile<P, N, 2>(a, tile_of_index(t, a.gx, a.gy));
 }
@@ -1980,28 +2002,6 @@ __global__ void k_advance(int* it, const int* __restrict__ nt, int n, int K)
 // during the launch.  The per-cell code is that of the other step kernels, so
 // results are bit-identical to them.
 // --------------------------------------------------------------------------
-// Tables nobody writes during a launch, read through the constant address
-// space: the compiler then loads them into scalar registers.  Read as ordinary
-// memory after the stores of a tile loop they came through vector loads, and
-// the descriptor's constants took ~50 VGPRs of the tile code's.
-// (The host pass, which only parses the kernels, sees a plain pointer.)
-#if defined(__HIP_DEVICE_COMPILE__)
-template <class T> using cptr = const __attribute__((address_space(4))) T*;
-#else
-template <class T> using cptr = const T*;
-#endif
-
-template <class T> __device__ __forceinline__ int find_shot_c(cptr<T> start, int n, unsigned t)
-{
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 template <class P> __device__ __forceinline__ bool batch_rescales(const BatchArgs& c, int k)
 {
     if constexpr (!P::scaled) {
