@@ -292,6 +292,9 @@ SCHEDULES = (
     {"waves_x": 2},                   # two waves side by side along x (128-cell tiles)
     {"waves_x": 1, "cells": 1},       # every kernel one wave per 64-cell row
     {"planes": 3, "rows": 2},         # 3-plane z chunks (ragged at the top), 64 x 2 tiles
+    {"rows": 4},                      # 64 x 4 tiles (comp16f's default is 64 x 1)
+    {"rows": 1, "planes": 16},        # one-wave blocks, 16-plane chunks
+    {"rows": 2, "waves_x": 2},        # 128 x 1 tiles
 )
 
 
