@@ -28,15 +28,18 @@ __device__ __forceinline__ float round32(float v, unsigned z) { return __uint_as
 // zero/subnormal) gives 2^(max(e5, 1) - 33), i.e. float bits
 // (max(e5,1)+94) << 23 with the 127 bias: e5 = 1 -> 2^-32, the subnormal value.
 // --------------------------------------------------------------------------
+// (The exponent field is used in place, e5 << 10 shifted by 13 to e5 << 23: an
+// and, a max and a shift-add per value, where extracting e5 first compiled to
+// five instructions.)
 __device__ __forceinline__ float c16_unit_bits(unsigned short hb)
 {
-    const unsigned e5 = (hb >> 10) & 31u;
-    return __uint_as_float(((e5 > 1u ? e5 : 1u) + 94u) << 23);
+    const unsigned m = (unsigned)hb & 0x7c00u;
+    return __uint_as_float(((m > 0x0400u ? m : 0x0400u) << 13) + (94u << 23));
 }
 __device__ __forceinline__ float c16_inv_unit_bits(unsigned short hb)
 {
-    const unsigned e5 = (hb >> 10) & 31u;
-    return __uint_as_float((160u - (e5 > 1u ? e5 : 1u)) << 23);
+    const unsigned m = (unsigned)hb & 0x7c00u;
+    return __uint_as_float((160u << 23) - ((m > 0x0400u ? m : 0x0400u) << 13));
 }
 __device__ __forceinline__ float c16_decode(_Float16 hi, int8_t lo)
 {
