@@ -108,8 +108,11 @@ def test_each_shot_alone_through_the_server_is_identical(volume, alone):
             srv.release(i)
 
 
+# persistent rounds: comp16's rescale at step 100 falls inside a round (32, 7
+# steps) and at a round's first step (25, 1)
 @pytest.mark.parametrize("kw", [dict(use_graphs=False), dict(batched=False), dict(round_steps=1),
-                                dict(persistent=True), dict(persistent=True, round_steps=7)])
+                                dict(persistent=True), dict(persistent=True, round_steps=7),
+                                dict(persistent=True, round_steps=25), dict(persistent=True, round_steps=1)])
 def test_round_modes_are_identical(volume, alone, kw):
     shots, ref = alone
     res, _, _ = run_server(volume, shots, max_active=4, **kw)
