@@ -43,7 +43,9 @@ typedef enum {
     PW_ERR_UNSTABLE = 5,      /* Courant condition violated by the medium */
     PW_ERR_UNSUPPORTED = 6,   /* valid request the library does not implement */
     PW_ERR_IO = 7,            /* file could not be written */
-    PW_ERR_TIMEOUT = 8        /* pw_server_wait: the shot did not finish in time */
+    PW_ERR_TIMEOUT = 8,       /* pw_server_wait: the shot did not finish in time */
+    PW_ERR_CANCELLED = 9,     /* the shot was cancelled (pw_server_cancel) */
+    PW_ERR_NONFINITE = 10     /* the shot's seismogram holds NaN or Inf (an unstable run) */
 } pw_status;
 
 /* Storage precision of wavefields and CPML memory variables.  Arithmetic is FP32
@@ -455,6 +457,11 @@ pw_status pw_server_result(pw_server* srv, long long id, float* seis);
 /* Forgets a finished or failed shot and its host-side result (PW_ERR_STATE while
  * the shot is queued or running, or while another thread waits on it). */
 pw_status pw_server_release(pw_server* srv, long long id);
+/* Cancels a shot.  A queued shot fails at once; a running one leaves the device
+ * at the end of the current round (its memory is freed then).  Either way it
+ * ends in PW_SHOT_FAILED with PW_ERR_CANCELLED, and the other shots are not
+ * disturbed.  PW_ERR_STATE for a shot that has already finished. */
+pw_status pw_server_cancel(pw_server* srv, long long id);
 pw_status pw_server_get_stats(pw_server* srv, pw_server_stats* out);
 /* Seconds since pw_server_create (the clock of pw_shot_status). */
 double pw_server_clock(pw_server* srv);

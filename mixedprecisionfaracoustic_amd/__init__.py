@@ -23,7 +23,8 @@ PRECISIONS = {"fp32": 0, "comp16": 1, "bf16": 2, "fp64": 3, "comp16f": 4}
 # The reference's seismogram channels (Float/main.cu:1085-1093, 1213-1222).
 REFERENCE_CHANNELS = [("sisx", "txx"), ("sisz", "tzz"), ("sisvu", "vux"), ("sisvw", "vwx"), ("sisp", "p")]
 STATUS = {0: "PW_OK", 1: "PW_ERR_INVALID", 2: "PW_ERR_HIP", 3: "PW_ERR_OUT_OF_MEMORY", 4: "PW_ERR_STATE",
-          5: "PW_ERR_UNSTABLE", 6: "PW_ERR_UNSUPPORTED", 7: "PW_ERR_IO", 8: "PW_ERR_TIMEOUT"}
+          5: "PW_ERR_UNSTABLE", 6: "PW_ERR_UNSUPPORTED", 7: "PW_ERR_IO", 8: "PW_ERR_TIMEOUT",
+          9: "PW_ERR_CANCELLED", 10: "PW_ERR_NONFINITE"}
 SHOT_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 MAX_LAYERS = 16
 
@@ -146,6 +147,7 @@ _SIGS = {
     "pw_server_status": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(ShotStatus)]),
     "pw_server_result": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(C.c_float)]),
     "pw_server_release": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "pw_server_cancel": (C.c_int, [C.c_void_p, C.c_longlong]),
     "pw_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "pw_server_clock": (C.c_double, [C.c_void_p]),
 }
@@ -580,6 +582,10 @@ class Server:
         out = np.zeros((max(n, 1), self._nt[sid]), np.float32)
         _check(lib().pw_server_result(self._h, int(sid), _fp(out)))
         return out[:n]
+
+    def cancel(self, sid):
+        """Cancels a queued or running shot (it then fails with PW_ERR_CANCELLED)."""
+        _check(lib().pw_server_cancel(self._h, int(sid)))
 
     def release(self, sid):
         _check(lib().pw_server_release(self._h, int(sid)))

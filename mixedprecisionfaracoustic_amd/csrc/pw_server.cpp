@@ -49,6 +49,7 @@ struct Shot {
     std::string errmsg;
     double t_submit = -1, t_admit = -1, t_done = -1;
     int waiters = 0;       // threads inside pw_server_wait on this shot (release refuses meanwhile)
+    bool cancel = false;   // pw_server_cancel on a running shot: retired at the end of the round
     size_t bytes = 0;      // device bytes while admitted (estimate used for admission)
     long long cells = 0;
     // while admitted
@@ -385,14 +386,38 @@ void worker_main(pw_server* srv)
         // each retired shot's own status: a failed read-back fails that shot only
         std::vector<std::pair<pw_status, std::string>> outcome;
         if (st == PW_OK) {
-            for (Shot* q : srv->active) {
-                q->it += K;
-                if (q->it >= q->req.nt) done.push_back(q);
+            std::vector<char> cancelled;
+            {
+                // cancel flags are set under the lock by pw_server_cancel
+                std::lock_guard<std::mutex> lc(srv->mu);
+                for (Shot* q : srv->active) {
+                    q->it += K;
+                    if (q->it >= q->req.nt || q->cancel) {
+                        done.push_back(q);
+                        cancelled.push_back(q->it < q->req.nt ? 1 : 0);
+                    }
+                }
             }
-            for (Shot* q : done) {
+            for (size_t i = 0; i < done.size(); ++i) {
+                Shot* q = done[i];
+                if (cancelled[i]) {
+                    outcome.emplace_back(PW_ERR_CANCELLED, "shot cancelled after " + std::to_string(q->it) + " steps");
+                    continue;
+                }
                 q->seis.assign((size_t)q->s->nrec * q->req.nt, 0.0f);
                 pw_status g = q->s->nrec > 0 ? pw_get_seismogram(q->s, q->seis.data()) : PW_OK;
-                outcome.emplace_back(g, g == PW_OK ? std::string() : std::string(pw_last_error()));
+                std::string m = g == PW_OK ? std::string() : std::string(pw_last_error());
+                if (g == PW_OK) {
+                    // an unstable run (e.g. a source far beyond the float range)
+                    // shows up as NaN or Inf at the receivers
+                    for (float v : q->seis)
+                        if (!std::isfinite(v)) {
+                            g = PW_ERR_NONFINITE;
+                            m = "shot's seismogram holds NaN or Inf values";
+                            break;
+                        }
+                }
+                outcome.emplace_back(g, m);
             }
         } else {
             done = srv->active;  // a failed round fails every shot in it
@@ -696,6 +721,24 @@ extern "C" pw_status pw_server_release(pw_server* srv, long long id)
         return fail(PW_ERR_STATE, "pw_server_release: shot still queued or running");
     if (q->waiters > 0) return fail(PW_ERR_STATE, "pw_server_release: another thread is waiting on the shot");
     srv->shots.erase(id);
+    return PW_OK;
+}
+
+extern "C" pw_status pw_server_cancel(pw_server* srv, long long id)
+{
+    if (!srv) return fail(PW_ERR_INVALID, "pw_server_cancel: null server");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    Shot* q = find_shot(srv, id);
+    if (!q) return fail(PW_ERR_INVALID, "pw_server_cancel: unknown shot id");
+    if (q->state == PW_SHOT_DONE || q->state == PW_SHOT_FAILED)
+        return fail(PW_ERR_STATE, "pw_server_cancel: shot already finished");
+    if (q->state == PW_SHOT_QUEUED) {
+        auto it = std::find(srv->pending.begin(), srv->pending.end(), q);
+        if (it != srv->pending.end()) srv->pending.erase(it);
+        finish(srv, q, PW_ERR_CANCELLED, "shot cancelled before admission");
+        return PW_OK;
+    }
+    q->cancel = true;  // running (or being admitted): the worker retires it after the round
     return PW_OK;
 }
 

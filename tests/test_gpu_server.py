@@ -201,6 +201,40 @@ def test_failing_shots_do_not_disturb_others(volume, alone):
         assert ei.value.status == 3  # PW_ERR_OUT_OF_MEMORY
 
 
+def test_cancel_and_nonfinite_fail_only_their_shots(volume, alone):
+    """A shot cancelled while queued, one cancelled mid-run and one whose source
+    overflows the float range (NaN / Inf at its receivers) each fail with their
+    own status; the shots beside them finish bit-identical to the Solver."""
+    shots, ref = alone
+    with pw.Server(max_active=2, round_steps=8) as srv:
+        srv.set_medium(volume)
+        a = submit(srv, shots[0])                      # FP32, 120 steps
+        long_shot = dict(shots[2], nt=4000)
+        b = submit(srv, long_shot)                     # FP32, 4000 steps: cancelled while running
+        c = submit(srv, shots[1])                      # comp16, 150 steps
+        d = submit(srv, shots[3])                      # queued behind c: cancelled at once
+        srv.cancel(d)
+        bad = dict(shots[0], w=(3.0e38, 3.0e38, 3.0e38, 3.0e38))
+        e = submit(srv, bad)
+        srv.wait(a, timeout=120)
+        assert srv.status(b)["state"] == "running"
+        srv.cancel(b)
+        for sid, code in ((b, 9), (d, 9), (e, 10)):    # PW_ERR_CANCELLED, PW_ERR_NONFINITE
+            with pytest.raises(pw.PorowaveError) as ei:
+                srv.wait(sid, timeout=120)
+            assert ei.value.status == code, (sid, str(ei.value))
+            st = srv.status(sid)
+            assert st["state"] == "failed" and st["error"] == code
+        srv.wait(c, timeout=120)
+        assert np.array_equal(srv.result(a), ref[0])
+        assert np.array_equal(srv.result(c), ref[1])
+        with pytest.raises(pw.PorowaveError) as ei:
+            srv.cancel(a)                              # already finished
+        assert ei.value.status == 4
+        stats = srv.stats()
+    assert stats["failed"] == 3 and stats["completed"] == 2
+
+
 def test_server_shots_match_the_oracle(volume):
     """Server output against the CPU oracle, not only against the single-shot
     path: in a mix of four shots, the FP32 shot is within 1e-5 of the Float

@@ -110,9 +110,11 @@ def test_struct_layouts_match_the_header(tmp_path):
 
 
 def test_server_needs_a_device():
-    """Without a GPU the server refuses to start (no CPU fallback)."""
+    """Without a GPU the server refuses to start (no CPU fallback); server calls
+    on no server are refused."""
     with pytest.raises(pw.PorowaveError):
         pw.Server()
+    assert pw.lib().pw_server_cancel(None, 1) == 1  # PW_ERR_INVALID
 
 
 def test_config_validation_errors():
