@@ -392,13 +392,22 @@ pw_status pw::solver_create(const pw_config* cfg, pw_solver** out)
     const int zc_def = c.nz >= 512 && c16f ? 16 : (c.nz >= 256 ? 8 : 4);
     s->b.zc = c.tile_planes > 0 ? c.tile_planes : zc_def;
     s->b.ty = c.tile_rows > 0 ? c.tile_rows : (c16f ? 1 : 4);
+    // FP32's 4th-order velocity kernel from nz >= 512: one-wave blocks
+    // marching 16 planes (3 % faster at 512^3; at order 2 no different, and
+    // 2 % slower at 192^3); every other velocity kernel takes the stress
+    // kernel's rows and planes
+    const bool v1 = s->prec == PW_FP32 && order == 4 && c.nz >= 512;
+    s->b.zc_vel = c.tile_planes > 0 ? c.tile_planes : (v1 ? 16 : s->b.zc);
+    s->b.ty_vel = c.tile_rows > 0 ? c.tile_rows : (v1 ? 1 : s->b.ty);
     // waves side by side along x per tile: BF16's velocity kernel and FP32's
     // stress kernel run fastest with two (128-cell tiles), every other kernel
     // with one
     const bool wx_set = c.tile_waves_x > 0;
     s->b.wx = wx_set ? c.tile_waves_x : (s->prec == PW_FP32 ? 2 : 1);
     s->b.wx_vel = wx_set ? c.tile_waves_x : (s->prec == PW_BF16 ? 2 : 1);
-    if (s->b.ty == 1) s->b.wx = s->b.wx_vel = 1;  // a two-wave row needs two waves in the block
+    // a two-wave row needs two waves in the block
+    if (s->b.ty == 1) s->b.wx = 1;
+    if (s->b.ty_vel == 1) s->b.wx_vel = 1;
     // cells per lane along x: wider row segments per wave.  comp16's velocity
     // kernel runs fastest with 2 cells per lane (its FP16 and int8 rows then
     // load whole cache lines); every other kernel with one

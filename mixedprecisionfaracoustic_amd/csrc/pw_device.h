@@ -163,8 +163,10 @@ struct DevBuffers {
     void* mem[3][kMemPerAxis];
     void* tab[3];    // per axis: n entries of {c_full, e_full, c_half, e_half} (float or double)
     double dt, h;
-    int zc;          // z cells per block in the step kernels
-    int ty;          // waves per block in the step kernels (block 64 x ty)
+    int zc;          // z cells per block in the stress kernel
+    int zc_vel;      // ... and in the velocity kernel
+    int ty;          // waves per block in the stress kernel (block 64 x ty)
+    int ty_vel;      // ... and in the velocity kernel
     int wx;          // waves side by side along x in a stress-kernel tile (1 or 2)
     int wx_vel;      // ... and in a velocity-kernel tile
     int vx;          // cells per lane along x in the stress kernel (1, 2, 4)

@@ -2144,6 +2144,8 @@ hipError_t launch_velocity(const DevBuffers& b, hipStream_t st, int zb, int ze)
     DevBuffers bv = b;
     bv.vx = b.vx_vel;  // the velocity kernel's own row-segment width and tile shape
     bv.wx = b.wx_vel;
+    bv.ty = b.ty_vel;
+    bv.zc = b.zc_vel;
     return dispatch(b.prec, [&](auto pol) {
         using P = decltype(pol);
         const StepArgs<P> a = range_args<P>(bv, zb, ze);
