@@ -399,12 +399,16 @@ pw_status pw::solver_create(const pw_config* cfg, pw_solver** out)
     const bool v1 = s->prec == PW_FP32 && order == 4 && c.nz >= 512;
     s->b.zc_vel = c.tile_planes > 0 ? c.tile_planes : (v1 ? 16 : s->b.zc);
     s->b.ty_vel = c.tile_rows > 0 ? c.tile_rows : (v1 ? 1 : s->b.ty);
-    // waves side by side along x per tile: BF16's velocity kernel and FP32's
-    // stress kernel run fastest with two (128-cell tiles), every other kernel
-    // with one
+    // waves side by side along x per tile: FP32's two kernels and BF16's
+    // velocity kernel run fastest with two (128-cell tiles; FP32's velocity
+    // kernel 1.6 % faster at 512^3), every other kernel with one.  Only where
+    // 128-cell tiles leave no more of a row idle than 64-cell ones: a row of
+    // 192 cells would leave half its second 128-cell tile idle (measured 10 %
+    // slower there, FP32 stress and BF16 velocity)
+    const bool wide = (c.nx + 127) / 128 * 128 == (c.nx + 63) / 64 * 64;
     const bool wx_set = c.tile_waves_x > 0;
-    s->b.wx = wx_set ? c.tile_waves_x : (s->prec == PW_FP32 ? 2 : 1);
-    s->b.wx_vel = wx_set ? c.tile_waves_x : (s->prec == PW_BF16 ? 2 : 1);
+    s->b.wx = wx_set ? c.tile_waves_x : (s->prec == PW_FP32 && wide ? 2 : 1);
+    s->b.wx_vel = wx_set ? c.tile_waves_x : ((s->prec == PW_FP32 || s->prec == PW_BF16) && wide ? 2 : 1);
     // a two-wave row needs two waves in the block
     if (s->b.ty == 1) s->b.wx = 1;
     if (s->b.ty_vel == 1) s->b.wx_vel = 1;
