@@ -414,10 +414,12 @@ pw_status pw::solver_create(const pw_config* cfg, pw_solver** out)
     if (s->b.ty_vel == 1) s->b.wx_vel = 1;
     // cells per lane along x: wider row segments per wave.  comp16's velocity
     // kernel runs fastest with 2 cells per lane (its FP16 and int8 rows then
-    // load whole cache lines); every other kernel with one
+    // load whole cache lines) where 128-cell segments fill the row as well as
+    // 64-cell ones (at 192 and 320 cells one lane per cell is 22 % and 13 %
+    // faster); every other kernel with one
     const bool vx_set = c.tile_cells > 0;
     s->b.vx = vx_set ? c.tile_cells : 1;
-    s->b.vx_vel = vx_set ? c.tile_cells : (s->prec == PW_COMP16 ? 2 : 1);
+    s->b.vx_vel = vx_set ? c.tile_cells : (s->prec == PW_COMP16 && wide ? 2 : 1);
     // XCD-aware block order (measured: without it the step kernels read 15-19 %
     // more from HBM)
     s->b.xcd_map = 1;
