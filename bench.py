@@ -242,7 +242,8 @@ def precision_report(args):
     res = {}
     for prec in ("fp64", "fp32", "comp16", "comp16f", "bf16"):
         stream = torch.cuda.Stream()
-        s = pw.Solver(n, n, n, np_, h, dt, nt, precision=prec, cpml_r=1e-4, cpml_v0=CPML_V0, stream=stream.cuda_stream)
+        s = pw.Solver(n, n, n, np_, h, dt, nt, precision=prec, cpml_r=1e-4, cpml_v0=CPML_V0, stream=stream.cuda_stream,
+                      space_order=args.space_order)
         s.set_medium_layered(spec)
         s.set_source(n // 2, n // 2, n // 2, 25.0, (1.0, 1.0, 1.0, -1.0))
         s.set_receivers(fields, idx)
@@ -270,7 +271,8 @@ def precision_report(args):
         b = np.asarray(b, np.float64)
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
     out = {"metric": "rel-L2 vs FP64 per seismogram channel and step-1000 txx snapshot; Gcell-steps/s per precision",
-           "config": {"workload": f"biot_two_layer_{n}^3", "steps": nt, "receivers": len(idx), "snapshot_step": snap_step},
+           "config": {"workload": f"biot_two_layer_{n}^3" + ("_order4" if args.space_order == 4 else ""), "steps": nt,
+                      "receivers": len(idx), "snapshot_step": snap_step, "space_order": args.space_order},
            "data": "synthetic (two-layer perturbed Biot medium generated on the GPU, seed 0)", "precisions": {}}
     nr = len(idx) // len(chans)
     for prec in ("fp32", "comp16", "comp16f", "bf16", "fp64"):
