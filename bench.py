@@ -210,6 +210,7 @@ def serving_main(args):
     for q, a, b in zip(trace, seis_b, seis_s):
         if q["prec"] == "fp32":
             err.append(float(np.max(np.abs(a - b))))
+    acc = comp16f_vs_fp32(trace, seis_b, seis_s)
     out = {"metric": "shot-serving throughput and p50/p99 latency vs sequential FP32 FCFS",
            "value": batched["gcell_steps_per_s"], "unit": "Gcell-steps/s", "n_gpus": 1,
            "higher_is_better": True, "data": "synthetic (Poisson trace, seeded)",
@@ -222,7 +223,8 @@ def serving_main(args):
            "throughput_ratio": round(batched["gcell_steps_per_s"] / seq["gcell_steps_per_s"], 3),
            "p50_ratio": round(batched["p50_latency_s"] / seq["p50_latency_s"], 3),
            "p99_ratio": round(batched["p99_latency_s"] / seq["p99_latency_s"], 3),
-           "fp32_shots_identical_to_sequential": all(e == 0.0 for e in err)}
+           "fp32_shots_identical_to_sequential": all(e == 0.0 for e in err),
+           "comp16f_shots_vs_fp32": acc}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -311,6 +313,17 @@ def survey_fracs(ref_b, cells, ms_v, ms_t):
             "bytes": {"k_velocity": round(ref_b[0], 1), "k_stress": round(ref_b[1], 1)}}
 
 
+def comp16f_vs_fp32(trace, seis_srv, seis_seq):
+    """The comp16f-tagged shots of a serving run against the same shots run in
+    FP32 (the sequential baseline): relative L2 of each shot's seismogram."""
+    import numpy as np
+    e = [float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64)))
+         for q, a, b in zip(trace, seis_srv, seis_seq) if q["prec"] == "comp16f" and np.linalg.norm(b) > 0]
+    if not e:
+        return None
+    return {"shots": len(e), "median_rel_l2": float(f"{np.median(e):.3e}"), "max_rel_l2": float(f"{max(e):.3e}")}
+
+
 def serving_summary(shots, rate, seed):
     """The default line's serving object: the first `shots` requests of the
     shot-serving trace (Poisson arrivals, grids of 96^3-256^3, 500-1500 steps,
@@ -320,9 +333,9 @@ def serving_summary(shots, rate, seed):
     n_vol, np_, h, dt = 512, 20, 5.0, 0.5e-3
     spec = workload(n_vol, np_)
     trace = serving_trace(shots, rate, seed, n_vol, np_)
-    srv, _ = run_serving(trace, n_vol, np_, h, dt, spec)
-    seq, _ = run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=True, max_active=1, round_steps=64,
-                         use_graphs=False, batched=False)
+    srv, seis_srv = run_serving(trace, n_vol, np_, h, dt, spec)
+    seq, seis_seq = run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=True, max_active=1, round_steps=64,
+                                use_graphs=False, batched=False)
     keys = ("shots_per_s", "gcell_steps_per_s", "p50_latency_s", "p99_latency_s", "mean_latency_s", "makespan_s",
             "peak_active", "finite")
     return {"trace": {"shots": shots, "rate_per_s": rate, "seed": seed, "grids": [96, 128, 192, 256],
@@ -330,7 +343,9 @@ def serving_summary(shots, rate, seed):
             "server": {k: srv[k] for k in keys}, "sequential_fp32": {k: seq[k] for k in keys},
             "throughput_ratio": round(srv["gcell_steps_per_s"] / seq["gcell_steps_per_s"], 3),
             "p50_ratio": round(srv["p50_latency_s"] / seq["p50_latency_s"], 3),
-            "p99_ratio": round(srv["p99_latency_s"] / seq["p99_latency_s"], 3)}
+            "p99_ratio": round(srv["p99_latency_s"] / seq["p99_latency_s"], 3),
+            # "at equal accuracy": the comp16f shots against the same shots in FP32
+            "comp16f_shots_vs_fp32": comp16f_vs_fp32(trace, seis_srv, seis_seq)}
 
 
 def rank_share_main(args):
