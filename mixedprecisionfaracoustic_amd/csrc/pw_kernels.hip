@@ -12,16 +12,21 @@
 // elements).  A block is a 64 x 4 (x, y) column tile that marches up a chunk of
 // z (8 planes from 256 planes up); each wave owns one 64-cell x row, so every
 // load is a coalesced 256 B (FP32) row segment.  (Tunable per kernel: N cells
-// per lane, wx waves side by side along x; comp16's velocity kernel uses N = 2,
-// BF16's velocity and FP32's stress kernel wx = 2.)  x neighbours come from the adjacent lane (one DPP move; only
-// the edge lane loads), y neighbours hit L1/L2, and z neighbours are carried in
-// registers from the previous iteration (each value is loaded once per block).
+// per lane, wx waves side by side along x, rows and planes per block; by
+// default comp16's velocity kernel uses N = 2 and FP32's two kernels and BF16's
+// velocity kernel wx = 2, where those wider segments fill the rows, comp16f
+// runs 64 x 1 tiles, 16 planes from nz >= 512, and FP32's 4th-order velocity
+// kernel 64 x 1 tiles of 16 planes from nz >= 512.)  x neighbours come from
+// the adjacent lane (one DPP move; only the edge lane loads), y neighbours hit
+// L1/L2, and z neighbours are carried in registers from the previous iteration
+// (each value is loaded once per block).
 // A plane's loads are issued ahead of its CPML memory-variable stores, which
 // the compiler cannot prove disjoint from them.  Each field puts one pointer
 // into the kernels' scalar registers (FA below).
 //
 // Also here: the batched kernels of the shot server (one launch per kernel over
-// many shots), the rescale, snapshot, decode and fill side kernels.
+// many shots, or one persistent cooperative launch per round), the rescale,
+// snapshot, decode and fill side kernels.
 //
 // CPML: the memory variables exist only on the boundary slabs of each axis (in
 // the interior the reference's psi is identically zero).  Each term keeps ONE
@@ -33,8 +38,9 @@
 //
 // Storage policies: FP32 and FP64 use the reference Float formulas; comp16
 // (FP16 hi + int8 residual, Half/main.cu:49-100) and BF16 use the reference
-// Half formulas with FP16 prescaled coefficients.  All arithmetic is FP32
-// except the FP64 policy.
+// Half formulas with FP16 prescaled coefficients; comp16f keeps comp16's state
+// under the Float equations in FP32 (stored units, compensated coefficient
+// records).  All arithmetic is FP32 except the FP64 policy.
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
