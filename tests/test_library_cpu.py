@@ -201,3 +201,17 @@ def test_codec_entry_points_refuse_bad_arguments():
         with pytest.raises(pw.PorowaveError) as ei:
             call()
         assert ei.value.status == 1
+
+
+def test_integration_stub_config_matches_the_library():
+    """The ctypes stub shown in INTEGRATION.md declares pw_config with every
+    field, in order (pw_config_default writes all of them)."""
+    import ctypes as C
+    src = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    i = src.index("class Config(C.Structure):")
+    j = src.index("# every field of pw_config", i)
+    ns = {"C": C}
+    exec(src[i:j], ns)
+    stub = ns["Config"]
+    assert C.sizeof(stub) == C.sizeof(pw.Config)
+    assert [f for f, _ in stub._fields_] == [f for f, _ in pw.Config._fields_]
