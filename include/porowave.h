@@ -153,7 +153,11 @@ const char* pw_last_error(void);
  * cell-centred interior fields of nxi*nyi*nzi doubles (nxi = nx - 2*np, ...),
  * x fastest.  The pad is filled by edge replication, staggered coefficients are
  * computed in FP64 on the GPU and stored in the handle's precision.  Resets the
- * wavefields to zero and the step counter to 0. */
+ * wavefields to zero and the step counter to 0.  PW_HH and PW_H2U are H and
+ * H - 2 mu (Float/main.cu:1644-1645): when they agree with PW_MU to 1e-6 of H,
+ * as the reference's formulas make them, FP32 forms H as (H - 2 mu) + 2 mu in
+ * its stress update; comp16f always does, and refuses a medium where they do
+ * not agree (PW_ERR_UNSUPPORTED). */
 pw_status pw_set_medium(pw_solver* s, const double* const* base, int nxi, int nyi, int nzi);
 
 /* pw_set_medium for base fields already in device memory, allocating nothing.

@@ -523,6 +523,14 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
     if (e != hipSuccess) return fail(PW_ERR_HIP, std::string("pw_set_medium: reduction: ") + hipGetErrorString(e));
     if (!(res[0] > 0) || !std::isfinite(res[0]) || !std::isfinite(res[1]) || !std::isfinite(res[2]))
         return fail(PW_ERR_INVALID, "pw_set_medium: medium has no positive finite modulus");
+    // H and H - 2 mu built by one formula agree to rounding (a few FP32 ulps
+    // when the three arrays were each rounded to float); FP32 then forms H in
+    // the stress kernel instead of streaming it, comp16f always does
+    const bool h_consistent = res[kSlotHMismatch] <= kHMismatchTol;
+    if (s->prec == PW_COMP16F && !h_consistent)
+        return fail(PW_ERR_UNSUPPORTED, "pw_set_medium: comp16f forms H as (H - 2 mu) + 2 mu, but this medium's H "
+                                        "differs from it by " + std::to_string(res[kSlotHMismatch]) + " of H");
+    s->b.derive_h = s->prec == PW_FP32 && h_consistent ? 1 : 0;
     s->vp_max = std::sqrt(res[2]);
     // von Neumann limit of the staggered leapfrog: vp dt / h * sqrt(3) * sum|c_k| <= 1,
     // sum|c_k| = 1 (2nd order) or 9/8 + 1/24 = 7/6 (4th order)
@@ -1092,8 +1100,10 @@ extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
     // stress: reads vu x3 + vw x3, read-modify-writes tau x6 + p (20 touches).
     // coefficient arrays: scaled form 15 (velocity) + 7 (stress) FP16 arrays;
     // physical form 4 (rho, rhof, C1, C2) + 5 (C, M, H, H-2mu, mu) cell-centred
-    // arrays (comp16f: two records of 4 compensated values, no H)
-    const int cv = is_scaled(s->prec) ? 15 : 4, ct = is_scaled(s->prec) ? 7 : (s->prec == PW_COMP16F ? 4 : 5);
+    // arrays (comp16f, and FP32 on a medium whose H is (H-2mu) + 2mu, form H
+    // from H-2mu and mu; comp16f: two records of 4 compensated values)
+    const bool form_h = s->prec == PW_COMP16F || s->b.derive_h;
+    const int cv = is_scaled(s->prec) ? 15 : 4, ct = is_scaled(s->prec) ? 7 : (form_h ? 4 : 5);
     // memory variables: each slot of a kernel read and written per damped axis
     const double mt = mem_slots(s->prec) * f * mem_bytes(s->prec);
     out->bytes_velocity = 19.0 * state_bytes(s->prec) + cv * coef_bytes(s->prec) + mt;

@@ -301,6 +301,7 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     StepArgs<P> a;
     for (int k = 0; k < PW_NFIELDS; ++k) a.f[k] = make_field<P>(b, k);
     for (int k = 0; k < kMaxCoef; ++k) a.c[k] = (const typename P::Coef*)b.coef[k];
+    if (b.derive_h) a.c[PC_HH] = nullptr;  // FP32, consistent medium: the stress kernel forms H
     for (int ax = 0; ax < 3; ++ax) {
         for (int k = 0; k < kMemPerAxis; ++k) a.m[ax][k] = (typename P::Mem*)b.mem[ax][k];
         a.tab[ax] = (const Tab4<R>*)b.tab[ax];
@@ -1573,8 +1574,20 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
             LDMU(o + PL + W, mu_yn);
             seg_right(L, mu_y, mu_xy, o + W + N, EMU, [&](R (&r)[N]) { LDMU(o + W + 1, r); });
             LDC(PC_C, o, C_); LDC(PC_M, o, M_); LDC(PC_H2U, o, H2u_);
-            // kFormFast forms H as (H - 2 mu) + 2 mu
-            if constexpr (!FAST) LDC(PC_HH, o, HH_);
+            // kFormFast forms H as (H - 2 mu) + 2 mu, and so does FP32 when the
+            // medium's H is that sum to rounding (c[PC_HH] null): one
+            // coefficient array less to stream.  FP64 reads H, to stay
+            // bit-identical to the FP64 oracle.
+            if constexpr (std::is_same<P, PolF32>::value) {
+                if (a.c[PC_HH]) {
+                    LDC(PC_HH, o, HH_);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < N; ++j) HH_[j] = H2u_[j] + (R)2 * mu_c[j];
+                }
+            } else if constexpr (!FAST) {
+                LDC(PC_HH, o, HH_);
+            }
         } else {
             LDC(SC_C, o, C_); LDC(SC_M, o, M_); LDC(SC_HH, o, HH_); LDC(SC_H2U, o, H2u_);
             LDC(SC_MUXY, o, mxy); LDC(SC_MUYZ, o, myz); LDC(SC_MUXZ, o, mxz);

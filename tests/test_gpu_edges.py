@@ -37,7 +37,15 @@ SHAPES = [
 ]
 
 
-def hetero_case(shape, np_, nt=60, src=None, seed=0):
+HH, H2U, MU = (pw.BASE.index(k) for k in ("HH", "H2u", "mu"))
+
+
+def hetero_case(shape, np_, nt=60, src=None, seed=0, consistent_h=False):
+    """Sandstone with every base field perturbed by up to 5 % on its own, so a
+    coefficient array read in place of another gives an O(1) error.  H is then
+    not (H - 2 mu) + 2 mu, and the FP32 stress kernel streams it; with
+    consistent_h H is set to that sum (as the reference's model builder makes
+    it), and FP32 and comp16f form H from the other two."""
     nx, ny, nz = shape
     grid = {"nx": nx, "ny": ny, "nz": nz, "np": np_, "h": 5.0, "dt": 0.5e-3, "rc": 1e-4, "v0": 4000.0,
             "f0": 30.0, "src": src or (nx // 2, ny // 2, nz // 2), "src_w": (1.0, 1.0, 1.0, -1.0)}
@@ -45,6 +53,8 @@ def hetero_case(shape, np_, nt=60, src=None, seed=0):
     rng = np.random.default_rng(seed)
     shape_i = (nz - 2 * np_, ny - 2 * np_, nx - 2 * np_)
     base = b[:, None, None, None] * (1.0 + 0.05 * rng.uniform(-1.0, 1.0, (9,) + shape_i))
+    if consistent_h:
+        base[HH] = base[H2U] + 2.0 * base[MU]
     return grid, base, nt
 
 
@@ -56,9 +66,10 @@ def line_receivers(grid):
     return [(f, x + sy * nx + sz * nx * ny) for f in ("txx", "tzz", "vux", "vwx", "p") for x in range(nx)]
 
 
+@pytest.mark.parametrize("consistent_h", [False, True], ids=["H-streamed", "H-formed"])
 @pytest.mark.parametrize("shape,np_", SHAPES)
-def test_ragged_grids_match_oracle(shape, np_):
-    grid, base, nt = hetero_case(shape, np_)
+def test_ragged_grids_match_oracle(shape, np_, consistent_h):
+    grid, base, nt = hetero_case(shape, np_, consistent_h=consistent_h)
     recs = line_receivers(grid)
     g32, _ = cases.run_gpu(grid, nt, "fp32", recs, base=base, fields=FIELDS)
     ref32 = cases.run_oracle("f32", grid, nt, recs, base=base, final=True)
@@ -94,16 +105,18 @@ def test_ragged_tilings_are_bit_identical(shape, np_, src):
     edge or CPML) in the last tile of every width. The source sits near the high
     x end so that the wave crosses those tile edges within the run."""
     grid, base, _ = hetero_case(shape, np_, src=src)
+    _, base_c, _ = hetero_case(shape, np_, src=src, consistent_h=True)
     nt = 80
     recs = line_receivers(grid)
     bad = []
-    for prec in ("fp32", "comp16", "comp16f", "bf16", "fp64"):
+    for prec, b in (("fp32", base), ("fp32", base_c), ("comp16", base), ("comp16f", base_c), ("bf16", base),
+                    ("fp64", base)):
         outs = []
         for tiling in ({},) + TILINGS:
             t = dict(tiling)
             if prec == "fp64" and t.get("cells") == 4:
                 t["cells"] = 2
-            g, _ = cases.run_gpu(dict(grid, tiling=t), nt, prec, recs, base=base, fields=FIELDS)
+            g, _ = cases.run_gpu(dict(grid, tiling=t), nt, prec, recs, base=b, fields=FIELDS)
             outs.append(g)
         ref = outs[0]
         assert np.abs(ref["seis"]).max() > 0, prec
@@ -186,6 +199,36 @@ def test_api_misuse_is_refused_and_the_handle_stays_usable():
     seis = s.seismogram()
     assert seis.shape == (2, nt) and np.isfinite(seis).all() and np.abs(seis).max() > 0
     s.close()
+
+
+def test_h_consistency_selects_the_stress_coefficients():
+    """pw_set_medium compares H with (H - 2 mu) + 2 mu: FP32 streams H only when
+    they differ (4 B per cell more in the stress kernel), and comp16f, which
+    always forms H, refuses such a medium and then takes a consistent one."""
+    shape, np_ = (20, 18, 22), 3
+    _, base_i, nt = hetero_case(shape, np_, nt=12)
+    _, base_c, _ = hetero_case(shape, np_, nt=12, consistent_h=True)
+    s = pw.Solver(*shape, np_, 5.0, 0.5e-3, nt, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
+    s.set_medium(base_i)
+    streamed = s.info()["bytes_stress"]
+    s.set_medium(base_c)
+    formed = s.info()["bytes_stress"]
+    # H rounded to float on its own stays within the tolerance (8 FP32 ulps)
+    b32 = base_c.astype(np.float32).astype(np.float64)
+    s.set_medium(b32)
+    assert s.info()["bytes_stress"] == formed
+    s.close()
+    assert streamed - formed == 4.0, (streamed, formed)
+    c = pw.Solver(*shape, np_, 5.0, 0.5e-3, nt, precision="comp16f", cpml_r=1e-4, cpml_v0=4000.0)
+    with pytest.raises(pw.PorowaveError) as ei:
+        c.set_medium(base_i)
+    assert ei.value.status == 6 and "H - 2 mu" in str(ei.value)
+    c.set_medium(base_c)
+    c.set_source(10, 9, 11, 30.0, (1.0, 1.0, 1.0, -1.0))
+    c.set_receivers(["p"], [11 + 9 * 20 + 11 * 20 * 18])
+    c.run(nt)
+    assert np.isfinite(c.seismogram()).all() and np.abs(c.seismogram()).max() > 0
+    c.close()
 
 
 def test_no_receivers_zero_steps_and_overrun():
