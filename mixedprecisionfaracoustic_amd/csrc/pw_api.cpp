@@ -531,6 +531,9 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
         return fail(PW_ERR_UNSUPPORTED, "pw_set_medium: comp16f forms H as (H - 2 mu) + 2 mu, but this medium's H "
                                         "differs from it by " + std::to_string(res[kSlotHMismatch]) + " of H");
     s->b.derive_h = s->prec == PW_FP32 && h_consistent ? 1 : 0;
+    // one fluid throughout (as in the reference's model, Float/main.cu:1632):
+    // FP32's velocity kernel takes rhof as a scalar, the same float value
+    s->b.rhof_uniform = s->prec == PW_FP32 && res[kSlotRhofSpread] == 0.0 ? res[4 + PW_RHOF] : 0.0;
     s->vp_max = std::sqrt(res[2]);
     // von Neumann limit of the staggered leapfrog: vp dt / h * sqrt(3) * sum|c_k| <= 1,
     // sum|c_k| = 1 (2nd order) or 9/8 + 1/24 = 7/6 (4th order)
@@ -1101,9 +1104,11 @@ extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
     // coefficient arrays: scaled form 15 (velocity) + 7 (stress) FP16 arrays;
     // physical form 4 (rho, rhof, C1, C2) + 5 (C, M, H, H-2mu, mu) cell-centred
     // arrays (comp16f, and FP32 on a medium whose H is (H-2mu) + 2mu, form H
-    // from H-2mu and mu; comp16f: two records of 4 compensated values)
+    // from H-2mu and mu; FP32 takes a uniform rhof as a scalar; comp16f: two
+    // records of 4 compensated values)
     const bool form_h = s->prec == PW_COMP16F || s->b.derive_h;
-    const int cv = is_scaled(s->prec) ? 15 : 4, ct = is_scaled(s->prec) ? 7 : (form_h ? 4 : 5);
+    const int cv = is_scaled(s->prec) ? 15 : (s->b.rhof_uniform > 0 ? 3 : 4),
+              ct = is_scaled(s->prec) ? 7 : (form_h ? 4 : 5);
     // memory variables: each slot of a kernel read and written per damped axis
     const double mt = mem_slots(s->prec) * f * mem_bytes(s->prec);
     out->bytes_velocity = 19.0 * state_bytes(s->prec) + cv * coef_bytes(s->prec) + mt;

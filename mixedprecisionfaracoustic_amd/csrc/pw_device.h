@@ -175,6 +175,7 @@ struct DevBuffers {
     float sc_stress, sc_vel, sc_vw;  // comp16f: stored value = physical * scale (powers of two)
     int xcd_map;     // 1: XCD-aware block -> tile mapping in the step kernels
     int derive_h;    // FP32: the stress kernel forms H as (H - 2 mu) + 2 mu (consistent medium)
+    double rhof_uniform;  // FP32, > 0: the medium's one rhof, which the velocity kernel takes as a scalar
 };
 
 // Step kernels over the planes [zb, ze) (the updated planes are 1 .. nz-2).
@@ -251,10 +252,13 @@ struct MediumArgs {
 // res[3] = max rhof / CC1 (the size of the relative fluid velocity against the solid's),
 // res[4 + f] = max |base field f| (f < PW_NBASE),
 // res[kSlotHMismatch] = max |H - (H - 2 mu) - 2 mu| / |H| (0 when the medium's H
-// and H - 2 mu come from one formula, as in the reference's model builders).
+// and H - 2 mu come from one formula, as in the reference's model builders),
+// res[kSlotRhofSpread] = max |rhof - rhof(first cell)| (1 if that rhof is not
+// positive): 0 when one fluid fills the medium.
 // d_res (kReduceSlots doubles) is cleared by a fill kernel first.
 constexpr int kSlotHMismatch = 4 + PW_NBASE;
-constexpr int kReduceSlots = kSlotHMismatch + 1;
+constexpr int kSlotRhofSpread = kSlotHMismatch + 1;
+constexpr int kReduceSlots = kSlotRhofSpread + 1;
 constexpr double kHMismatchTol = 1e-6;  // 8 FP32 ulps
 hipError_t medium_reduce(const MediumArgs& m, double* d_res, hipStream_t st);
 // The scratch of the allocation-free medium installation (pw_set_medium_device):

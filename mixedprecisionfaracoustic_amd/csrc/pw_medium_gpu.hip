@@ -94,7 +94,8 @@ __global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double dt, d
     __shared__ double sh[kReduceSlots][256];
     const long long n = (long long)nx * ny * nz;
     double kmax = 0.0, rmax = 0.0, vmax = 0.0, wmax = 0.0;
-    double fmx[PW_NBASE] = {}, hmis = 0.0;
+    double fmx[PW_NBASE] = {}, hmis = 0.0, rspread = 0.0;
+    const double rf0 = v.at(PW_RHOF, 0, 0, 0);
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const int ix = (int)(i % nx), iy = (int)((i / nx) % ny), iz = (int)(i / ((long long)nx * ny));
@@ -126,10 +127,14 @@ __global__ void k_medium_reduce(BaseView v, int nx, int ny, int nz, double dt, d
         // how far H is from (H - 2 mu) + 2 mu (NaN counts as far)
         const double dh = fabs(HH - H2u - 2.0 * mu);
         hmis = fmax(hmis, dh <= 0.0 ? 0.0 : (dh < fabs(HH) ? dh / fabs(HH) : 1.0));
+        // one fluid throughout? (NaN counts as not)
+        const double dr = fabs(rhof - rf0);
+        rspread = fmax(rspread, rf0 > 0.0 && dr <= 0.0 ? 0.0 : (dr > 0.0 ? dr : 1.0));
     }
     sh[0][threadIdx.x] = kmax; sh[1][threadIdx.x] = rmax; sh[2][threadIdx.x] = vmax; sh[3][threadIdx.x] = wmax;
     for (int f = 0; f < PW_NBASE; ++f) sh[4 + f][threadIdx.x] = fmx[f];
     sh[kSlotHMismatch][threadIdx.x] = hmis;
+    sh[kSlotRhofSpread][threadIdx.x] = rspread;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)

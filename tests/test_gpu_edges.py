@@ -106,10 +106,12 @@ def test_ragged_tilings_are_bit_identical(shape, np_, src):
     x end so that the wave crosses those tile edges within the run."""
     grid, base, _ = hetero_case(shape, np_, src=src)
     _, base_c, _ = hetero_case(shape, np_, src=src, consistent_h=True)
+    base_cu = base_c.copy()  # one fluid: FP32 forms H and takes rhof as a scalar
+    base_cu[pw.BASE.index("rhof")] = 1000.0
     nt = 80
     recs = line_receivers(grid)
     bad = []
-    for prec, b in (("fp32", base), ("fp32", base_c), ("comp16", base), ("comp16f", base_c), ("bf16", base),
+    for prec, b in (("fp32", base), ("fp32", base_cu), ("comp16", base), ("comp16f", base_c), ("bf16", base),
                     ("fp64", base)):
         outs = []
         for tiling in ({},) + TILINGS:
@@ -201,22 +203,33 @@ def test_api_misuse_is_refused_and_the_handle_stays_usable():
     s.close()
 
 
-def test_h_consistency_selects_the_stress_coefficients():
+def test_medium_properties_select_the_coefficient_streams():
     """pw_set_medium compares H with (H - 2 mu) + 2 mu: FP32 streams H only when
     they differ (4 B per cell more in the stress kernel), and comp16f, which
-    always forms H, refuses such a medium and then takes a consistent one."""
+    always forms H, refuses such a medium and then takes a consistent one.  A
+    medium with one rhof gives FP32's velocity kernel a scalar instead of the
+    rhof array (4 B per cell less); a rhof that differs in one cell does not."""
     shape, np_ = (20, 18, 22), 3
     _, base_i, nt = hetero_case(shape, np_, nt=12)
     _, base_c, _ = hetero_case(shape, np_, nt=12, consistent_h=True)
     s = pw.Solver(*shape, np_, 5.0, 0.5e-3, nt, precision="fp32", cpml_r=1e-4, cpml_v0=4000.0)
     s.set_medium(base_i)
-    streamed = s.info()["bytes_stress"]
+    streamed, vel_arr = s.info()["bytes_stress"], s.info()["bytes_velocity"]
     s.set_medium(base_c)
     formed = s.info()["bytes_stress"]
+    assert s.info()["bytes_velocity"] == vel_arr
     # H rounded to float on its own stays within the tolerance (8 FP32 ulps)
     b32 = base_c.astype(np.float32).astype(np.float64)
     s.set_medium(b32)
     assert s.info()["bytes_stress"] == formed
+    rhof = pw.BASE.index("rhof")
+    b_u = base_c.copy()
+    b_u[rhof] = 1000.0
+    s.set_medium(b_u)
+    assert vel_arr - s.info()["bytes_velocity"] == 4.0
+    b_u[rhof][5, 6, 7] = 1000.5
+    s.set_medium(b_u)
+    assert s.info()["bytes_velocity"] == vel_arr
     s.close()
     assert streamed - formed == 4.0, (streamed, formed)
     c = pw.Solver(*shape, np_, 5.0, 0.5e-3, nt, precision="comp16f", cpml_r=1e-4, cpml_v0=4000.0)
