@@ -168,7 +168,7 @@ pw_status pw_set_medium(pw_solver* s, const double* const* base, int nxi, int ny
  * pw_set_medium takes them in host memory; the installed medium is
  * bit-identical.  The medium's maxima (stability check, Half exponents, comp16f
  * coefficient scales) are reduced into a file-scope device array of 64 doubles
- * instead of an allocation; the reduction needs 13 of them whatever the grid
+ * instead of an allocation; the reduction needs 15 of them whatever the grid
  * size.  Every call
  * on a device shares that device's array and clears it first; calls in one
  * process are serialised on it (calls that overlapped on one device would
