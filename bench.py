@@ -142,9 +142,9 @@ def serving_trace(nshots, rate, seed, n_vol, np_):
 
 
 def run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=False, **server_kw):
-    """(server_kw: pw.Server keyword arguments; none gives the server's defaults.)"""
     """Replays the trace against one server: each request is submitted at its
-    arrival time; returns per-shot timings and the server's statistics."""
+    arrival time; returns per-shot timings and the server's statistics.
+    server_kw: pw.Server keyword arguments; none gives the server's defaults."""
     import threading
     import numpy as np
     import mixedprecisionfaracoustic_amd as pw
@@ -199,7 +199,7 @@ def serving_main(args):
     print(f"serving: {len(trace)} shots over {trace[-1]['t']:.1f} s of arrivals", file=sys.stderr, flush=True)
     batched, seis_b = run_serving(trace, n_vol, np_, h, dt, spec, max_active=args.slots, round_steps=args.round_steps,
                                   use_graphs=not args.no_graphs, batched=True, fill_cells=args.fill_cells,
-                                  persistent=args.persistent)
+                                  persistent=args.persistent, max_wait_s=args.max_wait)
     print("  batched:", json.dumps(batched), file=sys.stderr, flush=True)
     seq, seis_s = run_serving(trace, n_vol, np_, h, dt, spec, force_fp32=True, max_active=1, round_steps=64,
                               use_graphs=False, batched=False)
@@ -218,7 +218,7 @@ def serving_main(args):
                       "seed": args.seed, "grids": [96, 128, 192, 256], "steps": [500, 1500],
                       "precisions": "fp32/comp16f 50/50", "slots": args.slots, "round_steps": args.round_steps,
                       "graphs": not args.no_graphs, "fill_cells": args.fill_cells,
-                      "persistent": args.persistent},
+                      "persistent": args.persistent, "max_wait_s": args.max_wait},
            "serving": batched, "sequential_fp32": seq,
            "throughput_ratio": round(batched["gcell_steps_per_s"] / seq["gcell_steps_per_s"], 3),
            "p50_ratio": round(batched["p50_latency_s"] / seq["p50_latency_s"], 3),
@@ -625,6 +625,9 @@ def main():
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--round-steps", type=int, default=32)
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--max-wait", type=float, default=0.0,
+                    help="--serving: admit the least-work shot first unless one has waited this many seconds "
+                         "(0: first come first served)")
     ap.add_argument("--persistent", action="store_true",
                     help="--serving: the server runs each round as one persistent cooperative launch")
     ap.add_argument("--fill-cells", type=int, default=None,

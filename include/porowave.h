@@ -389,6 +389,12 @@ typedef struct pw_server_config {
                                launch (as many blocks as the device holds at once, grid
                                barriers between the phases of a step) instead of one launch
                                per step kernel; results are bit-identical.  Needs batched */
+    double max_wait_s;      /* admission order.  0: first come first served.  > 0: the queued
+                               shot with the least work (cells x steps) first, except that
+                               shots queued longer than max_wait_s seconds go first, oldest
+                               first: short shots pass long ones, which wait at most about
+                               that long before they are next.  No counterpart in the
+                               reference */
 } pw_server_config;
 
 typedef struct pw_shot {

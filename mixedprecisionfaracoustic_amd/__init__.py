@@ -67,7 +67,7 @@ class Layered(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("stream", C.c_void_p), ("hbm_budget", C.c_longlong),
                 ("max_active", C.c_int), ("round_steps", C.c_int), ("use_graphs", C.c_int), ("batched", C.c_int),
-                ("fill_cells", C.c_longlong), ("persistent", C.c_int)]
+                ("fill_cells", C.c_longlong), ("persistent", C.c_int), ("max_wait_s", C.c_double)]
 
 
 class ShotSpec(C.Structure):
@@ -487,15 +487,16 @@ def reference_solver(precision="fp32", nt=3500, **kw):
 
 
 class Server:
-    """Shot server (a pw_server handle): admits shots first come, first served by
-    device-memory budget and slot count, keeps them resident and advances all
+    """Shot server (a pw_server handle): admits shots first come, first served (or,
+    with max_wait_s > 0, least work first unless a shot has waited longer than
+    that) by device-memory budget and slot count, keeps them resident and advances all
     active shots together (one launch per step kernel per precision, rounds of
     steps replayed as hipGraphs; or, with persistent=True, one persistent
     cooperative launch per precision and round).  Shots crop their medium from
     one volume set with set_medium / set_medium_layered."""
 
     def __init__(self, device=0, max_active=16, round_steps=32, use_graphs=True, batched=True, hbm_budget=0,
-                 stream=None, fill_cells=None, persistent=False):
+                 stream=None, fill_cells=None, persistent=False, max_wait_s=0.0):
         L = lib()
         cfg = ServerConfig()
         L.pw_server_config_default(C.byref(cfg))
@@ -505,6 +506,7 @@ class Server:
         if fill_cells is not None:  # None keeps the library's default
             cfg.fill_cells = int(fill_cells)
         cfg.persistent = int(bool(persistent))
+        cfg.max_wait_s = float(max_wait_s)
         h = C.c_void_p()
         _check(L.pw_server_create(C.byref(cfg), C.byref(h)))
         self._h = h

@@ -314,6 +314,23 @@ void finish(pw_server* srv, Shot* q, pw_status st, const std::string& msg)
     srv->cv_done.notify_all();
 }
 
+// The queued shot admission considers next: the front (first come first
+// served), or with max_wait_s > 0 the oldest shot queued longer than that, else
+// the one with the least work (cells x steps; ties: the earlier one).
+std::deque<Shot*>::iterator next_pending(pw_server* srv)
+{
+    auto& pq = srv->pending;
+    if (srv->cfg.max_wait_s <= 0.0) return pq.begin();
+    const double now = srv->clock();
+    auto best = pq.begin();
+    for (auto it = pq.begin(); it != pq.end(); ++it) {  // queue order is arrival order
+        if (now - (*it)->t_submit > srv->cfg.max_wait_s) return it;
+        const double w = (double)(*it)->cells * (*it)->req.nt, wb = (double)(*best)->cells * (*best)->req.nt;
+        if (w < wb) best = it;
+    }
+    return best;
+}
+
 void worker_main(pw_server* srv)
 {
     (void)hipSetDevice(srv->cfg.device);
@@ -322,9 +339,10 @@ void worker_main(pw_server* srv)
     for (;;) {
         // 1. admission, first come first served
         while (!srv->stop && !srv->pending.empty() && (int)srv->active.size() < srv->cfg.max_active) {
-            Shot* q = srv->pending.front();
+            const auto qi = next_pending(srv);
+            Shot* q = *qi;
             if (q->bytes > srv->budget) {
-                srv->pending.pop_front();
+                srv->pending.erase(qi);
                 finish(srv, q, PW_ERR_OUT_OF_MEMORY,
                        "shot needs " + std::to_string(q->bytes) + " device bytes, above the server budget of " +
                            std::to_string(srv->budget));
@@ -338,7 +356,7 @@ void worker_main(pw_server* srv)
                 for (const Shot* a : srv->active) held += a->cells;
                 if (held + q->cells > srv->cfg.fill_cells) break;
             }
-            srv->pending.pop_front();
+            srv->pending.erase(qi);
             q->state = PW_SHOT_RUNNING;
             q->t_admit = srv->clock();
             srv->admitting++;
@@ -512,6 +530,7 @@ extern "C" void pw_server_config_default(pw_server_config* c)
     c->batched = 1;
     c->fill_cells = 1LL << 23;
     c->persistent = 0;
+    c->max_wait_s = 0.0;
 }
 
 extern "C" pw_status pw_server_create(const pw_server_config* cfg, pw_server** out)
@@ -525,6 +544,8 @@ extern "C" pw_status pw_server_create(const pw_server_config* cfg, pw_server** o
         return fail(PW_ERR_INVALID, "pw_server_create: persistent must be 0 or 1");
     if (cfg->persistent && !cfg->batched)
         return fail(PW_ERR_INVALID, "pw_server_create: persistent rounds need batched rounds");
+    if (!(cfg->max_wait_s >= 0.0) || !std::isfinite(cfg->max_wait_s))
+        return fail(PW_ERR_INVALID, "pw_server_create: max_wait_s must be finite and >= 0");
     int ndev = 0;
     PW_HIP_TRY(hipGetDeviceCount(&ndev));
     if (cfg->device < 0 || cfg->device >= ndev) return fail(PW_ERR_INVALID, "pw_server_create: no such device");

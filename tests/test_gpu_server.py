@@ -176,6 +176,35 @@ def test_cell_limit_limits_admission(volume, alone):
     assert [s["t_admit"] for s in st] == sorted(s["t_admit"] for s in st)
 
 
+@pytest.mark.parametrize("max_wait,expect", [(0.0, [0, 1, 2, 3, 4, 5]), (1000.0, [4, 2, 3, 0, 5, 1]),
+                                             (1e-6, [0, 1, 2, 3, 4, 5])])
+def test_admission_order_follows_max_wait(volume, alone, max_wait, expect):
+    """max_wait_s: with one slot, the six shots queued behind a long shot are
+    admitted first come first served (0); least work first (cells x steps:
+    1.97, 2.35, 2.57, 2.62, 2.88, 4.92 M for shots 4, 2, 3, 0, 5, 1) under a
+    bound none of them reaches; oldest first again once all have waited past a
+    tiny bound.  Results are unchanged."""
+    import time
+    shots, ref = alone
+    work = [q["nx"] * q["ny"] * q["nz"] * q["nt"] for q in shots]
+    assert sorted(range(6), key=lambda k: work[k]) == [4, 2, 3, 0, 5, 1]
+    with pw.Server(max_active=1, round_steps=8, max_wait_s=max_wait) as srv:
+        srv.set_medium(volume)
+        first = submit(srv, shots[2], nt=4000)  # holds the slot while the others queue
+        while srv.status(first)["state"] == "queued":
+            time.sleep(0.001)
+        ids = [submit(srv, q) for q in shots]
+        for i in [first] + ids:
+            srv.wait(i, timeout=120)
+        st = [srv.status(i) for i in ids]
+        t_first = srv.status(first)["t_done"]
+        res = [srv.result(i) for i in ids]
+    assert all(s["t_submit"] < t_first for s in st), "the long shot ended before the queue filled"
+    assert sorted(range(6), key=lambda k: st[k]["t_admit"]) == expect
+    for a, b in zip(res, ref):
+        assert np.array_equal(a, b)
+
+
 def test_failing_shots_do_not_disturb_others(volume, alone):
     shots, ref = alone
     with pw.Server(max_active=8, hbm_budget=1 << 30) as srv:

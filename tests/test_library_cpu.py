@@ -149,6 +149,10 @@ def test_config_validation_errors():
     with pytest.raises(pw.PorowaveError) as ei:
         pw.Server(persistent=True, batched=False)
     assert ei.value.status == 1 and "persistent" in str(ei.value)
+    for bad in (-1.0, float("nan"), float("inf")):
+        with pytest.raises(pw.PorowaveError) as ei:
+            pw.Server(max_wait_s=bad)
+        assert ei.value.status == 1 and "max_wait_s" in str(ei.value)
 
 
 def test_binary_files_round_trip(tmp_path):
