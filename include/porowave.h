@@ -30,7 +30,9 @@
 extern "C" {
 #endif
 
-#define PW_ABI_VERSION 2
+/* Raised whenever a struct below changes size or layout (3: pw_server_config
+ * gained persistent and max_wait_s). */
+#define PW_ABI_VERSION 3
 
 typedef struct pw_solver pw_solver;
 
