@@ -649,7 +649,7 @@ def main():
     ap.add_argument("--host-job", action="store_true",
                     help="time whole jobs from host buffers (set-up, medium upload, run, read-back) instead")
     ap.add_argument("--job-steps", type=int, default=2000)
-    ap.add_argument("--job-precisions", default="fp32,comp16")
+    ap.add_argument("--job-precisions", default="fp32,comp16f,comp16")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
