@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -311,6 +312,8 @@ void plan_layout(const pw_config& c, Layout& L)
     L.zero_bytes = L.total;
     for (int k = 0; k < kMaxCoef; ++k)
         if (coef_used(prec, k)) take(cells * cb);
+    if (has_srec(prec))
+        for (int r = 0; r < kScRecords; ++r) take(cells * 4 * sizeof(uint16_t));
     if (has_records(prec))
         for (int r = 0; r < kRecords; ++r) {
             take(cells * 2 * rec_width(r));  // the FP16 values
@@ -375,6 +378,11 @@ pw_status pw::solver_create(const pw_config* cfg, pw_solver** out)
             s->b.mem[ax][k] = used ? base + off[i++] : nullptr;
         }
     for (int k = 0; k < kMaxCoef; ++k) s->b.coef[k] = coef_used(s->prec, k) ? base + off[i++] : nullptr;
+    if (has_srec(s->prec)) {
+        char* rec[kScRecords];
+        for (int r = 0; r < kScRecords; ++r) rec[r] = base + off[i++];
+        for (int k = 0; k < SC_N; ++k) s->b.coef[k] = rec[sc_rec(k)] + sc_slot(k) * sizeof(uint16_t);
+    }
     for (int r = 0; r < kRecords; ++r) {
         s->b.rec_hi[r] = has_records(s->prec) ? base + off[i++] : nullptr;
         s->b.rec_lo[r] = has_records(s->prec) ? base + off[i++] : nullptr;
@@ -1107,8 +1115,10 @@ extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
     // from H-2mu and mu; FP32 takes a uniform rhof as a scalar; comp16f: two
     // records of 4 compensated values)
     const bool form_h = s->prec == PW_COMP16F || s->b.derive_h;
-    const int cv = is_scaled(s->prec) ? 15 : (s->b.rhof_uniform > 0 ? 3 : 4),
-              ct = is_scaled(s->prec) ? 7 : (form_h ? 4 : 5);
+    // (comp16: 15 and 7 coefficients in records of 4, so 16 and 8 FP16 values
+    // move)
+    const int cv = has_srec(s->prec) ? 16 : is_scaled(s->prec) ? 15 : (s->b.rhof_uniform > 0 ? 3 : 4),
+              ct = has_srec(s->prec) ? 8 : is_scaled(s->prec) ? 7 : (form_h ? 4 : 5);
     // memory variables: each slot of a kernel read and written per damped axis
     const double mt = mem_slots(s->prec) * f * mem_bytes(s->prec);
     out->bytes_velocity = 19.0 * state_bytes(s->prec) + cv * coef_bytes(s->prec) + mt;

@@ -100,8 +100,33 @@ inline bool has_lo(int prec) { return prec == PW_COMP16 || prec == PW_COMP16F; }
 // per slot (their rescale walks the slots, comp16 pairs psi with prev).
 inline bool mem_records(int prec) { return !is_scaled(prec); }
 inline int ncoef(int prec) { return is_scaled(prec) ? (int)SC_N : (int)PC_N; }
-// the coefficient arrays of a precision (comp16f holds records instead, below)
-inline bool coef_used(int prec, int k) { return k < ncoef(prec) && prec != PW_COMP16F; }
+// comp16 keeps the 22 FP16 coefficients of the scaled form in records (below)
+inline bool has_srec(int prec) { return prec == PW_COMP16; }
+// the coefficient arrays of a precision (comp16f and comp16 hold records
+// instead, below)
+inline bool coef_used(int prec, int k) { return k < ncoef(prec) && prec != PW_COMP16F && !has_srec(prec); }
+// comp16 keeps the 22 FP16 coefficients as records of four per cell (8 B, one
+// access each; 4 % faster than 22 arrays, although the two unused slots add
+// 4 B per cell-step; BF16, whose narrower state already moves its bytes at a
+// higher rate, keeps the arrays): per axis d, record d = {Aw, Bp, Bt, rho~^-1};
+// record 3 = {rhof rho^-1 along x, y, z, 0}; record 4 = {H~, H~ - 2mu~, C~, M~};
+// record 5 = {mu~xy, mu~xz, mu~yz, 0}.  coef[k] points at coefficient k's slot
+// of its record (element stride 4).
+constexpr int kScRecords = 6;
+__host__ __device__ constexpr int sc_rec(int k)
+{
+    return k <= SC_RITZ ? k - SC_RITX : k <= SC_FRZ ? 3 : k <= SC_AWZ ? k - SC_AWX : k <= SC_BPZ ? k - SC_BPX
+         : k <= SC_BTZ ? k - SC_BTX : k <= SC_M ? 4 : 5;
+}
+__host__ __device__ constexpr int sc_slot(int k)
+{
+    return k <= SC_RITZ ? 3 : k <= SC_FRZ ? k - SC_FRX : k <= SC_AWZ ? 0 : k <= SC_BPZ ? 1 : k <= SC_BTZ ? 2
+         : k <= SC_M ? k - SC_HH : k - SC_MUXY;
+}
+static_assert(sc_rec(SC_AWY) == 1 && sc_slot(SC_AWY) == 0 && sc_rec(SC_RITZ) == 2 && sc_slot(SC_RITZ) == 3 &&
+                  sc_rec(SC_FRY) == 3 && sc_slot(SC_FRY) == 1 && sc_rec(SC_BTX) == 0 && sc_slot(SC_BTX) == 2 &&
+                  sc_rec(SC_M) == 4 && sc_slot(SC_M) == 3 && sc_rec(SC_MUYZ) == 5 && sc_slot(SC_MUYZ) == 2,
+              "scaled coefficient records");
 // comp16f keeps its coefficients as compensated records instead: per cell the
 // FP16 values of a record's coefficients side by side, and their residual
 // bytes side by side in a second array.  Record 0 holds the velocity kernel's

@@ -89,6 +89,18 @@ template <int N, class T> __device__ __forceinline__ void st_pack(T* p, unsigned
 {
     *(Pack<T, N>*)(p + o) = v;
 }
+// comp16 / BF16 coefficients: the 4-value FP16 records of cells o .. o+N-1 in
+// one access (p: the record array, i.e. its slot-0 coefficient); v[q][j] is
+// slot q of cell o + j
+template <int N, class R, class C>
+__device__ __forceinline__ void ld_srec(const C* p, unsigned o, R (&v)[4][N])
+{
+    const Pack<C, 4 * N> t = *(const Pack<C, 4 * N>*)(p + 4 * (size_t)o);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q][j] = (R)t.v[4 * j + q];
+}
 
 // the run-time zero of a field (round32); plain FP32/FP64 fields carry none
 template <class F> __device__ __forceinline__ unsigned field_zero(const F& f)
@@ -149,6 +161,7 @@ struct PolC16 {
     using R = float; using Coef = f16; using Mem = f16;
     static constexpr int form = kFormHalf;
     static constexpr bool scaled = true;
+    static constexpr bool srec = true;  // coefficients in 4-value records
     // the reference Half program's CPML: (psi, previous derivative) FP16 pairs
     static constexpr bool pair_cpml = true;
     struct Field { f16* hi; int8_t* lo; unsigned z; };
@@ -228,6 +241,7 @@ struct PolB16 {
     using R = float; using Coef = f16; using Mem = bf16;
     static constexpr int form = kFormHalf;
     static constexpr bool scaled = true;
+    static constexpr bool srec = false;
     static constexpr bool pair_cpml = false;
     struct Field { bf16* p; unsigned z; };
     static __device__ __forceinline__ R ld(const Field& f, unsigned o) { return (float)f.p[o]; }
@@ -1259,24 +1273,39 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             // vu += rho~^-1*S - rhof*rho^-1*(vw_new - vw_old), in the reference's
             // order of float operations, without contraction
 #pragma clang fp contract(off)
-            R aw[N], bp[N], bt[N], ri[N], fr[N];
-            LDC(SC_AWX, o, aw); LDC(SC_BPX, o, bp); LDC(SC_BTX, o, bt); LDC(SC_RITX, o, ri); LDC(SC_FRX, o, fr);
+            // {Aw, Bp, Bt, rho~^-1} per axis and {rhof rho^-1 x, y, z}: one
+            // record each (comp16) or four arrays (BF16)
+            auto ldq = [&](int k0, int k1, int k2, int k3, R (&v)[4][N]) {
+                if constexpr (P::srec) {
+                    ld_srec<N>(a.c[k0], o - L.sh, v);
+                } else {
+                    LDC(k0, o, v[0]); LDC(k1, o, v[1]); LDC(k2, o, v[2]);
+                    if (k3 >= 0) LDC(k3, o, v[3]);
+                }
+            };
+            R fr3[4][N], ca[4][N];
+            R (&aw)[N] = ca[0];
+            R (&bp)[N] = ca[1];
+            R (&bt)[N] = ca[2];
+            R (&ri)[N] = ca[3];
+            ldq(SC_FRX, SC_FRY, SC_FRZ, -1, fr3);
+            ldq(SC_AWX, SC_BPX, SC_BTX, SC_RITX, ca);
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 wx1[j] = (aw[j] == 0.0f && bp[j] == 0.0f && bt[j] == 0.0f) ? 0.0f : aw[j] * wx0[j] - bt[j] * sx[j] - bp[j] * s1[j];
-                ux1[j] = ux0[j] + ri[j] * sx[j] - fr[j] * (wx1[j] - wx0[j]);
+                ux1[j] = ux0[j] + ri[j] * sx[j] - fr3[0][j] * (wx1[j] - wx0[j]);
             }
-            LDC(SC_AWY, o, aw); LDC(SC_BPY, o, bp); LDC(SC_BTY, o, bt); LDC(SC_RITY, o, ri); LDC(SC_FRY, o, fr);
+            ldq(SC_AWY, SC_BPY, SC_BTY, SC_RITY, ca);
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 wy1[j] = (aw[j] == 0.0f && bp[j] == 0.0f && bt[j] == 0.0f) ? 0.0f : aw[j] * wy0[j] - bt[j] * sy[j] - bp[j] * s2[j];
-                uy1[j] = uy0[j] + ri[j] * sy[j] - fr[j] * (wy1[j] - wy0[j]);
+                uy1[j] = uy0[j] + ri[j] * sy[j] - fr3[1][j] * (wy1[j] - wy0[j]);
             }
-            LDC(SC_AWZ, o, aw); LDC(SC_BPZ, o, bp); LDC(SC_BTZ, o, bt); LDC(SC_RITZ, o, ri); LDC(SC_FRZ, o, fr);
+            ldq(SC_AWZ, SC_BPZ, SC_BTZ, SC_RITZ, ca);
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 wz1[j] = (aw[j] == 0.0f && bp[j] == 0.0f && bt[j] == 0.0f) ? 0.0f : aw[j] * wz0[j] - bt[j] * sz[j] - bp[j] * s3[j];
-                uz1[j] = uz0[j] + ri[j] * sz[j] - fr[j] * (wz1[j] - wz0[j]);
+                uz1[j] = uz0[j] + ri[j] * sz[j] - fr3[2][j] * (wz1[j] - wz0[j]);
             }
         }
         if constexpr (C16F) {
@@ -1601,8 +1630,21 @@ __device__ __forceinline__ void stress_tile(const StepArgs<P>& a, const Tile tl)
                 LDC(PC_HH, o, HH_);
             }
         } else {
-            LDC(SC_C, o, C_); LDC(SC_M, o, M_); LDC(SC_HH, o, HH_); LDC(SC_H2U, o, H2u_);
-            LDC(SC_MUXY, o, mxy); LDC(SC_MUYZ, o, myz); LDC(SC_MUXZ, o, mxz);
+            // {H~, H~ - 2mu~, C~, M~} and {mu~xy, mu~xz, mu~yz}: one record
+            // each (comp16) or an array each (BF16)
+            R t4[4][N], m4[4][N];
+            if constexpr (P::srec) {
+                ld_srec<N>(a.c[SC_HH], o - L.sh, t4);
+                ld_srec<N>(a.c[SC_MUXY], o - L.sh, m4);
+            } else {
+                LDC(SC_HH, o, t4[0]); LDC(SC_H2U, o, t4[1]); LDC(SC_C, o, t4[2]); LDC(SC_M, o, t4[3]);
+                LDC(SC_MUXY, o, m4[0]); LDC(SC_MUXZ, o, m4[1]); LDC(SC_MUYZ, o, m4[2]);
+            }
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                HH_[j] = t4[0][j]; H2u_[j] = t4[1][j]; C_[j] = t4[2][j]; M_[j] = t4[3][j];
+                mxy[j] = m4[0][j]; mxz[j] = m4[1][j]; myz[j] = m4[2][j];
+            }
         }
 #pragma unroll
         for (int j = 0; j < N; ++j) {

@@ -148,6 +148,7 @@ template <class T> __device__ __forceinline__ T to_store(double v) { return (T)v
 
 struct CoefOut {
     void* c[kMaxCoef];
+    int cstride;  // elements between a coefficient's consecutive cells (scaled form)
     int pitch;
     long long plane;
 };
@@ -220,6 +221,9 @@ __global__ void k_coef_scaled(BaseView v, int nx, int ny, int nz, float DT, floa
     const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
     if (ix >= nx) return;
     const size_t o = (size_t)ix + (size_t)iy * out.pitch + (size_t)iz * out.plane;
+    // comp16: the cell's record, c[k] pointing at coefficient k's slot in it;
+    // BF16: one array per coefficient
+    const size_t r = o * (size_t)out.cstride;
     f16** c = (f16**)out.c;
     auto F = [&](int f, int dx, int dy, int dz) { return (float)v.at(f, ix + dx, iy + dy, iz + dz); };
     const float rhof0 = F(PW_RHOF, 0, 0, 0);
@@ -250,19 +254,23 @@ __global__ void k_coef_scaled(BaseView v, int nx, int ny, int nz, float DT, floa
             bp = ism * DT / den;
             bt = ism * DT * rhof0 * ri[d] / den;
         }
-        c[SC_RITX + d][o] = (f16)round32(ism * DT * ri[d], z);
-        c[SC_FRX + d][o] = (f16)round32(rhof0 * ri[d], z);
-        c[SC_AWX + d][o] = (f16)round32(aw, z);
-        c[SC_BPX + d][o] = (f16)round32(bp, z);
-        c[SC_BTX + d][o] = (f16)round32(bt, z);
+        c[SC_RITX + d][r] = (f16)round32(ism * DT * ri[d], z);
+        c[SC_FRX + d][r] = (f16)round32(rhof0 * ri[d], z);
+        c[SC_AWX + d][r] = (f16)round32(aw, z);
+        c[SC_BPX + d][r] = (f16)round32(bp, z);
+        c[SC_BTX + d][r] = (f16)round32(bt, z);
     }
-    c[SC_HH][o] = (f16)round32(sm * DT * F(PW_HH, 0, 0, 0), z);
-    c[SC_H2U][o] = (f16)round32(sm * DT * F(PW_H2U, 0, 0, 0), z);
-    c[SC_C][o] = (f16)round32(sm * DT * F(PW_CC, 0, 0, 0), z);
-    c[SC_M][o] = (f16)round32(sm * DT * F(PW_MM, 0, 0, 0), z);
-    c[SC_MUXZ][o] = (f16)round32(sm * DT * muxz, z);
-    c[SC_MUXY][o] = (f16)round32(sm * DT * muxy, z);
-    c[SC_MUYZ][o] = (f16)round32(sm * DT * muyz, z);
+    c[SC_HH][r] = (f16)round32(sm * DT * F(PW_HH, 0, 0, 0), z);
+    c[SC_H2U][r] = (f16)round32(sm * DT * F(PW_H2U, 0, 0, 0), z);
+    c[SC_C][r] = (f16)round32(sm * DT * F(PW_CC, 0, 0, 0), z);
+    c[SC_M][r] = (f16)round32(sm * DT * F(PW_MM, 0, 0, 0), z);
+    c[SC_MUXZ][r] = (f16)round32(sm * DT * muxz, z);
+    c[SC_MUXY][r] = (f16)round32(sm * DT * muxy, z);
+    c[SC_MUYZ][r] = (f16)round32(sm * DT * muyz, z);
+    if (out.cstride == 4) {  // the unused fourth slots of records 3 and 5
+        c[SC_FRX][r + 3] = (f16)0.0f;
+        c[SC_MUXY][r + 3] = (f16)0.0f;
+    }
 }
 
 static BaseView make_view(const MediumArgs& m)
@@ -324,6 +332,7 @@ hipError_t medium_coefficients(const MediumArgs& m, const DevBuffers& b, double 
 {
     CoefOut out;
     for (int k = 0; k < kMaxCoef; ++k) out.c[k] = b.coef[k];
+    out.cstride = has_srec(b.prec) ? 4 : 1;
     out.pitch = b.g.pitch;
     out.plane = b.g.plane;
     const dim3 grid((unsigned)((m.nx + 127) / 128), (unsigned)m.ny, (unsigned)m.nz), block(128);
