@@ -299,15 +299,11 @@ void plan_layout(const pw_config& c, Layout& L)
             L.field_off[k] = place(cells * sb);
         }
     }
-    // memory variables: one array per slot, or (mem_records) one array of
-    // 4-variable records for each kernel's four terms of the axis
+    // memory variables: per axis one array of records for each kernel (the
+    // velocity kernel's, then the stress kernel's), a record per damped cell
     for (int ax = 0; ax < 3; ++ax) {
-        if (mem_records(prec)) {
-            take((size_t)g.slab_cells[ax] * 4 * mb);
-            take((size_t)g.slab_cells[ax] * 4 * mb);
-        } else {
-            for (int k = 0; k < mem_slots(prec); ++k) take((size_t)g.slab_cells[ax] * mb);
-        }
+        take((size_t)g.slab_cells[ax] * mem_rec_width(prec) * mb);
+        take((size_t)g.slab_cells[ax] * mem_rec_width(prec) * mb);
     }
     L.zero_bytes = L.total;
     for (int k = 0; k < kMaxCoef; ++k)
@@ -374,7 +370,7 @@ pw_status pw::solver_create(const pw_config* cfg, pw_solver** out)
     for (int k = 0; k < PW_NFIELDS; ++k) s->b.lo[k] = has_lo(s->prec) ? (int8_t*)(base + L.field_lo[k]) : nullptr;
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k < kMemPerAxis; ++k) {
-            const bool used = mem_records(s->prec) ? (k == 0 || k == 4) : k < mem_slots(s->prec);
+            const bool used = k == 0 || k == 4;  // the two record arrays
             s->b.mem[ax][k] = used ? base + off[i++] : nullptr;
         }
     for (int k = 0; k < kMaxCoef; ++k) s->b.coef[k] = coef_used(s->prec, k) ? base + off[i++] : nullptr;

@@ -94,11 +94,11 @@ constexpr int kMaxCoef = 22;
 inline bool is_scaled(int prec) { return prec == PW_COMP16 || prec == PW_BF16; }
 // the compensated FP16 state: FP16 "hi" arrays plus int8 "lo" residual arrays
 inline bool has_lo(int prec) { return prec == PW_COMP16 || prec == PW_COMP16F; }
-// CPML memory variables as 4-variable records (one per damped cell and axis and
-// kernel: mem[ax][0] the velocity kernel's, mem[ax][4] the stress kernel's) for
-// the precisions with one variable per term; the scaled forms keep one array
-// per slot (their rescale walks the slots, comp16 pairs psi with prev).
-inline bool mem_records(int prec) { return !is_scaled(prec); }
+// CPML memory variables as records, one per damped cell and axis and kernel
+// (mem[ax][0] the velocity kernel's, mem[ax][4] the stress kernel's): the four
+// variables of the axis's terms, or for comp16 the reference Half program's
+// (psi, previous derivative) pairs as {psi x4, prev x4}.
+inline int mem_rec_width(int prec) { return prec == PW_COMP16 ? 8 : 4; }
 inline int ncoef(int prec) { return is_scaled(prec) ? (int)SC_N : (int)PC_N; }
 // comp16 keeps the 22 FP16 coefficients of the scaled form in records (below)
 inline bool has_srec(int prec) { return prec == PW_COMP16; }

@@ -642,46 +642,52 @@ __device__ __forceinline__ Tile tile_of_block(int gx, int gy, int nblocks, int x
     return tile_of_index(xcd_remap(blockIdx.x, (unsigned)nblocks, xcd_map), gx, gy);
 }
 
-// One CPML term in single-variable form (see header comment), its memory
-// variable m already loaded from phi[k].
-template <class P>
-__device__ __forceinline__ typename P::R cpml_with(typename P::Mem* phi, unsigned k, typename P::R m, typename P::R c,
-                                                   typename P::R e, typename P::R der, unsigned z)
-{
-    using R = typename P::R;
-    const R psi = m - c * der;
-    P::stm(phi, k, e * (psi - c * der), z);
-    return der + psi;
-}
+// values in a CPML memory record: the four terms' variables, or comp16's
+// (psi, prev) pairs
+template <class P> __host__ __device__ constexpr int mem_width() { return P::pair_cpml ? 8 : 4; }
 
 // The four CPML terms of one damped axis.  All four memory variables are loaded
 // before any is stored: a store could alias the next load as far as the
 // compiler knows, so term-by-term code would make four memory round trips.
 // With P::pair_cpml the reference Half form instead (Half/main.cu:201-240):
 // psi = psi*e + (-c)*(e*prev + der) in float without contraction, psi and
-// prev = der stored as FP16, der += psi; prev lives 8 slots after psi.
+// prev = der stored as FP16, der += psi; the record is {psi x4, prev x4}.
+// Every form keeps the axis's variables of cell k as one record at
+// phi[0] + width*k, read and written with one access each.
 template <class P>
 __device__ __forceinline__ void cpml4(typename P::Mem* const* phi, unsigned k, const typename P::R (&c)[4],
                                       const typename P::R (&e)[4], typename P::R* const (&der)[4], unsigned z)
 {
     using R = typename P::R;
+    using M = typename P::Mem;
     if constexpr (P::pair_cpml) {
 #pragma clang fp contract(off)
-        R ps[4], pv[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            ps[q] = P::ldm(phi[q], k);
-            pv[q] = P::ldm(phi[q + 8], k);
-        }
+        M* rec = phi[0] + 8 * (size_t)k;
+        const Pack<M, 8> t = ld_pack<8>(rec, 0u);
+        Pack<M, 8> u;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const R d = *der[q];
-            const R nw = ps[q] * e[q] + (-c[q]) * (e[q] * pv[q] + d);
-            P::stm(phi[q], k, nw, z);
-            P::stm(phi[q + 8], k, d, z);
+            const R nw = (R)t.v[q] * e[q] + (-c[q]) * (e[q] * (R)t.v[q + 4] + d);
+            u.v[q] = (M)round32(nw, z);
+            u.v[q + 4] = (M)round32(d, z);
             *der[q] = d + nw;
         }
-    } else if constexpr (!P::scaled) {
+        st_pack<8>(rec, 0u, u);
+    } else if constexpr (P::scaled) {
+        // the single-variable form (file header), rounded as P::stm rounds
+        M* rec = phi[0] + 4 * (size_t)k;
+        const Pack<M, 4> t = ld_pack<4>(rec, 0u);
+        Pack<M, 4> u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const R d = *der[q];
+            const R psi = (R)t.v[q] - c[q] * d;
+            u.v[q] = (M)round32(e[q] * (psi - c[q] * d), z);
+            *der[q] = d + psi;
+        }
+        st_pack<4>(rec, 0u, u);
+    } else {
         // the four variables of (cell, axis) k as one record at phi[0] + 4k
         using M = typename P::Mem;
         M* rec = phi[0] + 4 * (size_t)k;
@@ -695,12 +701,6 @@ __device__ __forceinline__ void cpml4(typename P::Mem* const* phi, unsigned k, c
             *der[q] = d + psi;
         }
         st_pack<4>(rec, 0u, u);
-    } else {
-        R m[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) m[q] = P::ldm(phi[q], k);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) *der[q] = cpml_with<P>(phi[q], k, m[q], c[q], e[q], *der[q], z);
     }
 }
 
@@ -1848,7 +1848,7 @@ __global__ void k_rescale(const StepArgs<P> a, unsigned long long ncell, unsigne
     } else {
         const int k = arr - PW_NFIELDS;
         const int ax = k / kMemPerAxis, t = k % kMemPerAxis;
-        const unsigned long long n = ax == 0 ? nmem0 : (ax == 1 ? nmem1 : nmem2);
+        const unsigned long long n = (ax == 0 ? nmem0 : (ax == 1 ? nmem1 : nmem2)) * mem_width<P>();
         typename P::Mem* m = a.m[ax][t];
         if (!m) return;
         for (; i < n; i += stride) P::stm(m, (unsigned)i, P::ldm(m, (unsigned)i) * ratio, a.zero);
@@ -2025,7 +2025,8 @@ __device__ __forceinline__ void rescale_array(const A& a, int arr, unsigned long
         const int ax = q / kMemPerAxis, t = q % kMemPerAxis;
         typename P::Mem* m = a.m[ax][t];
         if (!m) return;
-        for (unsigned long long i = i0; i < nmem[ax]; i += stride) P::stm(m, (unsigned)i, P::ldm(m, (unsigned)i) * ratio, a.zero);
+        const unsigned long long n = nmem[ax] * mem_width<P>();  // the records' values
+        for (unsigned long long i = i0; i < n; i += stride) P::stm(m, (unsigned)i, P::ldm(m, (unsigned)i) * ratio, a.zero);
     }
 }
 
