@@ -1105,7 +1105,7 @@ extern "C" pw_status pw_get_info(pw_solver* s, pw_info* out)
     for (int ax = 0; ax < 3; ++ax) f += (g.lo[ax] + g.hi[ax]) / n3[ax];
     // velocity: reads tau x6 + p, read-modify-writes vu x3 + vw x3 (19 touches);
     // stress: reads vu x3 + vw x3, read-modify-writes tau x6 + p (20 touches).
-    // coefficient arrays: scaled form 15 (velocity) + 7 (stress) FP16 arrays;
+    // coefficients: scaled form 15 (velocity) + 7 (stress) FP16 values;
     // physical form 4 (rho, rhof, C1, C2) + 5 (C, M, H, H-2mu, mu) cell-centred
     // arrays (comp16f, and FP32 on a medium whose H is (H-2mu) + 2mu, form H
     // from H-2mu and mu; FP32 takes a uniform rhof as a scalar; comp16f: two
