@@ -536,8 +536,10 @@ pw_status pw::install_medium(pw_solver* s, const double* const* d_base, int nxi,
                                         "differs from it by " + std::to_string(res[kSlotHMismatch]) + " of H");
     s->b.derive_h = s->prec == PW_FP32 && h_consistent ? 1 : 0;
     // one fluid throughout (as in the reference's model, Float/main.cu:1632):
-    // FP32's velocity kernel takes rhof as a scalar, the same float value
-    s->b.rhof_uniform = s->prec == PW_FP32 && res[kSlotRhofSpread] == 0.0 ? res[4 + PW_RHOF] : 0.0;
+    // the FP32 and FP64 velocity kernels take rhof as a scalar, the same value
+    // the array holds
+    const bool physical = s->prec == PW_FP32 || s->prec == PW_FP64;
+    s->b.rhof_uniform = physical && res[kSlotRhofSpread] == 0.0 ? res[4 + PW_RHOF] : 0.0;
     s->vp_max = std::sqrt(res[2]);
     // von Neumann limit of the staggered leapfrog: vp dt / h * sqrt(3) * sum|c_k| <= 1,
     // sum|c_k| = 1 (2nd order) or 9/8 + 1/24 = 7/6 (4th order)

@@ -200,7 +200,7 @@ struct DevBuffers {
     float sc_stress, sc_vel, sc_vw;  // comp16f: stored value = physical * scale (powers of two)
     int xcd_map;     // 1: XCD-aware block -> tile mapping in the step kernels
     int derive_h;    // FP32: the stress kernel forms H as (H - 2 mu) + 2 mu (consistent medium)
-    double rhof_uniform;  // FP32, > 0: the medium's one rhof, which the velocity kernel takes as a scalar
+    double rhof_uniform;  // FP32, FP64, > 0: the medium's one rhof, which the velocity kernel takes as a scalar
 };
 
 // Step kernels over the planes [zb, ze) (the updated planes are 1 .. nz-2).

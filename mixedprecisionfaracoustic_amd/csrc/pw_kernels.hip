@@ -282,7 +282,7 @@ template <class P> struct StepArgs {
     const f16* rh[kRecords];
     const int8_t* rl[kRecords];
     float rs[kRecords][4];
-    typename P::R rhof;  // the medium's one rhof where c[PC_RHOF] is null (FP32)
+    typename P::R rhof;  // the medium's one rhof where c[PC_RHOF] is null (FP32, FP64)
 };
 
 // kFormFast constants; g = dt/h, s_u, s_w, s_t = storage scales of the solid
@@ -318,7 +318,7 @@ template <class P> __host__ StepArgs<P> make_args(const DevBuffers& b)
     for (int k = 0; k < kMaxCoef; ++k) a.c[k] = (const typename P::Coef*)b.coef[k];
     if (b.derive_h) a.c[PC_HH] = nullptr;  // FP32, consistent medium: the stress kernel forms H
     a.rhof = (R)b.rhof_uniform;
-    if (b.rhof_uniform > 0) a.c[PC_RHOF] = nullptr;  // FP32, one fluid: a scalar
+    if (b.rhof_uniform > 0) a.c[PC_RHOF] = nullptr;  // FP32 and FP64, one fluid: a scalar
     for (int ax = 0; ax < 3; ++ax) {
         for (int k = 0; k < kMemPerAxis; ++k) a.m[ax][k] = (typename P::Mem*)b.mem[ax][k];
         a.tab[ax] = (const Tab4<R>*)b.tab[ax];
@@ -1205,7 +1205,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs<P>& a, const Tile t
             LDC(PC_C2, o + PL, c2_n);
             seg_right(L, c2_c, c2_x, o + N, [&](unsigned q) { return P::ldc(a.c[PC_C2], q); }, [&](R (&r)[N]) { LDC(PC_C2, o + 1, r); });
             LDC(PC_C2, o + W, c2_y);
-            if constexpr (std::is_same<P, PolF32>::value) {
+            if constexpr (std::is_same<P, PolF32>::value || std::is_same<P, PolF64>::value) {
                 if (a.c[PC_RHOF]) {
                     LDC(PC_RHOF, o, rf);
                 } else {
