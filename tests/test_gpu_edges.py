@@ -232,6 +232,14 @@ def test_medium_properties_select_the_coefficient_streams():
     assert s.info()["bytes_velocity"] == vel_arr
     s.close()
     assert streamed - formed == 4.0, (streamed, formed)
+    # FP64 takes one rhof as a scalar too (8 B per cell less) and always reads H
+    d = pw.Solver(*shape, np_, 5.0, 0.5e-3, nt, precision="fp64", cpml_r=1e-4, cpml_v0=4000.0)
+    d.set_medium(base_c)
+    v64, t64 = d.info()["bytes_velocity"], d.info()["bytes_stress"]
+    b_u[rhof] = 1000.0
+    d.set_medium(b_u)
+    assert v64 - d.info()["bytes_velocity"] == 8.0 and d.info()["bytes_stress"] == t64
+    d.close()
     c = pw.Solver(*shape, np_, 5.0, 0.5e-3, nt, precision="comp16f", cpml_r=1e-4, cpml_v0=4000.0)
     with pytest.raises(pw.PorowaveError) as ei:
         c.set_medium(base_i)
